@@ -1,0 +1,190 @@
+#!/usr/bin/env python3
+"""Writes tests/golden/golden.json: inputs and the exact outputs (stdout,
+exit status, strategy file) of the reference L_OpenMP, compiled from the
+reference sources by `make -C oracle ref` (README.md:68 flags) and run in a
+scratch directory.  Run it in a container where /root/reference exists; the
+JSON it writes is the committed fixture.  Nothing at test time runs the
+reference.
+
+Usage: python tests/golden/make_golden.py [--ref-dir /root/reference]
+"""
+from __future__ import annotations
+
+import argparse
+import importlib.util
+import json
+import os
+import subprocess
+import sys
+import tempfile
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+
+
+def _load_gen():
+    path = os.path.join(ROOT, "l-norms_bruteforce_amd", "gen.py")
+    spec = importlib.util.spec_from_file_location("lnorm_gen", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+gen = _load_gen()
+
+
+def mt(rows):
+    return "".join(" ".join(str(v) for v in r) + "\n" for r in rows)
+
+
+def rnd(rows, cols, r, seed):
+    return gen.matrix_text(gen.gen_matrix(rows, cols, r, seed))
+
+
+def dup_variant(rows, cols, r, seed, dup_rows, dup_cols, zero_rows, neg=False):
+    """A base matrix with some rows/columns duplicated (optionally negated or
+    scaled) and some rows zeroed; returns the text."""
+    import numpy as np
+    M = gen.gen_matrix(rows, cols, r, seed).astype(np.int64)
+    for k, src in enumerate(dup_rows):
+        f = (-1 if (neg and k % 2) else 1) * (1 + (k % 2))
+        M = np.vstack([M, f * M[src:src + 1]])
+    for k, src in enumerate(dup_cols):
+        f = (-1 if (neg and k % 2 == 0) else 1)
+        M = np.hstack([M, f * M[:, src:src + 1]])
+    for z in zero_rows:
+        M[z, :] = 0
+    return gen.matrix_text(M)
+
+
+def cases():
+    c = []
+
+    def add(name, text, args, threads=3):
+        c.append({"name": name, "matrix": text, "args": args, "threads": threads})
+
+    # closed forms
+    add("chsh_l1", mt([[1, 1], [1, -1]]), ["1"], 2)
+    add("identity4_l1", mt([[int(i == j) for j in range(4)] for i in range(4)]), ["1"])
+    add("identity5_l2", mt([[int(i == j) for j in range(5)] for i in range(5)]), ["2"])
+    add("identity5_l3", mt([[int(i == j) for j in range(5)] for i in range(5)]), ["3"])
+    add("ones3x5_l1", mt([[1] * 5 for _ in range(3)]), ["1"])
+    add("ones4x4_l2_ties", mt([[1] * 4 for _ in range(4)]), ["2"], 5)
+    u, v = [1, -2, 3], [2, 1, -1, 4]
+    add("rank1_l1", mt([[a * b for b in v] for a in u]), ["1"])
+    add("rank1_l2", mt([[a * b for b in v] for a in u]), ["2"])
+    # zero matrices, single rows/columns, 1x1
+    add("zero3x4_l1", mt([[0] * 4 for _ in range(3)]), ["1"])
+    add("zero3x4_l1_y", mt([[0] * 4 for _ in range(3)]), ["1", "y"])
+    add("zero3x4_l2_y", mt([[0] * 4 for _ in range(3)]), ["2", "y"])
+    add("zero3x4_lm", mt([[0] * 4 for _ in range(3)]), ["1", "m"])
+    add("zero3x4_lm_y", mt([[0] * 4 for _ in range(3)]), ["1", "y", "m"])
+    add("onerow_l1", mt([[3, -1, 4, -1, 5]]), ["1"])
+    add("onecol_l1", mt([[3], [-1], [4], [0], [5]]), ["1"])
+    add("onecol_l2", mt([[3], [-1], [4], [0], [5]]), ["2"])
+    add("one_by_one", mt([[-7]]), ["1"])
+    add("one_by_one_l3", mt([[-7]]), ["3"])
+    # order capped by the number of rows (L_OpenMP.c:346-350)
+    add("cap_l4_3rows", rnd(3, 5, 9, 11), ["4"])
+    add("cap_l10_5rows", rnd(5, 3, 9, 12), ["10"])
+    add("cap_l2_1row", mt([[2, -3, 1]]), ["2"])
+    # transposition for L_1 (functions.h:295-307)
+    add("transpose_l1", rnd(7, 3, 20, 13), ["1"])
+    add("transpose_lm", rnd(7, 4, 20, 14), ["1", "m"])
+    add("transpose_lm_y", dup_variant(6, 4, 9, 15, [2], [], [4]), ["1", "y", "m"])
+    add("no_transpose_l2", rnd(7, 3, 20, 13), ["2"])
+    # preprocessing (functions.h:109-289)
+    base = dup_variant(6, 5, 9, 16, [1, 3, 2], [0, 4], [5], neg=True)
+    for d in ("1", "2", "3"):
+        for flag in ("y", "r", "c", "Y", "x"):
+            add("prep_%s_%s" % (flag, d), base, [d, flag])
+    add("prep_lm_y", base, ["1", "y", "m"])
+    add("prep_lm_r", base, ["1", "r", "M"])
+    add("prep_lm_c", base, ["1", "c", "m"])
+    add("prep_lm_n", base, ["1", "n", "m"])
+    zr = [[0, 0, 0, 0], [0, 2, -1, 3], [0, -1, 4, 1], [0, 5, 0, -2]]
+    add("marg_first_zero_y", mt(zr), ["1", "y", "m"])
+    add("marg_first_zero_m", mt(zr), ["1", "m"])
+    zc = [[0, 1, 2], [0, 3, -1], [0, 0, 0], [0, -2, 5]]
+    add("zero_row_col_y_l1", mt(zc), ["1", "y"])
+    add("zero_row_col_y_l2", mt(zc), ["2", "y"])
+    add("zero_row_col_y_lm", mt(zc), ["1", "y", "m"])
+    sgn = [[1, 2, -1, 3, 0], [2, 1, -2, 1, 1], [1, 0, -1, -4, 2], [3, 1, 0, 2, -1]]
+    add("single_sign_cols_l2_y", mt(sgn), ["2", "y"])
+    add("single_sign_cols_l3_c", mt(sgn), ["3", "c"])
+    add("single_sign_cols_l1_y", mt(sgn), ["1", "y"])
+    prop = [[2, 4, -6], [1, 2, -3], [-3, -6, 9], [1, 0, 1], [5, 1, 2]]
+    add("prop_rows_l1_r", mt(prop), ["1", "r"])
+    add("prop_rows_l2_r", mt(prop), ["2", "r"])
+    add("prop_rows_l3_y", mt(prop), ["3", "y"])
+    # reader: scientific notation, other separators, CRLF, blank lines
+    add("reader_sci", "3.10000000e+01 -2.0e0 7\n1e1,-4,2.5\n", ["1"])
+    add("reader_seps", "1;-2;3\r\n4\t5\t-6\r\n\r\n-7  8 9\r\n", ["1"])
+    add("reader_trailing", "1 2 3\n4 5 6", ["2"])
+    add("reader_decimal_trunc", "1.9 -2.7 3\n4 5.5 -6\n", ["1"])
+    # L_d for several d
+    add("l4_6x5", rnd(6, 5, 9, 17), ["4"])
+    add("l5_7x4", rnd(7, 4, 9, 18), ["5"])
+    add("l6_6x3", rnd(6, 3, 9, 19), ["6"])
+    add("l10_8x3", rnd(8, 3, 5, 20), ["10"], 4)
+    add("l3_9x7", rnd(9, 7, 30, 21), ["3"])
+    add("l2_12x9", rnd(12, 9, 30, 22), ["2"])
+    add("l3_12x5_y", dup_variant(10, 5, 9, 23, [2, 7], [1], [3]), ["3", "y"])
+    # reduced-size versions of each workload
+    add("cfg1_l1_20x20_seed101", rnd(20, 20, 10, 101), ["1"], 8)
+    add("cfg2_lm_20x20", rnd(20, 20, 100, 102), ["1", "m"], 8)
+    add("cfg2_lm_20x20_y_dups", dup_variant(20, 20, 100, 102, [3, 5, 7, 9], [2, 4, 6], [11, 13]),
+        ["1", "y", "m"], 8)
+    add("cfg3_l2_18x32_seed103", rnd(18, 32, 50, 103), ["2"], 8)
+    add("cfg3_l3_18x32_seed103", rnd(18, 32, 50, 103), ["3"], 8)
+    add("cfg4a_l1_64x16_int8", rnd(64, 16, 127, 104), ["1"], 8)
+    add("cfg4b_l1_128x14_int16", rnd(128, 14, 30000, 105), ["1"], 8)
+    add("ns_l1_18x18", rnd(18, 18, 100, 106), ["1"], 8)
+    add("l1_22x40_wide", rnd(22, 40, 100, 107), ["1"], 8)
+    add("lm_16x40", rnd(16, 40, 100, 108), ["1", "m"], 8)
+    # too large after reduction (L_OpenMP.c:374): 42 rows for d=3
+    add("too_many_rows_l3", rnd(42, 3, 5, 109), ["3"])
+    return c
+
+
+def run_ref(binary, case):
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "matrix.txt"), "w") as f:
+            f.write(case["matrix"])
+        cmd = [binary, str(case["threads"]), "matrix.txt"] + case["args"]
+        p = subprocess.run(cmd, cwd=td, capture_output=True, timeout=600)
+        strat = {}
+        for fn in sorted(os.listdir(td)):
+            if fn.startswith("strategy_"):
+                with open(os.path.join(td, fn)) as f:
+                    strat[fn] = f.read()
+        return p.returncode, p.stdout.decode(), strat
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--ref-dir", default="/root/reference")
+    ap.add_argument("--only", default=None)
+    a = ap.parse_args()
+    subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "REF_DIR=" + a.ref_dir],
+                   check=True)
+    binary = os.path.join(ROOT, "oracle", "_ref", "L_OpenMP")
+    out = []
+    for case in cases():
+        if a.only and a.only not in case["name"]:
+            continue
+        rc, stdout, strat = run_ref(binary, case)
+        rec = dict(case)
+        rec.update({"returncode": rc, "stdout": stdout, "strategy": strat})
+        out.append(rec)
+        print("%-32s rc=%d %s" % (case["name"], rc, stdout.strip().splitlines()[-1] if stdout.strip() else ""),
+              flush=True)
+    with open(os.path.join(HERE, "golden.json"), "w") as f:
+        json.dump({"generator": "tests/golden/make_golden.py",
+                   "reference_build": "gcc -O3 L_OpenMP.c -fopenmp -o L_OpenMP -lm",
+                   "cases": out}, f, indent=1)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

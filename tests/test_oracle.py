@@ -1,0 +1,207 @@
+"""Pins the oracle (oracle/lnorm_oracle.c and oracle/host_oracle.py):
+golden vectors of the reference L_OpenMP, an independent brute force,
+closed forms and invariants.  CPU only."""
+import itertools
+
+import numpy as np
+import pytest
+
+import oracle
+from oracle import host_oracle as ho
+
+
+def _run_oracle_cli(case):
+    """The oracle's restatement of `L_OpenMP T file d [p] [m]`: stdout lines,
+    exit status, strategy files."""
+    argv = ["L_OpenMP", str(case["threads"]), "matrix.txt"] + case["args"]
+    threads, _, _, n0, stat, marg = ho.parse_args(argv, "omp")
+    try:
+        p = ho.load(case["matrix"].encode(), n0, stat, marg, "omp", threads)
+    except ho.HostExit as e:
+        return e.lines, 255, {}
+    lines = list(p.lines)
+    if p.R == 0:
+        lines.append("This is a zero matrix.")
+        norm, strat = 0, []
+    else:
+        A = np.array(p.A, dtype=np.int32).reshape(p.R, p.C)
+        norm, strat, _ = oracle.calc_lnorm(A, p.n, p.marginal, threads)
+    line, fname, slines = ho.result_lines(p, norm, strat)
+    lines.append(line)
+    return lines, 0, {fname: "".join(s + "\n" for s in slines)}
+
+
+def test_golden_vectors(golden):
+    assert len(golden) >= 60
+    for case in golden:
+        lines, rc, strat = _run_oracle_cli(case)
+        assert rc == (case["returncode"] & 0xFF), case["name"]
+        assert "".join(l + "\n" for l in lines) == case["stdout"], case["name"]
+        assert strat == case["strategy"], case["name"]
+
+
+def _rand(rng, R, C, r):
+    return rng.integers(-r, r + 1, size=(R, C)).astype(np.int32)
+
+
+@pytest.mark.parametrize("n,marg", [(1, 0), (1, 1), (2, 0), (3, 0), (4, 0), (5, 0)])
+def test_walk_matches_bruteforce(n, marg):
+    rng = np.random.default_rng(1000 + 10 * n + marg)
+    for trial in range(25):
+        R = int(rng.integers(1, 9 if n <= 2 else 6))
+        C = int(rng.integers(1, 7))
+        A = _rand(rng, R, C, int(rng.choice([1, 3, 50])))
+        nb, rb = oracle.brute(A, n, marg)
+        for threads in (1, 3, 8):
+            nw, sw, rw = oracle.calc_lnorm(A, n, marg, threads)
+            assert (nw, rw) == (nb, rb), (n, marg, A.tolist(), threads)
+            assert sw == oracle.labels_of_rank(rb, R, n)
+            assert oracle.objective(A, n, marg, sw) == nb
+
+
+def test_digits_closed_form():
+    # BRGC bit i of N is bit i of N ^ (N >> 1); the reflected ternary code
+    # visits every labeling once and consecutive words differ in one digit by 1.
+    for N in range(300):
+        for i in range(9):
+            assert oracle.digit(N, i, 2) == ((N ^ (N >> 1)) >> i) & 1
+    for n in (3, 4, 5):
+        seen = set()
+        prev = None
+        for N in range(n ** 4):
+            w = tuple(oracle.digit(N, i, n) for i in range(4))
+            seen.add(w)
+            if prev is not None:
+                diff = [abs(a - b) for a, b in zip(w, prev)]
+                assert sorted(diff) == [0, 0, 0, 1]
+            prev = w
+        assert len(seen) == n ** 4
+
+
+def test_closed_forms():
+    assert oracle.calc_lnorm([[1, 1], [1, -1]], 1)[0] == 2                     # CHSH
+    for k in (1, 2, 5, 9):
+        assert oracle.calc_lnorm(np.eye(k, dtype=np.int32), 1)[0] == k        # identity
+    assert oracle.calc_lnorm(np.ones((3, 5), np.int32), 1)[0] == 15           # all-ones
+    u, v = np.array([1, -2, 3, 5]), np.array([2, 1, -1, 4, -3])
+    assert oracle.calc_lnorm(np.outer(u, v).astype(np.int32), 1)[0] == 11 * 11  # rank one
+    rng = np.random.default_rng(7)
+    for R in (2, 3, 4):
+        A = _rand(rng, R, 5, 20)
+        # d >= k: every row gets its own label, L_k = sum |A|
+        assert oracle.calc_lnorm(A, R)[0] == int(np.abs(A).sum())
+    assert oracle.calc_lnorm(np.zeros((4, 3), np.int32), 1)[0] == 0
+    assert oracle.calc_lnorm(np.zeros((4, 3), np.int32), 3)[0] == 0
+
+
+def test_invariants():
+    rng = np.random.default_rng(11)
+    for _ in range(10):
+        R, C = int(rng.integers(2, 8)), int(rng.integers(2, 8))
+        A = _rand(rng, R, C, 30)
+        l1 = oracle.calc_lnorm(A, 1)[0]
+        assert oracle.calc_lnorm(A.T.copy(), 1)[0] == l1                       # transpose
+        P = A[rng.permutation(R)][:, rng.permutation(C)]
+        S = P * rng.choice([-1, 1], size=(R, 1)) * rng.choice([-1, 1], size=(1, C))
+        assert oracle.calc_lnorm(S.astype(np.int32), 1)[0] == l1             # perms, sign flips
+        assert oracle.calc_lnorm((3 * A).astype(np.int32), 1)[0] == 3 * l1   # scaling
+        Z = np.vstack([A, np.zeros((1, C), np.int32)])
+        assert oracle.calc_lnorm(Z, 1)[0] == l1                                # zero row
+        D = np.vstack([A, A[:1]])
+        prev = oracle.calc_lnorm(A, 1)[0]
+        for d in (2, 3, 4):
+            ld = oracle.calc_lnorm(A, d)[0]
+            assert ld >= prev                                                   # monotone in d
+            prev = ld
+            Pc = A[:, rng.permutation(C)] * rng.choice([-1, 1], size=(1, C))
+            assert oracle.calc_lnorm(Pc.astype(np.int32), d)[0] == ld          # column perms/signs
+        assert oracle.calc_lnorm(D, 1)[0] >= l1
+
+
+def test_worker_count_independence():
+    rng = np.random.default_rng(5)
+    A = _rand(rng, 10, 6, 2)        # small entries: many ties
+    ref = oracle.calc_lnorm(A, 1, 0, 1)
+    for t in (2, 3, 7, 64, 511):
+        assert oracle.calc_lnorm(A, 1, 0, t) == ref
+    ref3 = oracle.calc_lnorm(A[:7], 3, 0, 1)
+    for t in (2, 5, 13):
+        assert oracle.calc_lnorm(A[:7], 3, 0, t) == ref3
+
+
+def test_walk_segments_compose():
+    # the max over segments equals the whole range's max, first rank kept
+    rng = np.random.default_rng(3)
+    A = _rand(rng, 9, 5, 4)
+    whole = oracle.walk(A, 2, 0, 0, 255)
+    parts = [oracle.walk(A, 2, 0, a, a + 31) for a in range(0, 256, 32)]
+    best = max(p[0] for p in parts)
+    first = min(p[1] for p in parts if p[0] == best)
+    assert (best, first) == whole[:2]
+
+
+def test_planted_mistakes_are_caught(golden):
+    # Each plausible slip in a restatement must break a golden vector:
+    # dropping the fixed last row, flipping the label convention, or using
+    # |y_0| for the marginal column.
+    cases = {c["name"]: c for c in golden}
+
+    def norm_of(case, mutate):
+        argv = ["x", str(case["threads"]), "m.txt"] + case["args"]
+        threads, _, _, n0, stat, marg = ho.parse_args(argv, "omp")
+        p = ho.load(case["matrix"].encode(), n0, stat, marg, "omp", threads)
+        A = np.array(p.A, dtype=np.int32).reshape(p.R, p.C)
+        return mutate(A, p)
+
+    def drop_last(A, p):
+        return oracle.calc_lnorm(A[:-1], p.n, p.marginal)[0] if p.R > 1 else -1
+
+    def abs_marginal(A, p):
+        return oracle.calc_lnorm(A, p.n, 0)[0]
+
+    def unswapped_marginal(A, p):
+        B = A.copy()
+        B[[0, -1]] = B[[-1, 0]]
+        return oracle.calc_lnorm(B, p.n, p.marginal)[0]
+
+    def want(name):
+        return int(cases[name]["stdout"].strip().splitlines()[-1].split()[-1])
+
+    assert norm_of(cases["cfg1_l1_20x20_seed101"], drop_last) != want("cfg1_l1_20x20_seed101")
+    assert norm_of(cases["lm_16x40"], abs_marginal) != want("lm_16x40")
+    assert norm_of(cases["prep_lm_n"], abs_marginal) != want("prep_lm_n")
+    assert norm_of(cases["cfg2_lm_20x20"], unswapped_marginal) != want("cfg2_lm_20x20")
+    # label convention: the strategy file of an asymmetric case pins 1 <-> +1
+    c = cases["transpose_l1"]
+    lines, _, strat = _run_oracle_cli(c)
+    flipped = {k: "".join("%d\n" % -int(x) for x in v.split()) for k, v in strat.items()}
+    assert strat == c["strategy"] and flipped != c["strategy"]
+
+
+def test_host_preprocessing_preserves_norm():
+    rng = np.random.default_rng(17)
+    for trial in range(30):
+        R, C = int(rng.integers(2, 7)), int(rng.integers(2, 7))
+        M = _rand(rng, R, C, 3)
+        # plant proportional rows/columns and zero lines
+        if R > 2:
+            M[2] = -2 * M[0] if trial % 2 else 3 * M[1]
+        if C > 3:
+            M[:, 3] = M[:, 1] * (-1 if trial % 3 else 2)
+        if trial % 4 == 0:
+            M[R - 1] = 0
+        text = "".join(" ".join(str(v) for v in r) + "\n" for r in M).encode()
+        for d in (1, 2, 3):
+            for marg in ((0, 1) if d == 1 else (0,)):
+                norms = set()
+                for stat in ("n", "y", "r", "c"):
+                    try:
+                        p = ho.load(text, d, stat, marg, "omp", 4)
+                    except ho.HostExit:
+                        continue
+                    if p.R == 0:
+                        norms.add(0)
+                        continue
+                    A = np.array(p.A, dtype=np.int32).reshape(p.R, p.C)
+                    norms.add(oracle.calc_lnorm(A, p.n, p.marginal, 4)[0])
+                assert len(norms) == 1, (M.tolist(), d, marg, norms)
