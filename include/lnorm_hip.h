@@ -1,0 +1,136 @@
+/*
+ * lnorm_hip.h -- C ABI of the MI355X (gfx950) L_d enumeration engine.
+ *
+ * Drop-in for the reference's compute boundary
+ *     void calc_Lnorm(item_calc* second, int* num_of_blocks,
+ *                     int* num_of_threads_per_block)        (L_CUDA.cu:281-359)
+ *     void calc_Lnorm(item_calc* second)                    (L_OpenMP.c:241-297)
+ * which reads second->{n, iRows_reduced, iCols_reduced, marginal, mtx_as_vec}
+ * (functions.h:7-19) and writes second->Lnorm and second->strategy.
+ *
+ * Plain C types only; no HIP or torch types cross this boundary.  All entry
+ * points return 0 on success or a negative LNORM_E_* code; the message of the
+ * last failure on the calling thread is lnorm_hip_last_error().  Unlike the
+ * reference, whose CUDA errors are reported with perror and then ignored
+ * (L_CUDA.cu:292-358, README.md:98), a failure here never yields a norm.
+ */
+#ifndef LNORM_HIP_H
+#define LNORM_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LNORM_HIP_ABI_VERSION 1
+
+/* error codes */
+#define LNORM_OK 0
+#define LNORM_E_ARG (-1)        /* invalid argument (sizes, d, NULL pointers) */
+#define LNORM_E_BOUND (-2)      /* sum |A| > INT32_MAX: the reference's int arithmetic would overflow */
+#define LNORM_E_SIZE (-3)       /* beyond the reference limits (rows > maxRows, cols > 4096, d > 10) */
+#define LNORM_E_HIP (-4)        /* HIP runtime error (no device, launch or copy failure) */
+#define LNORM_E_UNSUPPORTED (-5) /* requested kernel cannot run this problem */
+
+/* kernel selection */
+#define LNORM_KERNEL_AUTO 0     /* blocked when applicable, else plain, else general */
+#define LNORM_KERNEL_PLAIN 1    /* register-resident Gray walk, lane per segment */
+#define LNORM_KERNEL_BLOCKED 2  /* L1/LM: MFMA-built low-row table + scalar high walk */
+#define LNORM_KERNEL_GENERAL 3  /* any width: wave per segment, columns over lanes */
+
+typedef struct lnorm_hip_opts {
+    int32_t kernel;             /* LNORM_KERNEL_* */
+    int32_t want_strategy;      /* nonzero: recover the strategy (smallest maximising Gray rank) */
+    int32_t seg_log;            /* plain/general: segment = base^seg_log ranks per lane; 0 = auto */
+    int32_t patterns_per_lane;  /* blocked: 1, 2, 4 or 8 low patterns per lane; 0 = auto */
+    int32_t wave_log2;          /* blocked: 2^wave_log2 high ranks per wave; 0 = auto */
+    int32_t block;              /* threads per workgroup hint (multiple of 64); 0 = auto */
+    int32_t device;             /* HIP device ordinal; -1 = current device */
+    int32_t reserved0;
+    void *stream;               /* hipStream_t to run on; NULL = the library's own stream */
+} lnorm_hip_opts;
+
+typedef struct lnorm_hip_result {
+    int32_t Lnorm;              /* second->Lnorm */
+    int32_t kernel;             /* LNORM_KERNEL_* actually used */
+    uint64_t total;             /* labelings enumerated: total_num_to_calc (L_OpenMP.c:352-367) */
+    uint64_t best_rank;         /* smallest Gray rank attaining Lnorm (if the strategy was recovered) */
+    double ms_enumerate;        /* HIP-event time of the enumeration phase (table, walk, reduction) */
+    double ms_kernel;           /* HIP-event time of the enumeration kernel launches alone */
+    double ms_strategy;         /* HIP-event time of the strategy recovery */
+    double ms_total;            /* host wall time of the call, copies included */
+    uint64_t units;             /* units (segments or waves) of the enumeration launch */
+    int32_t unit_log;           /* log_base of ranks per lane-segment (plain) or per wave (blocked) */
+    int32_t patterns_per_lane;  /* blocked: P used */
+    char kernel_name[64];       /* instantiation that ran, e.g. "blocked16<np=20,P=4,marg=0>" */
+} lnorm_hip_result;
+
+/*
+ * calc_Lnorm replacement (L_CUDA.cu:281).
+ *   mtx_as_vec : host int32, iRows x iCols row-major, as convert_mtx_to_vec
+ *                leaves it (functions.h:291-329): transposed for d=1 when
+ *                needed, marginal row swapped to the last position.
+ *   n          : effective order second->n (1..10) after calc_Parameters.
+ *   marginal   : second->marginal (only meaningful for n == 1).
+ *   num_of_blocks, num_of_threads_per_block : the CLI's launch hints
+ *                (L_CUDA.cu:361-412).  The result does not depend on them: the
+ *                engine sizes its own launch, and the strategy is always the
+ *                one the reference reports (smallest maximising Gray rank).
+ *   Lnorm      : out, second->Lnorm.
+ *   strategy   : out, caller-allocated iRows-1 ints (second->strategy):
+ *                0/1 for n <= 2 (1 = +1 sign / label 1), labels 0..n-1 for n >= 3.
+ */
+int lnorm_hip_calc_Lnorm(const int32_t *mtx_as_vec, int iRows, int iCols, int n, int marginal,
+                         int num_of_blocks, int num_of_threads_per_block,
+                         int32_t *Lnorm, int32_t *strategy);
+
+/* Same computation with explicit options and timing (opts may be NULL;
+ * strategy may be NULL when opts->want_strategy == 0; res may be NULL). */
+int lnorm_hip_calc_ex(const int32_t *mtx_as_vec, int iRows, int iCols, int n, int marginal,
+                      const lnorm_hip_opts *opts, int32_t *strategy, lnorm_hip_result *res);
+
+/* One problem of a batch (cfg: many matrices of uneven shape and d). */
+typedef struct lnorm_hip_item {
+    const int32_t *mtx_as_vec;
+    int32_t iRows, iCols, n, marginal;
+    int32_t *strategy;          /* iRows-1 ints, or NULL */
+    int32_t Lnorm;              /* out */
+    int32_t status;             /* out: LNORM_OK or LNORM_E_* for this item */
+} lnorm_hip_item;
+
+/* Computes every item; returns LNORM_OK when all items succeeded, else the
+ * first failing item's code (every item's own status is filled in). */
+int lnorm_hip_calc_batch(lnorm_hip_item *items, int count, const lnorm_hip_opts *opts);
+
+/* Fills opts with the defaults (AUTO kernel, strategy on, current device). */
+void lnorm_hip_default_opts(lnorm_hip_opts *opts);
+
+/* Message of the last failure on the calling thread ("" if none). */
+const char *lnorm_hip_last_error(void);
+
+/* LNORM_HIP_ABI_VERSION of the loaded library. */
+int lnorm_hip_abi_version(void);
+
+/* Number of visible HIP devices, or a negative LNORM_E_HIP. */
+int lnorm_hip_device_count(void);
+
+/* Releases the library's device buffers and stream on the current device. */
+void lnorm_hip_release(void);
+
+/* Device self-test of the absolute-difference instructions the kernels are
+ * built on (v_sad_u32, v_sad_u16) against their definitions on edge and
+ * pseudo-random operands.  Returns the number of mismatches (0 = pass) or a
+ * negative LNORM_E_* code. */
+int lnorm_hip_selftest(void);
+
+/* Diagnostic: the packed low-pattern table the blocked kernel would use
+ * (P*(np+marginal)*64 words) for the first kl = 6+log2(P) rows of A.
+ * Returns the number of words written, or a negative LNORM_E_* code. */
+int lnorm_hip_debug_table(const int32_t *mtx_as_vec, int iRows, int iCols, int marginal, int P,
+                          uint32_t *out, int out_words);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LNORM_HIP_H */

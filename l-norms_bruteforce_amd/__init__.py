@@ -1,1 +1,196 @@
+"""lnorm_amd -- MI355X (gfx950) exact L_d matrix-norm engine.
 
+Python view of the C ABI in include/lnorm_hip.h (the drop-in for the
+reference's calc_Lnorm, L_CUDA.cu:281-359) and include/lnorm_host.h (the
+reference host pipeline: reader, preprocessing, layout, printer).  The
+compute path is the HIP library lib/liblnorm_hip.so; there is no CPU
+fallback: loading fails loudly when the library is missing.
+
+This directory is imported under the name `lnorm_amd` (its folder name is not
+a Python identifier); see load() in tests/conftest.py and bench.py.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.abspath(__file__))
+LIB_DIR = os.path.join(PKG_DIR, "lib")
+HIP_LIB = os.path.join(LIB_DIR, "liblnorm_hip.so")
+HOST_LIB = os.path.join(LIB_DIR, "liblnorm_host.so")
+CLI = os.path.join(PKG_DIR, "bin", "L_HIP")
+
+KERNEL_AUTO, KERNEL_PLAIN, KERNEL_BLOCKED, KERNEL_GENERAL = 0, 1, 2, 3
+KERNELS = {"auto": 0, "plain": 1, "blocked": 2, "general": 3}
+ERRORS = {0: "OK", -1: "E_ARG", -2: "E_BOUND", -3: "E_SIZE", -4: "E_HIP", -5: "E_UNSUPPORTED"}
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int32), ("want_strategy", ctypes.c_int32),
+                ("seg_log", ctypes.c_int32), ("patterns_per_lane", ctypes.c_int32),
+                ("wave_log2", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("stream", ctypes.c_void_p)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("Lnorm", ctypes.c_int32), ("kernel", ctypes.c_int32),
+                ("total", ctypes.c_uint64), ("best_rank", ctypes.c_uint64),
+                ("ms_enumerate", ctypes.c_double), ("ms_kernel", ctypes.c_double),
+                ("ms_strategy", ctypes.c_double), ("ms_total", ctypes.c_double),
+                ("units", ctypes.c_uint64), ("unit_log", ctypes.c_int32),
+                ("patterns_per_lane", ctypes.c_int32), ("kernel_name", ctypes.c_char * 64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "kernel_name"}
+        d["kernel_name"] = self.kernel_name.decode()
+        return d
+
+
+class Item(ctypes.Structure):
+    _fields_ = [("mtx_as_vec", ctypes.POINTER(ctypes.c_int32)),
+                ("iRows", ctypes.c_int32), ("iCols", ctypes.c_int32),
+                ("n", ctypes.c_int32), ("marginal", ctypes.c_int32),
+                ("strategy", ctypes.POINTER(ctypes.c_int32)),
+                ("Lnorm", ctypes.c_int32), ("status", ctypes.c_int32)]
+
+
+# symbols declared in include/lnorm_hip.h
+HIP_SYMBOLS = ["lnorm_hip_calc_Lnorm", "lnorm_hip_calc_ex", "lnorm_hip_calc_batch",
+               "lnorm_hip_default_opts", "lnorm_hip_last_error", "lnorm_hip_abi_version",
+               "lnorm_hip_device_count", "lnorm_hip_release", "lnorm_hip_selftest",
+               "lnorm_hip_debug_table"]
+
+_hip = None
+
+
+class LnormError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__("%s (%d): %s" % (ERRORS.get(code, "?"), code, msg))
+        self.code = code
+
+
+def hip_lib():
+    """The HIP engine.  Raises if it was not built (no CPU fallback)."""
+    global _hip
+    if _hip is None:
+        if not os.path.exists(HIP_LIB):
+            raise RuntimeError("HIP engine not built: %s is missing (run __graft_entry__.build())" % HIP_LIB)
+        L = ctypes.CDLL(HIP_LIB)
+        i32p = ctypes.POINTER(ctypes.c_int32)
+        L.lnorm_hip_calc_Lnorm.restype = ctypes.c_int
+        L.lnorm_hip_calc_Lnorm.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.c_int, i32p, i32p]
+        L.lnorm_hip_calc_ex.restype = ctypes.c_int
+        L.lnorm_hip_calc_ex.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                        ctypes.POINTER(Opts), i32p, ctypes.POINTER(Result)]
+        L.lnorm_hip_calc_batch.restype = ctypes.c_int
+        L.lnorm_hip_calc_batch.argtypes = [ctypes.POINTER(Item), ctypes.c_int, ctypes.POINTER(Opts)]
+        L.lnorm_hip_default_opts.restype = None
+        L.lnorm_hip_default_opts.argtypes = [ctypes.POINTER(Opts)]
+        L.lnorm_hip_last_error.restype = ctypes.c_char_p
+        L.lnorm_hip_last_error.argtypes = []
+        L.lnorm_hip_abi_version.restype = ctypes.c_int
+        L.lnorm_hip_device_count.restype = ctypes.c_int
+        L.lnorm_hip_release.restype = None
+        L.lnorm_hip_selftest.restype = ctypes.c_int
+        L.lnorm_hip_debug_table.restype = ctypes.c_int
+        L.lnorm_hip_debug_table.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
+        _hip = L
+    return _hip
+
+
+def last_error() -> str:
+    return hip_lib().lnorm_hip_last_error().decode()
+
+
+def _check(rc):
+    if rc < 0:
+        raise LnormError(rc, last_error())
+    return rc
+
+
+def _i32(A):
+    a = np.ascontiguousarray(np.asarray(A, dtype=np.int32))
+    if a.ndim != 2:
+        raise ValueError("matrix must be 2-D")
+    return a
+
+
+def default_opts(**kw) -> Opts:
+    o = Opts()
+    hip_lib().lnorm_hip_default_opts(ctypes.byref(o))
+    for k, v in kw.items():
+        if k == "kernel" and isinstance(v, str):
+            v = KERNELS[v]
+        setattr(o, k, v)
+    return o
+
+
+def calc_lnorm(A, n: int, marginal: int = 0, want_strategy: bool = True, **opts):
+    """The reference's calc_Lnorm on the GPU for a laid-out matrix
+    (mtx_as_vec, R x C).  Returns (Lnorm, strategy list, result dict)."""
+    a = _i32(A)
+    R, C = a.shape
+    o = default_opts(want_strategy=int(want_strategy), **opts)
+    strat = np.zeros(max(R - 1, 1), dtype=np.int32)
+    res = Result()
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    _check(hip_lib().lnorm_hip_calc_ex(a.ctypes.data_as(i32p), R, C, n, marginal, ctypes.byref(o),
+                                       strat.ctypes.data_as(i32p), ctypes.byref(res)))
+    return int(res.Lnorm), [int(v) for v in strat[: R - 1]] if want_strategy else None, res.as_dict()
+
+
+def calc_lnorm_simple(A, n: int, marginal: int = 0, blocks: int = 512, threads: int = 32):
+    """lnorm_hip_calc_Lnorm: the calc_Lnorm(item_calc*, B, T) signature."""
+    a = _i32(A)
+    R, C = a.shape
+    norm = ctypes.c_int32()
+    strat = np.zeros(max(R - 1, 1), dtype=np.int32)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    _check(hip_lib().lnorm_hip_calc_Lnorm(a.ctypes.data_as(i32p), R, C, n, marginal, blocks, threads,
+                                          ctypes.byref(norm), strat.ctypes.data_as(i32p)))
+    return int(norm.value), [int(v) for v in strat[: R - 1]]
+
+
+def calc_batch(problems, want_strategy=True, **opts):
+    """problems: iterable of (A, n, marginal).  Returns [(Lnorm, strategy, status)]."""
+    mats = [_i32(A) for A, _, _ in problems]
+    strats = [np.zeros(max(m.shape[0] - 1, 1), dtype=np.int32) for m in mats]
+    items = (Item * len(mats))()
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    for it, m, s, (_, n, marg) in zip(items, mats, strats, problems):
+        it.mtx_as_vec = m.ctypes.data_as(i32p)
+        it.iRows, it.iCols = m.shape
+        it.n, it.marginal = n, marg
+        it.strategy = s.ctypes.data_as(i32p) if want_strategy else None
+    o = default_opts(**opts)
+    hip_lib().lnorm_hip_calc_batch(items, len(mats), ctypes.byref(o))
+    out = []
+    for it, m, s in zip(items, mats, strats):
+        out.append((int(it.Lnorm), [int(v) for v in s[: m.shape[0] - 1]] if want_strategy else None,
+                    int(it.status)))
+    return out
+
+
+def selftest() -> int:
+    return _check(hip_lib().lnorm_hip_selftest())
+
+
+def debug_table(A, marginal: int, P: int):
+    a = _i32(A)
+    R, C = a.shape
+    np_ = ((C - marginal) + 1) // 2
+    words = P * (max(np_, 4) + 2 + marginal) * 64
+    out = np.zeros(words, dtype=np.uint32)
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    n = _check(hip_lib().lnorm_hip_debug_table(a.ctypes.data_as(i32p), R, C, marginal, P,
+                                                out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), words))
+    return out[:n]
+
+
+def device_count() -> int:
+    return hip_lib().lnorm_hip_device_count()

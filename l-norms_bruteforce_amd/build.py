@@ -1,0 +1,86 @@
+"""Builds the engine in-tree:
+
+  l-norms_bruteforce_amd/lib/liblnorm_hip.so   HIP kernels + C ABI (include/lnorm_hip.h)
+  l-norms_bruteforce_amd/lib/liblnorm_host.so  reader / preprocessing / layout / printer
+                                                 (include/lnorm_host.h)
+  l-norms_bruteforce_amd/bin/L_HIP             the CLI (L_CUDA.cu's argv), linked to both
+
+hipcc cross-compiles gfx950 code without a GPU.  Every HIP translation unit
+is compiled with packed-fp32 instructions disabled.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(PKG)
+CSRC = os.path.join(PKG, "csrc")
+HOST = os.path.join(PKG, "host")
+LIB = os.path.join(PKG, "lib")
+BIN = os.path.join(PKG, "bin")
+OBJ = os.path.join(PKG, "_build")
+INC = os.path.join(ROOT, "include")
+
+ARCH = os.environ.get("LNORM_OFFLOAD_ARCH", "gfx950")
+HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
+HIP_SOURCES = ["lnorm_plain.hip", "lnorm_blocked.hip", "lnorm_general.hip", "lnorm_api.hip"]
+HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wall",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-I" + CSRC, "-I" + INC]
+HOST_SOURCES = ["lnorm_host.c"]
+C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-Wall", "-Wextra", "-I" + INC]
+
+
+def _run(cmd):
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError("command failed (%d): %s\n%s%s" % (p.returncode, " ".join(cmd), p.stdout, p.stderr))
+    return p
+
+
+def build_hip(jobs: int = 8) -> str:
+    os.makedirs(OBJ, exist_ok=True)
+    os.makedirs(LIB, exist_ok=True)
+    objs = []
+    cmds = []
+    for src in HIP_SOURCES:
+        obj = os.path.join(OBJ, src.replace(".hip", ".o"))
+        objs.append(obj)
+        cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        for f in [ex.submit(_run, c) for c in cmds]:
+            f.result()
+    out = os.path.join(LIB, "liblnorm_hip.so")
+    tmp = out + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", "--offload-arch=" + ARCH, *objs, "-o", tmp])
+    os.replace(tmp, out)
+    return out
+
+
+def build_host() -> tuple:
+    os.makedirs(LIB, exist_ok=True)
+    os.makedirs(BIN, exist_ok=True)
+    lib = os.path.join(LIB, "liblnorm_host.so")
+    srcs = [os.path.join(HOST, s) for s in HOST_SOURCES]
+    _run(["gcc", *C_FLAGS, "-shared", *srcs, "-o", lib + ".tmp", "-lm"])
+    os.replace(lib + ".tmp", lib)
+    exe = os.path.join(BIN, "L_HIP")
+    _run(["gcc", *C_FLAGS, os.path.join(HOST, "L_HIP.c"), *srcs, "-o", exe + ".tmp",
+          "-L" + LIB, "-llnorm_hip", "-Wl,-rpath,$ORIGIN/../lib", "-lm"])
+    os.replace(exe + ".tmp", exe)
+    return lib, exe
+
+
+def build(jobs: int = 8) -> None:
+    build_hip(jobs)
+    if os.path.exists(os.path.join(HOST, "lnorm_host.c")):
+        build_host()
+
+
+if __name__ == "__main__":
+    build(int(sys.argv[1]) if len(sys.argv) > 1 else 8)
+    print("built", LIB, BIN)

@@ -1,0 +1,743 @@
+// lnorm_api.hip -- the C ABI (include/lnorm_hip.h) over the gfx950 kernels.
+//
+// One call = the reference's calc_Lnorm (L_CUDA.cu:281-359): upload the
+// matrix, enumerate, reduce, return the norm and the strategy.  The search
+// runs in two phases:
+//   1. MODE_MAX over all ranks: every unit (a contiguous range of Gray ranks:
+//      a lane's segment for the plain kernels, a wave's range for the blocked
+//      kernel) records its best value; the global maximum is an atomic max.
+//   2. The first unit whose best equals the maximum is re-walked in
+//      MODE_FIRST, which returns the smallest rank attaining it.  That is the
+//      strategy the reference prints: within a worker it keeps the first
+//      strict improvement (L_OpenMP.c:79) and across workers the lowest
+//      index wins (L_OpenMP.c:291-294), and workers own ascending contiguous
+//      segments (calc_jmin_jmax, L_OpenMP.c:6-13).
+// The strategy is the label vector of that rank: BRGC bits for d <= 2
+// (L_OpenMP.c:63), reflected d-ary digits P[(N / d^i) mod 2d] for d >= 3
+// (L_OpenMP.c:140,204; functions.h:350-359).
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/lnorm_hip.h"
+#include "lnorm_internal.h"
+
+using namespace lnorm;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const char *fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  g_err = buf;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                  \
+  do {                                                                                \
+    hipError_t e_ = (expr);                                                           \
+    if (e_ != hipSuccess)                                                             \
+      return fail(LNORM_E_HIP, "%s failed: %s", #expr, hipGetErrorString(e_));        \
+  } while (0)
+
+constexpr int kMaxLabels = 10;      // RANK_OF_NORM, functions.h:3
+constexpr int kMaxCols = 4096;      // length, functions.h:2
+constexpr uint64_t kUnitsPerChunk = 1ull << 20;
+
+struct Buf {
+  void *p = nullptr;
+  size_t cap = 0;
+};
+
+struct DevCtx {
+  std::mutex mu;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  Buf A, D, units, scal, table, dpk, seed, dfull, lo, state;
+  bool init = false;
+};
+
+std::mutex g_ctx_mu;
+DevCtx g_ctx[64];
+
+int ensure(Buf &b, size_t bytes) {
+  if (bytes == 0) bytes = 8;
+  if (b.cap >= bytes) return 0;
+  if (b.p) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.cap = 0;
+  hipError_t e = hipMalloc(&b.p, bytes);
+  if (e != hipSuccess) return fail(LNORM_E_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  b.cap = bytes;
+  return 0;
+}
+
+int ctx_init(DevCtx &c) {
+  if (c.init) return 0;
+  HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
+  for (auto &e : c.ev) HIPCHK(hipEventCreate(&e));
+  c.init = true;
+  return 0;
+}
+
+// ---- problem description ----------------------------------------------------
+struct Problem {
+  const int32_t *A;
+  int R, C, n, marginal;
+  Kind kind;
+  int base;          // 2 for d <= 2, else d
+  uint64_t total;    // total_num_to_calc
+};
+
+uint64_t ipow(uint64_t b, int e) {
+  uint64_t r = 1;
+  for (int i = 0; i < e; ++i) r *= b;
+  return r;
+}
+
+int max_rows(int n) {  // calc_Parameters, L_OpenMP.c:354-366
+  if (n <= 2) return 63;
+  return (int)(std::floor(64.0 / std::log2((double)n)) + 1);
+}
+
+int validate(Problem &p) {
+  if (!p.A) return fail(LNORM_E_ARG, "mtx_as_vec is NULL");
+  if (p.R < 1 || p.C < 1) return fail(LNORM_E_ARG, "matrix must be at least 1 x 1 (got %d x %d)", p.R, p.C);
+  if (p.n < 1 || p.n > kMaxLabels) return fail(LNORM_E_SIZE, "order %d outside 1..%d", p.n, kMaxLabels);
+  if (p.marginal != 0 && p.marginal != 1) return fail(LNORM_E_ARG, "marginal must be 0 or 1");
+  if (p.marginal && p.n != 1) return fail(LNORM_E_ARG, "marginals are defined for n = 1 only");
+  if (p.R > max_rows(p.n))
+    return fail(LNORM_E_SIZE, "%d rows exceed maxRows %d for n = %d", p.R, max_rows(p.n), p.n);
+  if (p.C > kMaxCols) return fail(LNORM_E_SIZE, "%d columns exceed length %d", p.C, kMaxCols);
+  int64_t s = 0;
+  for (size_t i = 0; i < (size_t)p.R * p.C; ++i) s += p.A[i] < 0 ? -(int64_t)p.A[i] : (int64_t)p.A[i];
+  if (s > INT32_MAX)
+    return fail(LNORM_E_BOUND, "sum |A| = %lld exceeds INT32_MAX; the reference's int arithmetic would overflow",
+                (long long)s);
+  p.kind = p.n == 1 ? (p.marginal ? KIND_LM : KIND_L1) : (p.n == 2 ? KIND_L2 : KIND_LD);
+  p.base = p.n <= 2 ? 2 : p.n;
+  if (p.n <= 2) {
+    p.total = 1ull << (p.R - 1);
+  } else {
+    // the reference counts with pow() (L_OpenMP.c:361,365); exact below 2^53
+    const double dt = std::pow((double)p.n, (double)(p.R - 1));
+    if (dt >= 9007199254740992.0)
+      return fail(LNORM_E_SIZE, "%d^%d labelings exceed 2^53, where the reference's pow() count is inexact",
+                  p.n, p.R - 1);
+    p.total = ipow((uint64_t)p.n, p.R - 1);
+  }
+  return 0;
+}
+
+void labels_of_rank(uint64_t N, int R, int n, int32_t *out) {
+  for (int i = 0; i < R - 1; ++i) {
+    if (n <= 2) {
+      out[i] = (int32_t)(((N ^ (N >> 1)) >> i) & 1);
+    } else {
+      uint64_t x = N;
+      for (int k = 0; k < i; ++k) x /= (uint64_t)n;
+      const uint64_t r = x % (uint64_t)(2 * n);
+      out[i] = (int32_t)(r < (uint64_t)n ? r : 2 * n - 1 - r);
+    }
+  }
+}
+
+// ---- plain / general launch geometry --------------------------------------------
+struct Geometry {
+  int q;           // segment = base^q ranks
+  uint64_t seglen;
+  uint64_t nseg;   // segments covering the range
+};
+
+// Segments for a range of base^e ranks: aim at about 2^target_log lanes.
+Geometry geometry_for(int base, int e, int target_log, int forced, bool general) {
+  Geometry g;
+  const double lb = std::log2((double)base);
+  int qmax;
+  if (general) qmax = (int)(62.0 / lb);
+  else if (base == 2) qmax = 30;
+  else qmax = std::min(15, (int)(31.0 / lb));
+  int q = forced > 0 ? forced : std::max(0, e - (int)std::floor(target_log / lb));
+  q = std::min(q, std::min(qmax, e));
+  g.q = q;
+  g.seglen = ipow((uint64_t)base, q);
+  g.nseg = ipow((uint64_t)base, e - q);
+  return g;
+}
+
+struct Prepared {
+  int ny_pad = 0;      // plain width (0 = use the general kernel)
+  std::vector<int32_t> A, D;
+};
+
+void prepare_plain(const Problem &p, int ny_pad, Prepared &pp) {
+  pp.ny_pad = ny_pad;
+  const int W = ny_pad ? ny_pad : p.C;
+  pp.A.assign((size_t)p.R * W, 0);
+  for (int i = 0; i < p.R; ++i)
+    for (int l = 0; l < p.C; ++l) pp.A[(size_t)i * W + l] = p.A[(size_t)i * p.C + l];
+  pp.D.clear();
+  if (ny_pad && p.n <= 2 && p.R > 1) {
+    // D[1][j] = +f A[j], D[0][j] = -f A[j]; f = 2 for L1/LM (y += 2A), 1 for L2
+    const uint32_t f = p.n == 1 ? 2u : 1u;
+    pp.D.assign((size_t)2 * (p.R - 1) * W, 0);
+    for (int j = 0; j < p.R - 1; ++j)
+      for (int l = 0; l < W; ++l) {
+        const uint32_t a = (uint32_t)pp.A[(size_t)j * W + l] * f;
+        pp.D[((size_t)(p.R - 1) + j) * W + l] = (int32_t)a;
+        pp.D[(size_t)j * W + l] = (int32_t)(0u - a);
+      }
+  }
+}
+
+// Runs MODE_MAX over [0, total) with the plain or general kernel, chunked.
+// Returns the maximum and the first unit attaining it.
+struct Phase1Out {
+  int32_t best = INT32_MIN;
+  uint64_t first_unit = 0;
+  uint64_t units = 0;
+  double ms_kernel = 0;   // HIP-event time of the enumeration kernel launches alone
+};
+
+int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometry &g, hipStream_t st,
+                 int block, Phase1Out &out) {
+  const uint64_t chunk = kUnitsPerChunk;
+  out.units = g.nseg;
+  out.best = INT32_MIN;
+  out.first_unit = ~0ull;
+  int32_t *ub = (int32_t *)cx.units.p;
+  int32_t *gbest = (int32_t *)cx.scal.p;
+  unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
+  for (uint64_t c0 = 0; c0 < g.nseg; c0 += chunk) {
+    const uint64_t n = std::min(chunk, g.nseg - c0);
+    HIPCHK(launch_fill_i32(gbest, INT32_MIN, 1, st));
+    HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
+    PlainArgs a{};
+    a.A = (const int32_t *)cx.A.p;
+    a.D = (const int32_t *)cx.D.p;
+    a.R = p.R;
+    a.ny_pad = pp.ny_pad;
+    a.d = p.n;
+    a.q = g.q;
+    a.seg0 = c0;
+    a.nseg = n;
+    a.unit_best = ub;
+    a.gbest = gbest;
+    a.gfirst = gfirst;
+    a.state = (uint32_t *)cx.state.p;
+    HIPCHK(hipEventRecord(cx.ev[4], st));
+    hipError_t e = pp.ny_pad ? launch_plain(p.kind, MODE_MAX, a, st, block)
+                             : launch_general(p.kind, MODE_MAX, a, p.C, st);
+    HIPCHK(hipEventRecord(cx.ev[5], st));
+    if (e != hipSuccess) return fail(LNORM_E_HIP, "enumeration launch failed: %s", hipGetErrorString(e));
+    HIPCHK(launch_first_unit(ub, n, gbest, gfirst, st));
+    int32_t hb;
+    unsigned long long hf;
+    HIPCHK(hipMemcpyAsync(&hb, gbest, sizeof hb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hf, gfirst, sizeof hf, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float kms = 0;
+    HIPCHK(hipEventElapsedTime(&kms, cx.ev[4], cx.ev[5]));
+    out.ms_kernel += kms;
+    if (hb > out.best || out.first_unit == ~0ull) {
+      out.best = hb;
+      out.first_unit = c0 + hf;
+    }
+  }
+  return 0;
+}
+
+// MODE_FIRST over the rank range [u * base^e, (u+1) * base^e).
+int run_walk_first(DevCtx &cx, const Problem &p, const Prepared &pp, uint64_t u, int e, int32_t target,
+                   hipStream_t st, int block, uint64_t &rank) {
+  const Geometry g = geometry_for(p.base, e, 16, 0, pp.ny_pad == 0);
+  unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
+  HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
+  PlainArgs a{};
+  a.A = (const int32_t *)cx.A.p;
+  a.D = (const int32_t *)cx.D.p;
+  a.R = p.R;
+  a.ny_pad = pp.ny_pad;
+  a.d = p.n;
+  a.q = g.q;
+  a.seg0 = u * g.nseg;
+  a.nseg = g.nseg;
+  a.target = target;
+  a.unit_best = (int32_t *)cx.units.p;
+  a.gbest = (int32_t *)cx.scal.p;
+  a.gfirst = gfirst;
+  a.state = (uint32_t *)cx.state.p;
+  hipError_t err = pp.ny_pad ? launch_plain(p.kind, MODE_FIRST, a, st, block)
+                             : launch_general(p.kind, MODE_FIRST, a, p.C, st);
+  if (err != hipSuccess) return fail(LNORM_E_HIP, "strategy launch failed: %s", hipGetErrorString(err));
+  unsigned long long hf;
+  HIPCHK(hipMemcpyAsync(&hf, gfirst, sizeof hf, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  if (hf == ~0ull) return fail(LNORM_E_HIP, "strategy recovery found no rank attaining %d", target);
+  rank = hf;
+  return 0;
+}
+
+// ---- blocked L1/LM -----------------------------------------------------------
+struct BlockedPlan {
+  int P = 0, kl = 0, nh = 0, np = 0, NW = 0, s = 0, planes = 1;
+  uint64_t nwaves = 0;
+};
+
+// Abs columns of the blocked layout: L1 all columns, LM columns 1..C-1.
+bool blocked_plan(const Problem &p, const lnorm_hip_opts &o, BlockedPlan &bp, std::string &why) {
+  if (p.kind != KIND_L1 && p.kind != KIND_LM) { why = "blocked kernel handles n = 1 only"; return false; }
+  const int marg = p.kind == KIND_LM ? 1 : 0;
+  const int nabs = p.C - marg;
+  int np = (nabs + 1) / 2;
+  // 16-bit halves: every |.| column's absolute sum must stay below 2^15
+  for (int l = marg; l < p.C; ++l) {
+    int64_t cs = 0;
+    for (int i = 0; i < p.R; ++i) cs += std::llabs((long long)p.A[(size_t)i * p.C + l]);
+    if (cs > 32767) { why = "a column's absolute sum exceeds 32767 (16-bit halves)"; return false; }
+  }
+  int P = o.patterns_per_lane;
+  if (P == 0) {
+    P = 4;
+    int npp = blocked_np_pad(np, P, marg);
+    if (!npp || P * (npp + marg) > 132) P = 2;
+  }
+  const int npp = blocked_np_pad(np, P, marg);
+  if (!npp) { why = "no blocked instantiation for this width / P"; return false; }
+  int kl = 6;
+  for (int t = P; t > 1; t >>= 1) ++kl;
+  if (p.R - 1 < kl + 1) { why = "too few rows for the blocked split"; return false; }
+  if (marg) {
+    for (int i = 0; i < kl; ++i)
+      if (std::llabs((long long)p.A[(size_t)i * p.C]) > 32767) { why = "marginal column too large"; return false; }
+  }
+  bool int8ok = true;
+  for (int i = 0; i < kl; ++i)
+    for (int l = 0; l < p.C; ++l) {
+      const int32_t v = p.A[(size_t)i * p.C + l];
+      if (v < -128 || v > 127) int8ok = false;
+    }
+  bp.P = P;
+  bp.kl = kl;
+  bp.nh = p.R - 1 - kl;
+  bp.np = npp;
+  bp.NW = npp + marg;
+  bp.planes = int8ok ? 1 : 2;
+  int s = o.wave_log2 > 0 ? o.wave_log2 : std::max(0, bp.nh - 15);
+  s = std::min(std::min(s, bp.nh), 30);
+  bp.s = s;
+  bp.nwaves = 1ull << (bp.nh - s);
+  return true;
+}
+
+int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t st, Phase1Out &out) {
+  const int marg = p.kind == KIND_LM ? 1 : 0;
+  const int NW = bp.NW, kl = bp.kl, nh = bp.nh;
+  // column of packed half h (0 = low, 1 = high) of word c; -1 = zero padding
+  auto colof = [&](int c, int h) -> int {
+    const int l = marg + 2 * c + h;
+    return l < p.C ? l : -1;
+  };
+  auto Aat = [&](int i, int l) -> int64_t { return l < 0 ? 0 : (int64_t)p.A[(size_t)i * p.C + l]; };
+  // low-row planes for the MFMA table
+  std::vector<int8_t> lo((size_t)bp.planes * kl * p.C);
+  for (int i = 0; i < kl; ++i)
+    for (int l = 0; l < p.C; ++l) {
+      const int32_t v = p.A[(size_t)i * p.C + l];
+      if (bp.planes == 1) {
+        lo[(size_t)i * p.C + l] = (int8_t)v;
+      } else {
+        const int32_t lo8 = ((v + 128) & 255) - 128;
+        const int32_t hi8 = (v - lo8) / 256;
+        lo[(size_t)i * p.C + l] = (int8_t)hi8;
+        lo[((size_t)kl + i) * p.C + l] = (int8_t)lo8;
+      }
+    }
+  // scalar state: all high rows at -1, and the flip deltas
+  std::vector<uint32_t> seed(NW), dfull((size_t)std::max(nh, 1) * NW), dpk((size_t)2 * std::max(nh, 1) * NW);
+  auto yminus = [&](int l) -> int64_t {
+    if (l < 0) return 0;
+    int64_t y = Aat(p.R - 1, l);
+    for (int m = 0; m < nh; ++m) y -= Aat(kl + m, l);
+    return y;
+  };
+  for (int c = 0; c < bp.np; ++c) {
+    const int64_t ylo = yminus(colof(c, 0)), yhi = yminus(colof(c, 1));
+    seed[c] = ((uint32_t)(32768 - yhi) << 16) | ((uint32_t)(32768 - ylo) & 0xFFFFu);
+    for (int m = 0; m < nh; ++m) {
+      const int64_t dlo = 2 * Aat(kl + m, colof(c, 0)), dhi = 2 * Aat(kl + m, colof(c, 1));
+      const uint32_t w = (uint32_t)(int64_t)(-(dhi * 65536 + dlo));
+      dfull[(size_t)m * NW + c] = w;
+    }
+  }
+  if (marg) {
+    seed[bp.np] = (uint32_t)(int32_t)yminus(0);
+    for (int m = 0; m < nh; ++m) dfull[(size_t)m * NW + bp.np] = (uint32_t)(2 * Aat(kl + m, 0));
+  }
+  for (int j = 0; j < nh; ++j)
+    for (int c = 0; c < NW; ++c) {
+      dpk[((size_t)nh + j) * NW + c] = dfull[(size_t)j * NW + c];
+      dpk[(size_t)j * NW + c] = 0u - dfull[(size_t)j * NW + c];
+    }
+  if (int r = ensure(cx.lo, lo.size())) return r;
+  if (int r = ensure(cx.table, (size_t)bp.P * NW * 64 * 4)) return r;
+  if (int r = ensure(cx.seed, seed.size() * 4)) return r;
+  if (int r = ensure(cx.dfull, dfull.size() * 4)) return r;
+  if (int r = ensure(cx.dpk, dpk.size() * 4)) return r;
+  HIPCHK(hipMemcpyAsync(cx.lo.p, lo.data(), lo.size(), hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(cx.seed.p, seed.data(), seed.size() * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(cx.dfull.p, dfull.data(), dfull.size() * 4, hipMemcpyHostToDevice, st));
+  HIPCHK(hipMemcpyAsync(cx.dpk.p, dpk.data(), dpk.size() * 4, hipMemcpyHostToDevice, st));
+  TableArgs ta{};
+  ta.lo = (const int8_t *)cx.lo.p;
+  ta.planes = bp.planes;
+  ta.kl = kl;
+  ta.ny = p.C;
+  ta.marginal = marg;
+  ta.np = bp.np;
+  ta.P = bp.P;
+  ta.out = (uint32_t *)cx.table.p;
+  HIPCHK(launch_table(ta, st));
+
+  int32_t *ub = (int32_t *)cx.units.p;
+  int32_t *gbest = (int32_t *)cx.scal.p;
+  unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
+  out.units = bp.nwaves;
+  out.best = INT32_MIN;
+  out.first_unit = ~0ull;
+  for (uint64_t c0 = 0; c0 < bp.nwaves; c0 += kUnitsPerChunk) {
+    const uint64_t n = std::min(kUnitsPerChunk, bp.nwaves - c0);
+    HIPCHK(launch_fill_i32(gbest, INT32_MIN, 1, st));
+    HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
+    BlockedArgs a{};
+    a.table = (const uint32_t *)cx.table.p;
+    a.dpk = (const uint32_t *)cx.dpk.p;
+    a.seed = (const uint32_t *)cx.seed.p;
+    a.dfull = (const uint32_t *)cx.dfull.p;
+    a.nh = nh;
+    a.s = bp.s;
+    a.wave0 = c0;
+    a.nwaves = n;
+    a.unit_best = ub;
+    a.gbest = gbest;
+    HIPCHK(hipEventRecord(cx.ev[4], st));
+    hipError_t e = launch_blocked(bp.np, bp.P, marg, a, st);
+    HIPCHK(hipEventRecord(cx.ev[5], st));
+    if (e != hipSuccess) return fail(LNORM_E_HIP, "blocked launch failed: %s", hipGetErrorString(e));
+    HIPCHK(launch_first_unit(ub, n, gbest, gfirst, st));
+    int32_t hb;
+    unsigned long long hf;
+    HIPCHK(hipMemcpyAsync(&hb, gbest, sizeof hb, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(&hf, gfirst, sizeof hf, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float kms = 0;
+    HIPCHK(hipEventElapsedTime(&kms, cx.ev[4], cx.ev[5]));
+    out.ms_kernel += kms;
+    if (hb > out.best || out.first_unit == ~0ull) {
+      out.best = hb;
+      out.first_unit = c0 + hf;
+    }
+  }
+  return 0;
+}
+
+int select_device(const lnorm_hip_opts &o, int &dev) {
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count < 1)
+    return fail(LNORM_E_HIP, "no HIP device available (%s)", e == hipSuccess ? "0 devices" : hipGetErrorString(e));
+  if (o.device >= 0) {
+    if (o.device >= count || o.device >= 64) return fail(LNORM_E_ARG, "device %d out of range", o.device);
+    HIPCHK(hipSetDevice(o.device));
+    dev = o.device;
+  } else {
+    HIPCHK(hipGetDevice(&dev));
+  }
+  return 0;
+}
+
+int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *optsp,
+              int32_t *strategy, lnorm_hip_result *res) {
+  const auto t_start = std::chrono::steady_clock::now();
+  lnorm_hip_opts o;
+  lnorm_hip_default_opts(&o);
+  if (optsp) o = *optsp;
+  g_err.clear();
+  Problem p{A, R, C, n, marginal, KIND_L1, 2, 0};
+  if (int r = validate(p)) return r;
+  if (o.want_strategy && R > 1 && !strategy) return fail(LNORM_E_ARG, "strategy output is NULL");
+  int dev = 0;
+  if (int r = select_device(o, dev)) return r;
+  DevCtx &cx = g_ctx[dev];
+  std::lock_guard<std::mutex> lock(cx.mu);
+  if (int r = ctx_init(cx)) return r;
+  hipStream_t st = o.stream ? (hipStream_t)o.stream : cx.stream;
+
+  lnorm_hip_result rr;
+  std::memset(&rr, 0, sizeof rr);
+  rr.total = p.total;
+
+  // choose the kernel
+  BlockedPlan bp;
+  std::string why;
+  int kernel = o.kernel;
+  const bool can_block = (kernel == LNORM_KERNEL_AUTO || kernel == LNORM_KERNEL_BLOCKED) && blocked_plan(p, o, bp, why);
+  const int ny_plain = plain_ny_pad(p.kind, p.n, p.C);
+  if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
+    return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
+  if (kernel == LNORM_KERNEL_PLAIN && !ny_plain)
+    return fail(LNORM_E_UNSUPPORTED, "no plain instantiation for %d columns at n = %d", p.C, p.n);
+  if (kernel == LNORM_KERNEL_AUTO) kernel = can_block ? LNORM_KERNEL_BLOCKED : (ny_plain ? LNORM_KERNEL_PLAIN : LNORM_KERNEL_GENERAL);
+  if (kernel != LNORM_KERNEL_PLAIN && kernel != LNORM_KERNEL_BLOCKED && kernel != LNORM_KERNEL_GENERAL)
+    return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
+
+  // the plain (or general) walk is needed for the enumeration or the strategy
+  Prepared pp;
+  prepare_plain(p, kernel == LNORM_KERNEL_GENERAL ? 0 : ny_plain, pp);
+  const bool general = pp.ny_pad == 0;
+  const int nvec = (p.kind == KIND_L1 || p.kind == KIND_LM) ? 1 : p.n;
+  if (int r = ensure(cx.A, pp.A.size() * 4)) return r;
+  if (int r = ensure(cx.D, pp.D.size() * 4)) return r;
+  if (int r = ensure(cx.units, kUnitsPerChunk * 4)) return r;
+  if (int r = ensure(cx.scal, 256)) return r;
+  if (general)
+    if (int r = ensure(cx.state, (size_t)kGeneralSlices * nvec * p.C * 4)) return r;
+  HIPCHK(hipMemcpyAsync(cx.A.p, pp.A.data(), pp.A.size() * 4, hipMemcpyHostToDevice, st));
+  if (!pp.D.empty()) HIPCHK(hipMemcpyAsync(cx.D.p, pp.D.data(), pp.D.size() * 4, hipMemcpyHostToDevice, st));
+
+  Phase1Out p1;
+  int unit_e;   // log_base of the ranks per unit
+  HIPCHK(hipEventRecord(cx.ev[0], st));
+  if (kernel == LNORM_KERNEL_BLOCKED) {
+    if (int r = run_blocked(cx, p, bp, st, p1)) return r;
+    unit_e = bp.s + bp.kl;
+    rr.patterns_per_lane = bp.P;
+    snprintf(rr.kernel_name, sizeof rr.kernel_name, "blocked16<np=%d,P=%d,marg=%d,s=%d>", bp.np, bp.P,
+             p.kind == KIND_LM, bp.s);
+  } else {
+    const Geometry g = geometry_for(p.base, p.R - 1, 20, o.seg_log, general);
+    if (int r = run_walk_max(cx, p, pp, g, st, o.block, p1)) return r;
+    unit_e = g.q;
+    if (general)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "general<kind=%d,d=%d,q=%d>", (int)p.kind, p.n, g.q);
+    else
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain<kind=%d,d=%d,ny=%d,q=%d>", (int)p.kind, p.n,
+               pp.ny_pad, g.q);
+  }
+  HIPCHK(hipEventRecord(cx.ev[1], st));
+  rr.units = p1.units;
+  rr.unit_log = unit_e;
+  rr.kernel = kernel;
+  rr.Lnorm = p1.best;
+
+  if (o.want_strategy && R > 1) {
+    HIPCHK(hipEventRecord(cx.ev[2], st));
+    uint64_t rank = 0;
+    if (int r = run_walk_first(cx, p, pp, p1.first_unit, unit_e, p1.best, st, o.block, rank)) return r;
+    HIPCHK(hipEventRecord(cx.ev[3], st));
+    HIPCHK(hipEventSynchronize(cx.ev[3]));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, cx.ev[2], cx.ev[3]));
+    rr.ms_strategy = ms;
+    rr.best_rank = rank;
+    labels_of_rank(rank, R, n, strategy);
+  } else {
+    rr.best_rank = ~0ull;
+  }
+  HIPCHK(hipEventSynchronize(cx.ev[1]));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, cx.ev[0], cx.ev[1]));
+  rr.ms_enumerate = ms;
+  rr.ms_kernel = p1.ms_kernel;
+  rr.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  if (res) *res = rr;
+  return LNORM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+void lnorm_hip_default_opts(lnorm_hip_opts *o) {
+  if (!o) return;
+  std::memset(o, 0, sizeof *o);
+  o->kernel = LNORM_KERNEL_AUTO;
+  o->want_strategy = 1;
+  o->device = -1;
+}
+
+int lnorm_hip_calc_ex(const int32_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *opts,
+                      int32_t *strategy, lnorm_hip_result *res) {
+  return calc_impl(A, R, C, n, marginal, opts, strategy, res);
+}
+
+int lnorm_hip_calc_Lnorm(const int32_t *A, int R, int C, int n, int marginal, int num_of_blocks,
+                         int num_of_threads_per_block, int32_t *Lnorm, int32_t *strategy) {
+  (void)num_of_blocks;
+  if (!Lnorm) return fail(LNORM_E_ARG, "Lnorm output is NULL");
+  lnorm_hip_opts o;
+  lnorm_hip_default_opts(&o);
+  if (num_of_threads_per_block > 0 && num_of_threads_per_block % 64 == 0 && num_of_threads_per_block <= 1024)
+    o.block = num_of_threads_per_block;
+  lnorm_hip_result r;
+  const int rc = calc_impl(A, R, C, n, marginal, &o, strategy, &r);
+  if (rc == 0) *Lnorm = r.Lnorm;
+  return rc;
+}
+
+int lnorm_hip_calc_batch(lnorm_hip_item *items, int count, const lnorm_hip_opts *opts) {
+  if (count < 0 || (count > 0 && !items)) return fail(LNORM_E_ARG, "bad batch");
+  int first_err = 0;
+  for (int i = 0; i < count; ++i) {
+    lnorm_hip_item &it = items[i];
+    lnorm_hip_opts o;
+    lnorm_hip_default_opts(&o);
+    if (opts) o = *opts;
+    o.want_strategy = it.strategy != nullptr;
+    lnorm_hip_result r;
+    it.status = calc_impl(it.mtx_as_vec, it.iRows, it.iCols, it.n, it.marginal, &o, it.strategy, &r);
+    it.Lnorm = it.status == 0 ? r.Lnorm : 0;
+    if (it.status && !first_err) first_err = it.status;
+  }
+  return first_err;
+}
+
+const char *lnorm_hip_last_error(void) { return g_err.c_str(); }
+
+int lnorm_hip_abi_version(void) { return LNORM_HIP_ABI_VERSION; }
+
+int lnorm_hip_device_count(void) {
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) return fail(LNORM_E_HIP, "hipGetDeviceCount: %s", hipGetErrorString(e));
+  return c;
+}
+
+void lnorm_hip_release(void) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
+  DevCtx &c = g_ctx[dev];
+  std::lock_guard<std::mutex> lock(c.mu);
+  for (Buf *b : {&c.A, &c.D, &c.units, &c.scal, &c.table, &c.dpk, &c.seed, &c.dfull, &c.lo, &c.state}) {
+    if (b->p) (void)hipFree(b->p);
+    b->p = nullptr;
+    b->cap = 0;
+  }
+  if (c.init) {
+    for (auto &e : c.ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c.stream);
+    c.init = false;
+  }
+}
+
+int lnorm_hip_selftest(void) {
+  g_err.clear();
+  lnorm_hip_opts o;
+  lnorm_hip_default_opts(&o);
+  int dev = 0;
+  if (int r = select_device(o, dev)) return r;
+  DevCtx &cx = g_ctx[dev];
+  std::lock_guard<std::mutex> lock(cx.mu);
+  if (int r = ctx_init(cx)) return r;
+  // edge values of both halves plus a SplitMix64 stream
+  const uint32_t edge[] = {0u, 1u, 2u, 0x7FFFu, 0x8000u, 0x8001u, 0xFFFFu, 0x10000u, 0x7FFF8000u,
+                           0x80008000u, 0x7FFFFFFFu, 0x80000000u, 0x80000001u, 0xFFFFFFFFu, 0x12345678u};
+  const int ne = (int)(sizeof edge / sizeof edge[0]);
+  const int n = ne * ne + 4096;
+  std::vector<uint32_t> a(n), b(n), c(n), out(2 * (size_t)n);
+  uint64_t x = 0x243F6A8885A308D3ull;
+  auto next = [&]() {
+    x += 0x9E3779B97F4A7C15ull;
+    uint64_t z = x;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return (uint32_t)(z ^ (z >> 31));
+  };
+  for (int i = 0; i < n; ++i) {
+    if (i < ne * ne) {
+      a[i] = edge[i / ne];
+      b[i] = edge[i % ne];
+      c[i] = (i & 1) ? 0u : 0x00100000u;
+    } else {
+      a[i] = next();
+      b[i] = next();
+      c[i] = next() >> 8;
+    }
+  }
+  Buf bufs;
+  if (int r = ensure(bufs, (size_t)5 * n * 4)) return r;
+  uint32_t *da = (uint32_t *)bufs.p, *db = da + n, *dc = db + n, *dout = dc + n;
+  HIPCHK(hipMemcpyAsync(da, a.data(), (size_t)n * 4, hipMemcpyHostToDevice, cx.stream));
+  HIPCHK(hipMemcpyAsync(db, b.data(), (size_t)n * 4, hipMemcpyHostToDevice, cx.stream));
+  HIPCHK(hipMemcpyAsync(dc, c.data(), (size_t)n * 4, hipMemcpyHostToDevice, cx.stream));
+  HIPCHK(launch_sad_probe(da, db, dc, dout, n, cx.stream));
+  HIPCHK(hipMemcpyAsync(out.data(), dout, (size_t)2 * n * 4, hipMemcpyDeviceToHost, cx.stream));
+  HIPCHK(hipStreamSynchronize(cx.stream));
+  (void)hipFree(bufs.p);
+  int bad = 0;
+  for (int i = 0; i < n; ++i) {
+    const uint32_t e32 = (a[i] > b[i] ? a[i] - b[i] : b[i] - a[i]) + c[i];
+    const uint32_t al = a[i] & 0xFFFF, ah = a[i] >> 16, bl = b[i] & 0xFFFF, bh = b[i] >> 16;
+    const uint32_t e16 = (al > bl ? al - bl : bl - al) + (ah > bh ? ah - bh : bh - ah) + c[i];
+    if (out[2 * (size_t)i] != e32 || out[2 * (size_t)i + 1] != e16) {
+      if (!bad)
+        fail(LNORM_E_HIP, "sad mismatch at a=%08x b=%08x c=%08x: sad32 %08x (want %08x), sad16 %08x (want %08x)",
+             a[i], b[i], c[i], out[2 * (size_t)i], e32, out[2 * (size_t)i + 1], e16);
+      ++bad;
+    }
+  }
+  return bad;
+}
+
+int lnorm_hip_debug_table(const int32_t *A, int R, int C, int marginal, int P, uint32_t *out, int out_words) {
+  g_err.clear();
+  Problem p{A, R, C, 1, marginal, KIND_L1, 2, 0};
+  if (int r = validate(p)) return r;
+  lnorm_hip_opts o;
+  lnorm_hip_default_opts(&o);
+  o.patterns_per_lane = P;
+  BlockedPlan bp;
+  std::string why;
+  if (!blocked_plan(p, o, bp, why)) return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
+  const int words = bp.P * bp.NW * 64;
+  if (!out || out_words < words) return fail(LNORM_E_ARG, "output needs %d words", words);
+  int dev = 0;
+  if (int r = select_device(o, dev)) return r;
+  DevCtx &cx = g_ctx[dev];
+  std::lock_guard<std::mutex> lock(cx.mu);
+  if (int r = ctx_init(cx)) return r;
+  std::vector<int8_t> lo((size_t)bp.planes * bp.kl * C);
+  for (int i = 0; i < bp.kl; ++i)
+    for (int l = 0; l < C; ++l) {
+      const int32_t v = A[(size_t)i * C + l];
+      if (bp.planes == 1) {
+        lo[(size_t)i * C + l] = (int8_t)v;
+      } else {
+        const int32_t lo8 = ((v + 128) & 255) - 128;
+        lo[(size_t)i * C + l] = (int8_t)((v - lo8) / 256);
+        lo[((size_t)bp.kl + i) * C + l] = (int8_t)lo8;
+      }
+    }
+  if (int r = ensure(cx.lo, lo.size())) return r;
+  if (int r = ensure(cx.table, (size_t)words * 4)) return r;
+  HIPCHK(hipMemcpyAsync(cx.lo.p, lo.data(), lo.size(), hipMemcpyHostToDevice, cx.stream));
+  TableArgs ta{(const int8_t *)cx.lo.p, bp.planes, bp.kl, C, marginal, bp.np, bp.P, (uint32_t *)cx.table.p};
+  HIPCHK(launch_table(ta, cx.stream));
+  HIPCHK(hipMemcpyAsync(out, cx.table.p, (size_t)words * 4, hipMemcpyDeviceToHost, cx.stream));
+  HIPCHK(hipStreamSynchronize(cx.stream));
+  return words;
+}
+
+}  // extern "C"

@@ -1,0 +1,90 @@
+// lnorm_internal.h -- shared declarations of the HIP engine (kernels and the
+// C-ABI layer).  Device code targets gfx950 (wave64) only.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace lnorm {
+
+// Which objective a kernel evaluates (the reference's kernel selection,
+// L_OpenMP.c:251-290 / L_CUDA.cu:299-333).
+enum Kind : int { KIND_L1 = 0, KIND_LM = 1, KIND_L2 = 2, KIND_LD = 3 };
+
+// Phase of the two-phase search: MAX finds the norm and the best value of
+// every unit (a contiguous rank range); FIRST re-walks one unit and returns
+// the smallest Gray rank attaining a given target value.
+enum Mode : int { MODE_MAX = 0, MODE_FIRST = 1 };
+
+constexpr uint32_t kBias32 = 0x80000000u;   // y stored as y + 2^31 for v_sad_u32
+constexpr uint32_t kBias16 = 0x8000u;       // per half for v_sad_u16
+constexpr uint32_t kBias16x2 = 0x80008000u;
+
+// Plain Gray walk: lane-per-segment.  The ranks [rank0, rank0 + nseg*L) are
+// split into nseg segments of L = base^q ranks; segment `seg` (0-based,
+// relative) starts at rank rank0 + seg*L.  rank0 must be a multiple of L.
+struct PlainArgs {
+  const int32_t *A;        // R x ny_pad, row-major, zero-padded columns
+  const int32_t *D;        // binary: [2][R-1][ny_pad] signed row deltas
+  int R;                   // rows (row R-1 has the fixed label)
+  int ny_pad;              // padded column count (template NY)
+  int d;                   // labels (1 = L1/LM, 2, 3..10)
+  int q;                   // segment = d^q ranks (2^q for d <= 2)
+  uint64_t seg0;           // first segment index (rank0 / L)
+  uint64_t nseg;           // number of segments
+  int32_t target;          // FIRST mode: value to find
+  int32_t *unit_best;      // MAX mode: best per segment [nseg]
+  int32_t *gbest;          // MAX mode: atomic max
+  unsigned long long *gfirst;  // FIRST mode: atomic min of the rank
+  uint32_t *state;         // general kernel: per-wave scratch, kGeneralSlices * nvec * ny words
+};
+
+// The general kernel runs at most this many waves at once (its scratch size).
+constexpr uint64_t kGeneralSlices = 2048;
+
+// Blocked L_1 / L_M walk (low-row table + scalar high walk).
+struct BlockedArgs {
+  const uint32_t *table;   // [P][NP+MARG][64] packed low-pattern words (see kernel)
+  const uint32_t *dpk;     // [2][nh][NW] packed high-row deltas applied to the scalar state
+  const uint32_t *seed;    // [NW] packed scalar state at high rank 0 (before any flips)
+  const uint32_t *dfull;   // [nh][NW] packed sign-flip deltas: seed flip of high row m
+  int nh;                  // high rows walked by the Gray code
+  int s;                   // high ranks per wave = 2^s
+  uint64_t wave0;          // first wave index
+  uint64_t nwaves;
+  int32_t *unit_best;      // best per wave
+  int32_t *gbest;
+};
+
+struct TableArgs {
+  const int8_t *lo;        // low rows as int8 planes: [planes][kl][ny] (hi plane first)
+  int planes;              // 1 (entries fit int8) or 2 (int16: hi/lo planes)
+  int kl;                  // low rows (table has 2^kl patterns)
+  int ny;                  // columns of the table (before packing)
+  int marginal;            // LM: column 0 kept as an int32 word
+  int np;                  // packed words per pattern (excluding the marginal word)
+  int P;                   // patterns per lane
+  uint32_t *out;           // packed table
+};
+
+// Launchers (return hipSuccess or the launch error).  `grid_hint` <= 0 means
+// choose automatically.
+hipError_t launch_plain(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block);
+bool plain_supported(Kind kind, int d, int ny_pad);
+int plain_ny_pad(Kind kind, int d, int ny);     // smallest instantiated width >= ny, 0 if none
+
+hipError_t launch_blocked(int np, int P, int marginal, const BlockedArgs &a, hipStream_t st);
+bool blocked_supported(int np, int P, int marginal);
+int blocked_np_pad(int np, int P, int marginal);
+
+hipError_t launch_table(const TableArgs &a, hipStream_t st);
+
+hipError_t launch_general(Kind kind, Mode mode, const PlainArgs &a, int ny, hipStream_t st);
+
+hipError_t launch_fill_i32(int32_t *p, int32_t v, uint64_t n, hipStream_t st);
+hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t n, hipStream_t st);
+hipError_t launch_first_unit(const int32_t *unit_best, uint64_t n, const int32_t *gbest,
+                             unsigned long long *first, hipStream_t st);
+hipError_t launch_sad_probe(const uint32_t *a, const uint32_t *b, const uint32_t *c, uint32_t *out, int n,
+                            hipStream_t st);
+
+}  // namespace lnorm

@@ -1,0 +1,188 @@
+"""Parity of the HIP engine with the oracle and the reference's golden
+vectors (bit-exact norms, strategies and maximising Gray ranks).  GPU only:
+every call goes through the C ABI of lib/liblnorm_hip.so."""
+import numpy as np
+import pytest
+
+import oracle
+from oracle import host_oracle as ho
+
+pytestmark = pytest.mark.gpu
+
+
+def _laid_out(case):
+    argv = ["L", str(case["threads"]), "m.txt"] + case["args"]
+    threads, _, _, n0, stat, marg = ho.parse_args(argv, "omp")
+    p = ho.load(case["matrix"].encode(), n0, stat, marg, "omp", threads)
+    A = np.array(p.A, dtype=np.int32).reshape(p.R, p.C) if p.R else None
+    return p, A
+
+
+def _rand(rng, R, C, r):
+    return rng.integers(-r, r + 1, size=(R, C)).astype(np.int32)
+
+
+def _agree(eng, A, n, marg, **kw):
+    want = oracle.calc_lnorm(A, n, marg, 8)
+    norm, strat, res = eng.calc_lnorm(A, n, marg, **kw)
+    assert (norm, strat, res["best_rank"]) == want, (A.shape, n, marg, kw, res["kernel_name"])
+    return res
+
+
+def test_selftest(pkg):
+    print("selftest", flush=True)
+    assert pkg.selftest() == 0, pkg.last_error()
+
+
+def test_golden_vectors_all_kernels(pkg, golden):
+    n_checked = 0
+    for case in golden:
+        if case["returncode"] != 0:
+            continue
+        p, A = _laid_out(case)
+        if p.R == 0:
+            continue
+        want_line = case["stdout"].strip().splitlines()[-1]
+        for kernel in ("auto", "plain", "general", "blocked"):
+            try:
+                norm, strat, res = pkg.calc_lnorm(A, p.n, p.marginal, kernel=kernel)
+            except pkg.LnormError as e:
+                assert e.code == -5 and kernel in ("plain", "blocked"), (case["name"], kernel, str(e))
+                continue
+            line, fname, lines = ho.result_lines(p, norm, strat)
+            assert line == want_line, (case["name"], kernel, res["kernel_name"])
+            assert {fname: "".join(s + "\n" for s in lines)} == case["strategy"], (case["name"], kernel)
+            n_checked += 1
+    assert n_checked > 150
+
+
+@pytest.mark.parametrize("n,marg", [(1, 0), (1, 1), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (7, 0),
+                                    (8, 0), (9, 0), (10, 0)])
+def test_random_vs_oracle_plain_and_general(pkg, n, marg):
+    rng = np.random.default_rng(200 + 10 * n + marg)
+    rmax = {1: 17, 2: 15, 3: 10, 4: 8, 5: 7, 6: 6, 7: 6, 8: 5, 9: 5, 10: 5}[n]
+    for trial in range(12):
+        R = int(rng.integers(1, rmax + 1))
+        C = int(rng.integers(1, 41))
+        A = _rand(rng, R, C, int(rng.choice([1, 2, 9, 100, 5000])))
+        print("random n=%d marg=%d trial=%d shape=%s" % (n, marg, trial, A.shape), flush=True)
+        _agree(pkg, A, n, marg, kernel="general")
+        if pkg.KERNELS and C <= {1: 128, 2: 64, 3: 40, 4: 32, 5: 24, 6: 24}.get(n, 16):
+            _agree(pkg, A, n, marg, kernel="plain")
+
+
+@pytest.mark.parametrize("marg", [0, 1])
+@pytest.mark.parametrize("P", [1, 2, 4, 8])
+def test_blocked_vs_oracle(pkg, marg, P):
+    rng = np.random.default_rng(300 + P + 17 * marg)
+    kl = 6 + {1: 0, 2: 1, 4: 2, 8: 3}[P]
+    for trial in range(6):
+        R = int(rng.integers(kl + 2, kl + 9))
+        C = int(rng.integers(2 + marg, 65))
+        r = int(rng.choice([1, 3, 127, 300]))
+        A = _rand(rng, R, C, r)
+        print("blocked P=%d marg=%d trial=%d shape=%s r=%d" % (P, marg, trial, A.shape, r), flush=True)
+        res = _agree(pkg, A, 1, marg, kernel="blocked", patterns_per_lane=P)
+        assert res["kernel"] == pkg.KERNEL_BLOCKED
+
+
+def test_blocked_table_matches_numpy(pkg):
+    rng = np.random.default_rng(9)
+    for marg, P, r in [(0, 4, 127), (1, 2, 127), (0, 2, 3000), (1, 8, 20000), (0, 1, 5)]:
+        kl = 6 + {1: 0, 2: 1, 4: 2, 8: 3}[P]
+        C = 37
+        A = _rand(rng, kl + 3, C, r)
+        A[:, marg:] //= max(1, (kl + 3) * r // 30000)    # keep |.| column sums < 2^15
+        tab = pkg.debug_table(A, marg, P)
+        npk = ((C - marg) + 1) // 2
+        npk = max(4, npk + (npk & 1))
+        NW = npk + marg
+        tab = tab.reshape(P, NW, 64)
+        cols = list(range(marg, C))
+        for p in range(64 * P):
+            signs = np.array([1 if (p >> i) & 1 else -1 for i in range(kl)])
+            Y = signs @ A[:kl].astype(np.int64)
+            rr, L = divmod(p, 64)
+            for c in range(npk):
+                lo = cols[2 * c] if 2 * c < len(cols) else None
+                hi = cols[2 * c + 1] if 2 * c + 1 < len(cols) else None
+                ylo = Y[lo] if lo is not None else 0
+                yhi = Y[hi] if hi is not None else 0
+                assert int(tab[rr, c, L]) == ((int(yhi) + 32768) << 16) | (int(ylo) + 32768), (marg, P, p, c)
+            if marg:
+                assert int(tab[rr, npk, L]) == int(Y[0]) & 0xFFFFFFFF
+
+
+def test_segment_size_independence(pkg):
+    rng = np.random.default_rng(4)
+    A = _rand(rng, 16, 12, 2)          # tiny entries: many ties
+    want = oracle.calc_lnorm(A, 1, 0, 8)
+    for s in (1, 3, 7, 12, 15):
+        norm, strat, res = pkg.calc_lnorm(A, 1, 0, kernel="plain", seg_log=s)
+        assert (norm, strat, res["best_rank"]) == want
+    for wl in (1, 2, 5):
+        norm, strat, res = pkg.calc_lnorm(A, 1, 0, kernel="blocked", wave_log2=wl, patterns_per_lane=2)
+        assert (norm, strat, res["best_rank"]) == want
+    B = _rand(rng, 9, 7, 3)
+    want3 = oracle.calc_lnorm(B, 3, 0, 8)
+    for s in (1, 2, 5, 8):
+        norm, strat, res = pkg.calc_lnorm(B, 3, 0, kernel="plain", seg_log=s)
+        assert (norm, strat, res["best_rank"]) == want3
+
+
+def test_errors(pkg):
+    big = np.full((3, 2), 2 ** 30, dtype=np.int32)
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm(big, 1)
+    assert e.value.code == -2
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm(np.ones((42, 3), np.int32), 3)        # maxRows(3) = 41
+    assert e.value.code == -3
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm(np.ones((12, 3), np.int32), 2, kernel="blocked")
+    assert e.value.code == -5
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm(np.ones((3, 3), np.int32), 11)
+    assert e.value.code == -3
+
+
+def test_simple_abi_and_batch(pkg):
+    rng = np.random.default_rng(12)
+    probs = []
+    for i in range(20):
+        n = [1, 1, 2, 3, 4][i % 5]
+        marg = 1 if (n == 1 and i % 2) else 0
+        R = int(rng.integers(2, 12 if n <= 2 else 7))
+        probs.append((_rand(rng, R, int(rng.integers(1, 30)), 50), n, marg))
+    out = pkg.calc_batch(probs)
+    for (A, n, marg), (norm, strat, status) in zip(probs, out):
+        assert status == 0
+        assert (norm, strat) == oracle.calc_lnorm(A, n, marg, 4)[:2]
+        assert pkg.calc_lnorm_simple(A, n, marg) == (norm, strat)
+
+
+def test_workloads_vs_oracle(pkg):
+    from lnorm_amd import gen
+    # 20x20 L_1 from seed 101 (2^19 sign vectors) on every kernel
+    A = gen.gen_matrix(20, 20, 10, 101)
+    for k in ("auto", "plain", "blocked", "general"):
+        print("cfg1 kernel", k, flush=True)
+        _agree(pkg, A, 1, 0, kernel=k)
+    # L_2 and L_3 of the 18 x 32 file (3^17 labelings at d = 3)
+    B = gen.gen_matrix(18, 32, 50, 103)
+    print("cfg3 L2", flush=True)
+    _agree(pkg, B, 2, 0)
+    print("cfg3 L3", flush=True)
+    _agree(pkg, B, 3, 0)
+
+
+def test_blocked_equals_plain_large(pkg):
+    from lnorm_amd import gen
+    # 2^35 marginal sign vectors (36 x 36) and 2^33 for a 34 x 40 matrix
+    for (R, C, r, seed, marg) in [(36, 36, 100, 102, 1), (34, 40, 100, 7, 0), (33, 64, 127, 104, 0)]:
+        A = gen.gen_matrix(R, C, r, seed)
+        print("large blocked vs plain", R, C, marg, flush=True)
+        nb, sb, rb = pkg.calc_lnorm(A, 1, marg, kernel="blocked")
+        npl, sp, rp = pkg.calc_lnorm(A, 1, marg, kernel="plain")
+        assert (nb, sb, rb["best_rank"]) == (npl, sp, rp["best_rank"])
+        assert oracle.objective(A, 1, marg, sb) == nb
