@@ -554,7 +554,7 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
     rr.best_rank = rank;
     labels_of_rank(rank, R, n, strategy);
   } else {
-    rr.best_rank = ~0ull;
+    rr.best_rank = R == 1 ? 0 : ~0ull;   // one row: rank 0 is the only labeling
   }
   HIPCHK(hipEventSynchronize(cx.ev[1]));
   float ms = 0;

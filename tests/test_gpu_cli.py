@@ -1,0 +1,49 @@
+"""End-to-end drop-in check: the L_HIP binary (C host + HIP engine through
+the C ABI) reproduces the reference's stdout and strategy file byte for byte
+on every golden case (L_OpenMP argv form), and L_CUDA's output format in its
+own argv form."""
+import os
+import subprocess
+
+import pytest
+
+from oracle import host_oracle as ho
+
+pytestmark = pytest.mark.gpu
+
+
+def _run(pkg, args, cwd):
+    return subprocess.run([pkg.CLI] + args, cwd=cwd, capture_output=True, text=True, timeout=300)
+
+
+def test_cli_matches_reference_byte_for_byte(pkg, golden, tmp_path):
+    for case in golden:
+        d = tmp_path / case["name"]
+        d.mkdir()
+        (d / "matrix.txt").write_text(case["matrix"])
+        print("cli", case["name"], flush=True)
+        r = _run(pkg, ["--omp", str(case["threads"]), "matrix.txt"] + case["args"], d)
+        assert r.returncode == (case["returncode"] & 0xFF), (case["name"], r.stderr)
+        assert r.stdout == case["stdout"], case["name"]
+        files = {f: (d / f).read_text() for f in os.listdir(d) if f.startswith("strategy_")}
+        assert files == case["strategy"], case["name"]
+
+
+def test_cli_cuda_argv_form(pkg, golden, tmp_path):
+    for case in golden[::4]:
+        if case["returncode"] != 0:
+            continue
+        d = tmp_path / case["name"]
+        d.mkdir()
+        (d / "matrix.txt").write_text(case["matrix"])
+        B, T = 512, 32
+        r = _run(pkg, [str(B), str(T), "matrix.txt"] + case["args"], d)
+        assert r.returncode == 0, (case["name"], r.stderr)
+        argv = ["L_CUDA", str(B), str(T), "matrix.txt"] + case["args"]
+        _, blocks, _, n0, stat, marg = ho.parse_args(argv, "cuda")
+        p = ho.load(case["matrix"].encode(), n0, stat, marg, "cuda", T, blocks)
+        ref_lines = case["stdout"].splitlines()
+        want = p.lines + ref_lines[len(p.lines):]
+        assert r.stdout.splitlines() == want, case["name"]
+        files = {f: (d / f).read_text() for f in os.listdir(d) if f.startswith("strategy_")}
+        assert files == case["strategy"], case["name"]

@@ -76,9 +76,11 @@ def test_random_vs_oracle_plain_and_general(pkg, n, marg):
 def test_blocked_vs_oracle(pkg, marg, P):
     rng = np.random.default_rng(300 + P + 17 * marg)
     kl = 6 + {1: 0, 2: 1, 4: 2, 8: 3}[P]
+    # instantiated widths: at most 32 packed words, P * (words + marg) <= 192
+    cmax = min(64 + marg, 2 * (192 // P - marg) + marg)
     for trial in range(6):
         R = int(rng.integers(kl + 2, kl + 9))
-        C = int(rng.integers(2 + marg, 65))
+        C = int(rng.integers(2 + marg, cmax + 1))
         r = int(rng.choice([1, 3, 127, 300]))
         A = _rand(rng, R, C, r)
         print("blocked P=%d marg=%d trial=%d shape=%s r=%d" % (P, marg, trial, A.shape, r), flush=True)
