@@ -51,6 +51,8 @@ def valu_ops_per_unit(kernel: str, ny: int, marginal: int) -> int:
              the scalar unit), plus the marginal add and the maximum."""
     if kernel == "blocked":
         return (ny - marginal + 1) // 2 + 1 + marginal
+    if kernel == "blocked32":        # one v_sad_u32 per |.| column
+        return ny - marginal + 1 + marginal
     return 2 * ny + 1
 
 
@@ -95,18 +97,19 @@ def summarize(r, total, ny, kernel, marginal=0):
     }
 
 
-def cpu_baseline(gen, budget_rows: int = 33):
+def cpu_baseline(gen, rows: int = 64, r: int = 127, seed: int = 104, budget_cols: int = 35):
     """The reference L_OpenMP (built from its sources into oracle/_ref) on the
-    host cores over a bounded sample: the same 64-column int8 distribution
-    with `budget_rows` columns, i.e. 2^(budget_rows-1) sign vectors of 64
-    partial sums each.  Falls back to the oracle's restatement when the
-    reference binary is absent."""
+    host cores over a bounded sample: the workload's own entry distribution
+    with `budget_cols` columns, i.e. 2^(budget_cols-1) sign vectors of `rows`
+    partial sums each (the reference transposes it to budget_cols x rows).
+    Falls back to the oracle's restatement when the reference binary is
+    absent."""
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     ref = os.path.join(ROOT, "oracle", "_ref", "L_OpenMP")
-    M = gen.gen_matrix(64, budget_rows, 127, 104)
-    units = 2 ** (budget_rows - 1)
-    sample = "64 x %d int8 file (k = %d after the reference's transpose, 64 partial sums): 2^%d sign vectors" % (
-        budget_rows, budget_rows, budget_rows - 1)
+    M = gen.gen_matrix(rows, budget_cols, r, seed)
+    units = 2 ** (budget_cols - 1)
+    sample = "%d x %d file, entries in [-%d, %d] (k = %d after the reference's transpose, %d partial sums): " \
+             "2^%d sign vectors" % (rows, budget_cols, r, r, budget_cols, rows, budget_cols - 1)
     if os.path.exists(ref):
         with tempfile.TemporaryDirectory() as td:
             path = os.path.join(td, "m.txt")
@@ -205,12 +208,27 @@ def main():
         line["n40_m40"] = {"blocked": s1["value"], "plain": s2["value"], "speedup": s1["value"] / s2["value"],
                            "blocked_valu_frac": s1["valu_frac"], "plain_valu_frac": s2["valu_frac"],
                            "norm": s1["norm"]}
+        # 128 x 40 int16 (entries in [-30000, 30000], seed 105), laid out 40 x 128:
+        # column sums beyond 2^15, so the blocked path uses 32-bit words
+        # (hi/lo int8 MFMA planes for the table, one v_sad_u32 per column)
+        C16 = gen.gen_matrix(128, 40, 30000, 105).T.copy()
+        b16 = run_steps(pkg, C16, "blocked", 1, 0)
+        p16 = run_steps(pkg, C16, "plain", 1, 0)
+        if (b16["norm"], b16["best_rank"]) != (p16["norm"], p16["best_rank"]):
+            raise RuntimeError("plain and blocked kernels disagree on the int16 matrix")
+        s3 = summarize(b16, 2 ** 39, 128, "blocked32")
+        s4 = summarize(p16, 2 ** 39, 128, "plain")
+        line["int16_128x40"] = {"blocked": s3["value"], "plain": s4["value"], "speedup": s3["value"] / s4["value"],
+                                "blocked_valu_frac": s3["valu_frac"], "plain_valu_frac": s4["valu_frac"],
+                                "kernel": s3["kernel_name"], "norm": s3["norm"]}
     if not a.no_cpu:
         cb = cpu_baseline(gen)
         line["cpu_baseline"] = {k2: cb[k2] for k2 in ("value", "unit", "cores", "kind", "sample")}
         line["speedup_vs_cpu"] = sb["value"] / cb["value"]
         if "n40_m40" in line:
-            line["n40_m40"]["speedup_vs_cpu_64col_sample"] = line["n40_m40"]["blocked"] / cb["value"]
+            c40 = cpu_baseline(gen, rows=40, r=100, seed=106, budget_cols=35)
+            line["n40_m40"]["cpu_baseline"] = {k2: c40[k2] for k2 in ("value", "cores", "kind", "sample")}
+            line["n40_m40"]["speedup_vs_cpu"] = line["n40_m40"]["blocked"] / c40["value"]
     print(json.dumps(line), flush=True)
     return 0
 

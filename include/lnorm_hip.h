@@ -36,7 +36,7 @@ extern "C" {
 /* kernel selection */
 #define LNORM_KERNEL_AUTO 0     /* blocked when applicable, else plain, else general */
 #define LNORM_KERNEL_PLAIN 1    /* register-resident Gray walk, lane per segment */
-#define LNORM_KERNEL_BLOCKED 2  /* L1/LM: MFMA-built low-row table + scalar high walk */
+#define LNORM_KERNEL_BLOCKED 2  /* L1/LM: MFMA-built low-row table + scalar high walk (16- or 32-bit words) */
 #define LNORM_KERNEL_GENERAL 3  /* any width: wave per segment, columns over lanes */
 
 typedef struct lnorm_hip_opts {
@@ -45,9 +45,9 @@ typedef struct lnorm_hip_opts {
     int32_t seg_log;            /* plain/general: segment = base^seg_log ranks per lane; 0 = auto */
     int32_t patterns_per_lane;  /* blocked: 1, 2, 4 or 8 low patterns per lane; 0 = auto */
     int32_t wave_log2;          /* blocked: 2^wave_log2 high ranks per wave; 0 = auto */
-    int32_t block;              /* threads per workgroup hint (multiple of 64); 0 = auto */
+    int32_t block;              /* plain kernels: threads per workgroup, 64..256 in steps of 64; else 256 */
     int32_t device;             /* HIP device ordinal; -1 = current device */
-    int32_t reserved0;
+    int32_t word_bits;          /* blocked: 16 (two halves per word) or 32; 0 = 16 when column sums allow */
     void *stream;               /* hipStream_t to run on; NULL = the library's own stream */
 } lnorm_hip_opts;
 
@@ -124,11 +124,13 @@ void lnorm_hip_release(void);
  * negative LNORM_E_* code. */
 int lnorm_hip_selftest(void);
 
-/* Diagnostic: the packed low-pattern table the blocked kernel would use
- * (P*(np+marginal)*64 words) for the first kl = 6+log2(P) rows of A.
- * Returns the number of words written, or a negative LNORM_E_* code. */
+/* Diagnostic: the low-pattern table the blocked kernel would use
+ * (P*(words+marginal)*64 words) for the first kl = 6+log2(P) rows of A, in
+ * the 16-bit packed (word_bits 16) or 32-bit (word_bits 32) format; 0 = the
+ * format AUTO would pick.  Returns the number of words written, or a
+ * negative LNORM_E_* code. */
 int lnorm_hip_debug_table(const int32_t *mtx_as_vec, int iRows, int iCols, int marginal, int P,
-                          uint32_t *out, int out_words);
+                          int word_bits, uint32_t *out, int out_words);
 
 #ifdef __cplusplus
 }

@@ -31,7 +31,7 @@ class Opts(ctypes.Structure):
     _fields_ = [("kernel", ctypes.c_int32), ("want_strategy", ctypes.c_int32),
                 ("seg_log", ctypes.c_int32), ("patterns_per_lane", ctypes.c_int32),
                 ("wave_log2", ctypes.c_int32), ("block", ctypes.c_int32),
-                ("device", ctypes.c_int32), ("reserved0", ctypes.c_int32),
+                ("device", ctypes.c_int32), ("word_bits", ctypes.c_int32),
                 ("stream", ctypes.c_void_p)]
 
 
@@ -98,7 +98,7 @@ def hip_lib():
         L.lnorm_hip_selftest.restype = ctypes.c_int
         L.lnorm_hip_debug_table.restype = ctypes.c_int
         L.lnorm_hip_debug_table.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                            ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
+                                            ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
         _hip = L
     return _hip
 
@@ -180,14 +180,14 @@ def selftest() -> int:
     return _check(hip_lib().lnorm_hip_selftest())
 
 
-def debug_table(A, marginal: int, P: int):
+def debug_table(A, marginal: int, P: int, word_bits: int = 0):
+    """The blocked kernel's low-pattern table, flat [P][words][64]."""
     a = _i32(A)
     R, C = a.shape
-    np_ = ((C - marginal) + 1) // 2
-    words = P * (max(np_, 4) + 2 + marginal) * 64
+    words = P * (C + 40) * 64
     out = np.zeros(words, dtype=np.uint32)
     i32p = ctypes.POINTER(ctypes.c_int32)
-    n = _check(hip_lib().lnorm_hip_debug_table(a.ctypes.data_as(i32p), R, C, marginal, P,
+    n = _check(hip_lib().lnorm_hip_debug_table(a.ctypes.data_as(i32p), R, C, marginal, P, word_bits,
                                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), words))
     return out[:n]
 

@@ -291,80 +291,123 @@ int run_walk_first(DevCtx &cx, const Problem &p, const Prepared &pp, uint64_t u,
 }
 
 // ---- blocked L1/LM -----------------------------------------------------------
+// Two word formats: 16-bit (two biased halves per word, v_sad_u16; every |.|
+// column's absolute sum <= 2^15 - 1, at most 64 |.| columns) and 32-bit
+// (one word per column, v_sad_u32, up to 128 |.| columns over W waves).
 struct BlockedPlan {
+  bool wide = false;
   int P = 0, kl = 0, nh = 0, np = 0, NW = 0, s = 0, planes = 1;
-  uint64_t nwaves = 0;
+  int W = 1, NC = 0;       // wide: waves per unit, words per wave (np = W * NC)
+  uint64_t nunits = 0;     // waves (16-bit) or workgroups (32-bit)
 };
 
-// Abs columns of the blocked layout: L1 all columns, LM columns 1..C-1.
-bool blocked_plan(const Problem &p, const lnorm_hip_opts &o, BlockedPlan &bp, std::string &why) {
-  if (p.kind != KIND_L1 && p.kind != KIND_LM) { why = "blocked kernel handles n = 1 only"; return false; }
-  const int marg = p.kind == KIND_LM ? 1 : 0;
-  const int nabs = p.C - marg;
-  int np = (nabs + 1) / 2;
-  // 16-bit halves: every |.| column's absolute sum must stay below 2^15
-  for (int l = marg; l < p.C; ++l) {
-    int64_t cs = 0;
-    for (int i = 0; i < p.R; ++i) cs += std::llabs((long long)p.A[(size_t)i * p.C + l]);
-    if (cs > 32767) { why = "a column's absolute sum exceeds 32767 (16-bit halves)"; return false; }
-  }
-  int P = o.patterns_per_lane;
-  if (P == 0) {
-    P = 4;
-    int npp = blocked_np_pad(np, P, marg);
-    if (!npp || P * (npp + marg) > 132) P = 2;
-  }
-  const int npp = blocked_np_pad(np, P, marg);
-  if (!npp) { why = "no blocked instantiation for this width / P"; return false; }
-  int kl = 6;
-  for (int t = P; t > 1; t >>= 1) ++kl;
-  if (p.R - 1 < kl + 1) { why = "too few rows for the blocked split"; return false; }
-  if (marg) {
-    for (int i = 0; i < kl; ++i)
-      if (std::llabs((long long)p.A[(size_t)i * p.C]) > 32767) { why = "marginal column too large"; return false; }
-  }
-  bool int8ok = true;
+// Signed-digit int8 planes of the low rows: v = 256 * hi + lo, lo in [-128, 127].
+// Returns 1 or 2 planes, or 0 when an entry needs more than two.
+int low_planes(const Problem &p, int kl) {
+  int planes = 1;
   for (int i = 0; i < kl; ++i)
     for (int l = 0; l < p.C; ++l) {
       const int32_t v = p.A[(size_t)i * p.C + l];
-      if (v < -128 || v > 127) int8ok = false;
+      if (v < -128 || v > 127) planes = 2;
+      if (v < -32768 + 128 || v > 32767 - 128) return 0;
     }
+  return planes;
+}
+
+void split_planes(const Problem &p, int kl, int planes, std::vector<int8_t> &lo) {
+  lo.assign((size_t)planes * kl * p.C, 0);
+  for (int i = 0; i < kl; ++i)
+    for (int l = 0; l < p.C; ++l) {
+      const int32_t v = p.A[(size_t)i * p.C + l];
+      if (planes == 1) {
+        lo[(size_t)i * p.C + l] = (int8_t)v;
+      } else {
+        const int32_t lo8 = ((v + 128) & 255) - 128;
+        lo[(size_t)i * p.C + l] = (int8_t)((v - lo8) / 256);
+        lo[((size_t)kl + i) * p.C + l] = (int8_t)lo8;
+      }
+    }
+}
+
+int log2i(int x) {
+  int k = 0;
+  while ((1 << (k + 1)) <= x) ++k;
+  return k;
+}
+
+bool blocked_plan(const Problem &p, const lnorm_hip_opts &o, BlockedPlan &bp, std::string &why) {
+  if (p.kind != KIND_L1 && p.kind != KIND_LM) { why = "blocked kernels handle n = 1 only"; return false; }
+  const int marg = p.kind == KIND_LM ? 1 : 0;
+  const int nabs = p.C - marg;
+  int64_t maxcs = 0;
+  for (int l = marg; l < p.C; ++l) {
+    int64_t cs = 0;
+    for (int i = 0; i < p.R; ++i) cs += std::llabs((long long)p.A[(size_t)i * p.C + l]);
+    maxcs = std::max(maxcs, cs);
+  }
+  const int bits = o.word_bits;
+  if (bits != 0 && bits != 16 && bits != 32) { why = "word_bits must be 0, 16 or 32"; return false; }
+  bool wide;
+  if (bits == 16) {
+    if (maxcs > 32767) { why = "a column's absolute sum exceeds 32767 (16-bit halves)"; return false; }
+    if (nabs > 64) { why = "more than 64 |.| columns for 16-bit words"; return false; }
+    wide = false;
+  } else if (bits == 32) {
+    wide = true;
+  } else {
+    wide = maxcs > 32767 || nabs > 64;
+  }
+  int P = o.patterns_per_lane;
+  if (!wide) {
+    const int np = (nabs + 1) / 2;
+    if (P == 0) {
+      P = 4;
+      const int npp = blocked_np_pad(np, P, marg);
+      if (!npp || P * (npp + marg) > 132) P = 2;
+    }
+    const int npp = blocked_np_pad(np, P, marg);
+    if (!npp) { why = "no 16-bit blocked instantiation for this width / P"; return false; }
+    bp.np = npp;
+    bp.NW = npp + marg;
+  } else {
+    if (nabs > 128) { why = "more than 128 |.| columns"; return false; }
+    const int W = nabs <= 32 ? 1 : (nabs <= 64 ? 2 : 4);
+    const int NC = ((nabs + W - 1) / W + 7) / 8 * 8;
+    if (P == 0) P = 4 * NC <= 128 ? 4 : 2;
+    if (!blocked32_supported(NC, P, W, marg)) { why = "no 32-bit blocked instantiation for this width / P"; return false; }
+    bp.W = W;
+    bp.NC = NC;
+    bp.np = W * NC;
+    bp.NW = bp.np + marg;
+  }
+  bp.wide = wide;
+  if (!(P == 1 || P == 2 || P == 4 || P == 8)) { why = "patterns per lane must be 1, 2, 4 or 8"; return false; }
+  const int kl = 6 + log2i(P);
+  const int min_nh = wide ? 3 : 1;
+  if (p.R - 1 < kl + min_nh) { why = "too few rows for the blocked split"; return false; }
+  bp.planes = low_planes(p, kl);
+  if (!bp.planes) { why = "low-row entries beyond the two int8 planes of int16"; return false; }
   bp.P = P;
   bp.kl = kl;
   bp.nh = p.R - 1 - kl;
-  bp.np = npp;
-  bp.NW = npp + marg;
-  bp.planes = int8ok ? 1 : 2;
-  int s = o.wave_log2 > 0 ? o.wave_log2 : std::max(0, bp.nh - 15);
+  // about 2^19 waves of 2^12-ish steps: many rounds of the 1024 SIMDs keep
+  // the tail of the last round short (measured: 2^12 steps per wave run 1.3%
+  // faster than 2^16 on a 2^39 enumeration)
+  const int wl = wide ? log2i(bp.W) : 0;
+  int s = o.wave_log2 > 0 ? o.wave_log2 : std::max(0, bp.nh - 19 + wl);
   s = std::min(std::min(s, bp.nh), 30);
+  if (wide) s = std::max(s, 3);
   bp.s = s;
-  bp.nwaves = 1ull << (bp.nh - s);
+  bp.nunits = 1ull << (bp.nh - s);
   return true;
 }
 
 int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t st, Phase1Out &out) {
   const int marg = p.kind == KIND_LM ? 1 : 0;
   const int NW = bp.NW, kl = bp.kl, nh = bp.nh;
-  // column of packed half h (0 = low, 1 = high) of word c; -1 = zero padding
-  auto colof = [&](int c, int h) -> int {
-    const int l = marg + 2 * c + h;
-    return l < p.C ? l : -1;
-  };
   auto Aat = [&](int i, int l) -> int64_t { return l < 0 ? 0 : (int64_t)p.A[(size_t)i * p.C + l]; };
-  // low-row planes for the MFMA table
-  std::vector<int8_t> lo((size_t)bp.planes * kl * p.C);
-  for (int i = 0; i < kl; ++i)
-    for (int l = 0; l < p.C; ++l) {
-      const int32_t v = p.A[(size_t)i * p.C + l];
-      if (bp.planes == 1) {
-        lo[(size_t)i * p.C + l] = (int8_t)v;
-      } else {
-        const int32_t lo8 = ((v + 128) & 255) - 128;
-        const int32_t hi8 = (v - lo8) / 256;
-        lo[(size_t)i * p.C + l] = (int8_t)hi8;
-        lo[((size_t)kl + i) * p.C + l] = (int8_t)lo8;
-      }
-    }
+  std::vector<int8_t> lo;
+  split_planes(p, kl, bp.planes, lo);
   // scalar state: all high rows at -1, and the flip deltas
   std::vector<uint32_t> seed(NW), dfull((size_t)std::max(nh, 1) * NW), dpk((size_t)2 * std::max(nh, 1) * NW);
   auto yminus = [&](int l) -> int64_t {
@@ -373,13 +416,26 @@ int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t
     for (int m = 0; m < nh; ++m) y -= Aat(kl + m, l);
     return y;
   };
-  for (int c = 0; c < bp.np; ++c) {
-    const int64_t ylo = yminus(colof(c, 0)), yhi = yminus(colof(c, 1));
-    seed[c] = ((uint32_t)(32768 - yhi) << 16) | ((uint32_t)(32768 - ylo) & 0xFFFFu);
-    for (int m = 0; m < nh; ++m) {
-      const int64_t dlo = 2 * Aat(kl + m, colof(c, 0)), dhi = 2 * Aat(kl + m, colof(c, 1));
-      const uint32_t w = (uint32_t)(int64_t)(-(dhi * 65536 + dlo));
-      dfull[(size_t)m * NW + c] = w;
+  if (!bp.wide) {
+    // word c packs |.| columns (marg + 2c, marg + 2c + 1); -1 = zero padding
+    auto colof = [&](int c, int h) -> int {
+      const int l = marg + 2 * c + h;
+      return l < p.C ? l : -1;
+    };
+    for (int c = 0; c < bp.np; ++c) {
+      const int64_t ylo = yminus(colof(c, 0)), yhi = yminus(colof(c, 1));
+      seed[c] = ((uint32_t)(32768 - yhi) << 16) | ((uint32_t)(32768 - ylo) & 0xFFFFu);
+      for (int m = 0; m < nh; ++m) {
+        const int64_t dlo = 2 * Aat(kl + m, colof(c, 0)), dhi = 2 * Aat(kl + m, colof(c, 1));
+        dfull[(size_t)m * NW + c] = (uint32_t)(int64_t)(-(dhi * 65536 + dlo));
+      }
+    }
+  } else {
+    // word c holds |.| column marg + c: 2^31 - y_H
+    for (int c = 0; c < bp.np; ++c) {
+      const int l = marg + c < p.C ? marg + c : -1;
+      seed[c] = kBias32 - (uint32_t)(int64_t)yminus(l);
+      for (int m = 0; m < nh; ++m) dfull[(size_t)m * NW + c] = (uint32_t)(int64_t)(-2 * Aat(kl + m, l));
     }
   }
   if (marg) {
@@ -409,16 +465,17 @@ int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t
   ta.np = bp.np;
   ta.P = bp.P;
   ta.out = (uint32_t *)cx.table.p;
+  ta.wide = bp.wide ? 1 : 0;
   HIPCHK(launch_table(ta, st));
 
   int32_t *ub = (int32_t *)cx.units.p;
   int32_t *gbest = (int32_t *)cx.scal.p;
   unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
-  out.units = bp.nwaves;
+  out.units = bp.nunits;
   out.best = INT32_MIN;
   out.first_unit = ~0ull;
-  for (uint64_t c0 = 0; c0 < bp.nwaves; c0 += kUnitsPerChunk) {
-    const uint64_t n = std::min(kUnitsPerChunk, bp.nwaves - c0);
+  for (uint64_t c0 = 0; c0 < bp.nunits; c0 += kUnitsPerChunk) {
+    const uint64_t n = std::min(kUnitsPerChunk, bp.nunits - c0);
     HIPCHK(launch_fill_i32(gbest, INT32_MIN, 1, st));
     HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
     BlockedArgs a{};
@@ -433,7 +490,8 @@ int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t
     a.unit_best = ub;
     a.gbest = gbest;
     HIPCHK(hipEventRecord(cx.ev[4], st));
-    hipError_t e = launch_blocked(bp.np, bp.P, marg, a, st);
+    hipError_t e = bp.wide ? launch_blocked32(bp.NC, bp.P, bp.W, marg, a, st)
+                           : launch_blocked(bp.np, bp.P, marg, a, st);
     HIPCHK(hipEventRecord(cx.ev[5], st));
     if (e != hipSuccess) return fail(LNORM_E_HIP, "blocked launch failed: %s", hipGetErrorString(e));
     HIPCHK(launch_first_unit(ub, n, gbest, gfirst, st));
@@ -524,8 +582,12 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
     if (int r = run_blocked(cx, p, bp, st, p1)) return r;
     unit_e = bp.s + bp.kl;
     rr.patterns_per_lane = bp.P;
-    snprintf(rr.kernel_name, sizeof rr.kernel_name, "blocked16<np=%d,P=%d,marg=%d,s=%d>", bp.np, bp.P,
-             p.kind == KIND_LM, bp.s);
+    if (bp.wide)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "blocked32<nc=%d,P=%d,W=%d,marg=%d,s=%d>", bp.NC, bp.P, bp.W,
+               p.kind == KIND_LM, bp.s);
+    else
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "blocked16<np=%d,P=%d,marg=%d,s=%d>", bp.np, bp.P,
+               p.kind == KIND_LM, bp.s);
   } else {
     const Geometry g = geometry_for(p.base, p.R - 1, 20, o.seg_log, general);
     if (int r = run_walk_max(cx, p, pp, g, st, o.block, p1)) return r;
@@ -701,13 +763,15 @@ int lnorm_hip_selftest(void) {
   return bad;
 }
 
-int lnorm_hip_debug_table(const int32_t *A, int R, int C, int marginal, int P, uint32_t *out, int out_words) {
+int lnorm_hip_debug_table(const int32_t *A, int R, int C, int marginal, int P, int word_bits, uint32_t *out,
+                          int out_words) {
   g_err.clear();
   Problem p{A, R, C, 1, marginal, KIND_L1, 2, 0};
   if (int r = validate(p)) return r;
   lnorm_hip_opts o;
   lnorm_hip_default_opts(&o);
   o.patterns_per_lane = P;
+  o.word_bits = word_bits;
   BlockedPlan bp;
   std::string why;
   if (!blocked_plan(p, o, bp, why)) return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
@@ -718,22 +782,13 @@ int lnorm_hip_debug_table(const int32_t *A, int R, int C, int marginal, int P, u
   DevCtx &cx = g_ctx[dev];
   std::lock_guard<std::mutex> lock(cx.mu);
   if (int r = ctx_init(cx)) return r;
-  std::vector<int8_t> lo((size_t)bp.planes * bp.kl * C);
-  for (int i = 0; i < bp.kl; ++i)
-    for (int l = 0; l < C; ++l) {
-      const int32_t v = A[(size_t)i * C + l];
-      if (bp.planes == 1) {
-        lo[(size_t)i * C + l] = (int8_t)v;
-      } else {
-        const int32_t lo8 = ((v + 128) & 255) - 128;
-        lo[(size_t)i * C + l] = (int8_t)((v - lo8) / 256);
-        lo[((size_t)bp.kl + i) * C + l] = (int8_t)lo8;
-      }
-    }
+  std::vector<int8_t> lo;
+  split_planes(p, bp.kl, bp.planes, lo);
   if (int r = ensure(cx.lo, lo.size())) return r;
   if (int r = ensure(cx.table, (size_t)words * 4)) return r;
   HIPCHK(hipMemcpyAsync(cx.lo.p, lo.data(), lo.size(), hipMemcpyHostToDevice, cx.stream));
-  TableArgs ta{(const int8_t *)cx.lo.p, bp.planes, bp.kl, C, marginal, bp.np, bp.P, (uint32_t *)cx.table.p};
+  TableArgs ta{(const int8_t *)cx.lo.p, bp.planes, bp.kl, C, marginal, bp.np, bp.P, (uint32_t *)cx.table.p,
+               bp.wide ? 1 : 0};
   HIPCHK(launch_table(ta, cx.stream));
   HIPCHK(hipMemcpyAsync(out, cx.table.p, (size_t)words * 4, hipMemcpyDeviceToHost, cx.stream));
   HIPCHK(hipStreamSynchronize(cx.stream));

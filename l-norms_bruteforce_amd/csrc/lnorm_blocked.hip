@@ -110,21 +110,22 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 typedef int v16i __attribute__((ext_vector_type(16)));
 
 __global__ __launch_bounds__(64) void k_table_mfma(const int8_t *__restrict__ lo, int planes, int kl,
-                                                   int ny, int marginal, int np, int P,
+                                                   int ny, int marginal, int np, int P, int wide,
                                                    uint32_t *__restrict__ out) {
   const int lane = threadIdx.x & 63;
   const int g = lane >> 5, i32 = lane & 31;
   const int mtile = blockIdx.y, ptile = blockIdx.x;
   const int m = mtile * 32 + i32;        // output column slot of this lane's A row
   const int n = ptile * 32 + i32;        // pattern of this lane's B column
-  const int np2 = 2 * np;
+  // |.| slots: 2 per packed word, or 1 per word in the 32-bit (wide) layout
+  const int np2 = wide ? np : 2 * np;
   // matrix column feeding output slot m
   int col = -1;
   if (marginal) {
     if (m < np2) col = (m + 1 < ny) ? m + 1 : -1;
     else if (m == np2) col = 0;
   } else {
-    col = (m < ny) ? m : -1;
+    col = (m < ny && m < np2) ? m : -1;
   }
   auto plane = [&](int pl) -> v16i {
     int8_t a8[16], b8[16];
@@ -151,6 +152,16 @@ __global__ __launch_bounds__(64) void k_table_mfma(const int8_t *__restrict__ lo
   const int NW = np + (marginal ? 1 : 0);
   const int rr = n >> 6, L = n & 63;     // pattern n -> (register block, lane)
   if (rr >= P) return;
+  if (wide) {
+    // one word per column: Y + 2^31 for |.| columns, Y for the marginal column
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int mm = mtile * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
+      if (mm < np2) out[((size_t)rr * NW + mm) * 64 + L] = (uint32_t)d[r] + kBias32;
+      else if (marginal && mm == np2) out[((size_t)rr * NW + np) * 64 + L] = (uint32_t)d[r];
+    }
+    return;
+  }
 #pragma unroll
   for (int r = 0; r < 16; r += 2) {
     const int mm = mtile * 32 + 8 * (r >> 2) + 4 * g + (r & 3);
@@ -165,11 +176,182 @@ __global__ __launch_bounds__(64) void k_table_mfma(const int8_t *__restrict__ lo
 
 hipError_t launch_table(const TableArgs &a, hipStream_t st) {
   const int patterns = 64 * a.P;
-  const int mslots = 2 * a.np + (a.marginal ? 1 : 0);
+  const int mslots = (a.wide ? a.np : 2 * a.np) + (a.marginal ? 1 : 0);
   const dim3 grid((unsigned)(patterns / 32), (unsigned)((mslots + 31) / 32)), blk(64);
   hipLaunchKernelGGL(k_table_mfma, grid, blk, 0, st, a.lo, a.planes, a.kl, a.ny, a.marginal, a.np,
-                     a.P, a.out);
+                     a.P, a.wide, a.out);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// k_blocked32: the same decomposition with 32-bit words, for matrices whose
+// column sums exceed 2^15 (e.g. int16 entries).  table word = Y_L + 2^31,
+// scalar word = 2^31 - y_H, and one v_sad_u32 gives |y| + acc per column.
+// A workgroup of W waves shares the unit (high-rank range) and the low
+// patterns; wave w owns |.| words [w*NC, (w+1)*NC) in its SGPRs, so up to
+// 4 x 32 = 128 columns fit.  The waves' per-pattern partial sums meet in LDS
+// once every B steps, and each wave folds B/W of the steps into its maxima.
+// ---------------------------------------------------------------------------
+// Three waves per SIMD (<= 168 VGPRs) keep the VALU fed while a wave sits
+// at the combine barrier.
+template <int NC, int P, int W, int MARG>
+__global__ __launch_bounds__(64 * W, 3) void k_blocked32(
+    const uint32_t *__restrict__ table, const uint32_t *__restrict__ dpk,
+    const uint32_t *__restrict__ seed, const uint32_t *__restrict__ dfull, int nh, int s,
+    uint64_t unit0, uint64_t nunits, int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest,
+    uint32_t B32) {
+  constexpr int WNC = W * NC;            // |.| words of the whole group
+  constexpr int NWT = WNC + MARG;        // words per pattern (table) and per state
+  constexpr int B = 8;                   // steps per LDS combine
+  static_assert(B % W == 0, "steps per combine must split evenly over the waves");
+  __shared__ uint32_t part[W > 1 ? W * B * 64 * P : 1];
+  __shared__ int32_t wbest[W];
+  const int lane = threadIdx.x & 63;
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t gl = blockIdx.x;
+  if (gl >= nunits) return;              // uniform over the whole workgroup
+  const uint64_t g = unit0 + gl;
+  const int wbase = wv * NC;
+  const bool mw = MARG && wv == 0;       // the wave carrying the marginal column
+
+  uint32_t tab[P][NC];
+  uint32_t tabm[P];
+#pragma unroll
+  for (int r = 0; r < P; ++r) {
+#pragma unroll
+    for (int c = 0; c < NC; ++c) tab[r][c] = table[((size_t)r * NWT + wbase + c) * 64 + lane];
+    tabm[r] = mw ? table[((size_t)r * NWT + WNC) * 64 + lane] : 0u;
+  }
+
+  const uint64_t h0 = g << s;
+  const uint64_t g0 = h0 ^ (h0 >> 1);
+  const int myword = lane < NC ? wbase + lane : ((mw && lane == NC) ? WNC : -1);
+  uint32_t v = 0;
+  if (myword >= 0) {
+    v = seed[myword];
+    for (int m = 0; m < nh; ++m)
+      if ((g0 >> m) & 1) v += dfull[(size_t)m * NWT + myword];
+  }
+  uint32_t S[NC];
+#pragma unroll
+  for (int c = 0; c < NC; ++c) S[c] = (uint32_t)__builtin_amdgcn_readlane((int)v, c);
+  uint32_t Sm = MARG ? (uint32_t)__builtin_amdgcn_readlane((int)v, NC) : 0u;
+
+  int32_t best[P];
+#pragma unroll
+  for (int r = 0; r < P; ++r) best[r] = INT32_MIN;
+
+  const uint32_t nsteps = 1u << s;       // host guarantees s >= 3 (nsteps % B == 0)
+  const uint32_t nu = (uint32_t)(g & 1) << s;
+  for (uint32_t t0 = 0; t0 < nsteps; t0 += B) {
+    // not unrolled: each step's NC scalar deltas must not stay live across
+    // steps, or the state plus several steps' deltas overflow the SGPRs
+#pragma unroll 1
+    for (int b = 0; b < B; ++b) {
+      const uint32_t t = t0 + (uint32_t)b;
+      if (t > 0) {
+        const int j = __builtin_ctz(t);
+        const uint32_t up = (((nu | t) >> (j + 1)) & 1u) ^ 1u;
+        const uint32_t *dp = dpk + ((size_t)up * (uint32_t)nh + (uint32_t)j) * NWT;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) S[c] += dp[wbase + c];
+        if (mw) Sm += dp[WNC];
+      }
+#pragma unroll
+      for (int r = 0; r < P; ++r) {
+        uint32_t acc = mw ? tabm[r] + Sm : 0u;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) acc = sad32(tab[r][c], S[c], acc);
+        if constexpr (W == 1) best[r] = (int32_t)acc > best[r] ? (int32_t)acc : best[r];
+        else part[(((size_t)wv * B + b) * 64 + lane) * P + r] = acc;
+      }
+    }
+    if constexpr (W > 1) {
+      __syncthreads();
+#pragma unroll
+      for (int bb = 0; bb < B / W; ++bb) {
+        const int b = wv * (B / W) + bb;
+#pragma unroll
+        for (int r = 0; r < P; ++r) {
+          uint32_t tot = 0;
+#pragma unroll
+          for (int w2 = 0; w2 < W; ++w2) tot += part[(((size_t)w2 * B + b) * 64 + lane) * P + r];
+          best[r] = (int32_t)tot > best[r] ? (int32_t)tot : best[r];
+        }
+      }
+      __syncthreads();
+    }
+  }
+
+  int32_t bm = best[0];
+#pragma unroll
+  for (int r = 1; r < P; ++r) bm = best[r] > bm ? best[r] : bm;
+  bm = wave_max_i32(bm);
+  if constexpr (W > 1) {
+    if (lane == 0) wbest[wv] = bm;
+    __syncthreads();
+    if (wv == 0 && lane == 0) {
+      int32_t m = wbest[0];
+      for (int w2 = 1; w2 < W; ++w2) m = wbest[w2] > m ? wbest[w2] : m;
+      unit_best[gl] = m;
+      atomicMax(gbest, m);
+    }
+  } else if (lane == 0) {
+    unit_best[gl] = bm;
+    atomicMax(gbest, bm);
+  }
+}
+
+bool blocked32_supported(int nc, int P, int W, int marginal) {
+  if (!(nc == 8 || nc == 16 || nc == 24 || nc == 32)) return false;
+  if (!(P == 1 || P == 2 || P == 4) || !(W == 1 || W == 2 || W == 4)) return false;
+  if (marginal != 0 && marginal != 1) return false;
+  return P * nc <= 128;
+}
+
+template <int NC, int P, int W, int M>
+static hipError_t launch_b32(const BlockedArgs &a, hipStream_t st) {
+  hipLaunchKernelGGL((k_blocked32<NC, P, W, M>), dim3((unsigned)a.nwaves), dim3(64 * W), 0, st, a.table, a.dpk,
+                     a.seed, a.dfull, a.nh, a.s, a.wave0, a.nwaves, a.unit_best, a.gbest, kBias32);
+  return hipGetLastError();
+}
+
+template <int NC, int P, int M>
+static hipError_t launch_b32w(int W, const BlockedArgs &a, hipStream_t st) {
+  switch (W) {
+    case 1: return launch_b32<NC, P, 1, M>(a, st);
+    case 2: return launch_b32<NC, P, 2, M>(a, st);
+    case 4: return launch_b32<NC, P, 4, M>(a, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int NC, int M>
+static hipError_t launch_b32p(int P, int W, const BlockedArgs &a, hipStream_t st) {
+  switch (P) {
+    case 1: return launch_b32w<NC, 1, M>(W, a, st);
+    case 2: return launch_b32w<NC, 2, M>(W, a, st);
+    case 4:
+      if constexpr (4 * NC <= 128) return launch_b32w<NC, 4, M>(W, a, st);
+      break;
+  }
+  return hipErrorInvalidValue;
+}
+
+template <int M>
+static hipError_t launch_b32n(int nc, int P, int W, const BlockedArgs &a, hipStream_t st) {
+  switch (nc) {
+    case 8: return launch_b32p<8, M>(P, W, a, st);
+    case 16: return launch_b32p<16, M>(P, W, a, st);
+    case 24: return launch_b32p<24, M>(P, W, a, st);
+    case 32: return launch_b32p<32, M>(P, W, a, st);
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_blocked32(int nc, int P, int W, int marginal, const BlockedArgs &a, hipStream_t st) {
+  if (!blocked32_supported(nc, P, W, marginal)) return hipErrorInvalidValue;
+  return marginal ? launch_b32n<1>(nc, P, W, a, st) : launch_b32n<0>(nc, P, W, a, st);
 }
 
 // ---------------------------------------------------------------------------

@@ -61,9 +61,10 @@ struct TableArgs {
   int kl;                  // low rows (table has 2^kl patterns)
   int ny;                  // columns of the table (before packing)
   int marginal;            // LM: column 0 kept as an int32 word
-  int np;                  // packed words per pattern (excluding the marginal word)
+  int np;                  // |.| words per pattern (excluding the marginal word)
   int P;                   // patterns per lane
   uint32_t *out;           // packed table
+  int wide;                // 0: two biased 16-bit halves per word; 1: one Y + 2^31 word per column
 };
 
 // Launchers (return hipSuccess or the launch error).  `grid_hint` <= 0 means
@@ -75,6 +76,11 @@ int plain_ny_pad(Kind kind, int d, int ny);     // smallest instantiated width >
 hipError_t launch_blocked(int np, int P, int marginal, const BlockedArgs &a, hipStream_t st);
 bool blocked_supported(int np, int P, int marginal);
 int blocked_np_pad(int np, int P, int marginal);
+
+// 32-bit blocked walk: nc words per wave, W waves per workgroup (nwaves in
+// BlockedArgs counts workgroups, one unit each).
+hipError_t launch_blocked32(int nc, int P, int W, int marginal, const BlockedArgs &a, hipStream_t st);
+bool blocked32_supported(int nc, int P, int W, int marginal);
 
 hipError_t launch_table(const TableArgs &a, hipStream_t st);
 

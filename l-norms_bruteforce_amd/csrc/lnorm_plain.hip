@@ -352,7 +352,8 @@ static hipError_t launch_dary(Mode mode, const PlainArgs &a, hipStream_t st, int
 }
 
 hipError_t launch_plain(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block) {
-  if (block <= 0 || block % 64 || block > 1024) block = 256;
+  // the kernels are compiled for at most 256 threads (__launch_bounds__)
+  if (block <= 0 || block % 64 || block > 256) block = 256;
   if (!plain_supported(kind, a.d, a.ny_pad)) return hipErrorInvalidValue;
   if (kind == KIND_L1) {
     switch (a.ny_pad) {

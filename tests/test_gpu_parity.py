@@ -88,6 +88,54 @@ def test_blocked_vs_oracle(pkg, marg, P):
         assert res["kernel"] == pkg.KERNEL_BLOCKED
 
 
+@pytest.mark.parametrize("marg", [0, 1])
+@pytest.mark.parametrize("P", [1, 2, 4])
+def test_blocked32_vs_oracle(pkg, marg, P):
+    # 32-bit words: int16 entries (two int8 MFMA planes), up to 128 |.| columns
+    rng = np.random.default_rng(500 + P + 31 * marg)
+    kl = 6 + {1: 0, 2: 1, 4: 2}[P]
+    for trial in range(5):
+        R = int(rng.integers(kl + 4, kl + 9))
+        C = int(rng.integers(2 + marg, 129 + marg))
+        r = int(rng.choice([2, 300, 30000]))
+        A = _rand(rng, R, C, r)
+        print("blocked32 P=%d marg=%d trial=%d shape=%s r=%d" % (P, marg, trial, A.shape, r), flush=True)
+        res = _agree(pkg, A, 1, marg, kernel="blocked", patterns_per_lane=P, word_bits=32)
+        assert res["kernel_name"].startswith("blocked32"), res["kernel_name"]
+
+
+def test_blocked32_auto_for_int16_entries(pkg):
+    from lnorm_amd import gen
+    # int16 entries: column sums beyond 2^15, AUTO takes the 32-bit blocked kernel
+    A = gen.gen_matrix(128, 20, 30000, 105).T.copy()       # 20 x 128, 2^19 sign vectors
+    res = _agree(pkg, A, 1, 0)
+    assert res["kernel_name"].startswith("blocked32<nc=32,P=4,W=4"), res["kernel_name"]
+    res = _agree(pkg, A[:, :50].copy(), 1, 1)
+    assert res["kernel_name"].startswith("blocked32"), res["kernel_name"]
+
+
+def test_blocked32_table_matches_numpy(pkg):
+    rng = np.random.default_rng(19)
+    for marg, P, r, C in [(0, 4, 30000, 100), (1, 2, 20000, 41), (0, 1, 100, 7)]:
+        kl = 6 + {1: 0, 2: 1, 4: 2}[P]
+        A = _rand(rng, kl + 4, C, r)
+        tab = pkg.debug_table(A, marg, P, word_bits=32)
+        cols = list(range(marg, C))
+        W = 1 if len(cols) <= 32 else (2 if len(cols) <= 64 else 4)
+        NC = ((len(cols) + W - 1) // W + 7) // 8 * 8
+        NW = W * NC + marg
+        tab = tab.reshape(P, NW, 64)
+        for p in range(64 * P):
+            signs = np.array([1 if (p >> i) & 1 else -1 for i in range(kl)])
+            Y = signs @ A[:kl].astype(np.int64)
+            rr, L = divmod(p, 64)
+            for c in range(W * NC):
+                y = int(Y[cols[c]]) if c < len(cols) else 0
+                assert int(tab[rr, c, L]) == (y + 2 ** 31) & 0xFFFFFFFF, (marg, P, p, c)
+            if marg:
+                assert int(tab[rr, W * NC, L]) == int(Y[0]) & 0xFFFFFFFF
+
+
 def test_blocked_table_matches_numpy(pkg):
     rng = np.random.default_rng(9)
     for marg, P, r in [(0, 4, 127), (1, 2, 127), (0, 2, 3000), (1, 8, 20000), (0, 1, 5)]:
