@@ -28,8 +28,13 @@ INC = os.path.join(ROOT, "include")
 ARCH = os.environ.get("LNORM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 HIP_SOURCES = ["lnorm_plain.hip", "lnorm_blocked.hip", "lnorm_general.hip", "lnorm_api.hip"]
+# -simplifycfg-sink-common=false: the d-ary walks branch (uniformly) into one
+# move per (from, to) label pair; sinking those near-identical bodies into a
+# shared tail would index the label-vector array at run time and push it into
+# scratch memory (seen for d >= 4).
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", "--offload-arch=" + ARCH, "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-mllvm", "-simplifycfg-sink-common=false",
              "-I" + CSRC, "-I" + INC]
 HOST_SOURCES = ["lnorm_host.c"]
 C_FLAGS = ["-O2", "-std=c11", "-fPIC", "-Wall", "-Wextra", "-I" + INC]

@@ -161,13 +161,17 @@ struct Geometry {
   uint64_t nseg;   // segments covering the range
 };
 
+// The lane- or wave-per-segment walks.
+enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2 };
+
 // Segments for a range of base^e ranks: aim at about 2^target_log lanes.
-Geometry geometry_for(int base, int e, int target_log, int forced, bool general) {
+// The step counter of the d-ary and packed walks holds 15 base-d digits.
+Geometry geometry_for(int base, int e, int target_log, int forced, Walk walk) {
   Geometry g;
   const double lb = std::log2((double)base);
   int qmax;
-  if (general) qmax = (int)(62.0 / lb);
-  else if (base == 2) qmax = 30;
+  if (walk == WALK_GENERAL) qmax = (int)(62.0 / lb);
+  else if (walk == WALK_PLAIN && base == 2) qmax = 30;
   else qmax = std::min(15, (int)(31.0 / lb));
   int q = forced > 0 ? forced : std::max(0, e - (int)std::floor(target_log / lb));
   q = std::min(q, std::min(qmax, e));
@@ -178,18 +182,41 @@ Geometry geometry_for(int base, int e, int target_log, int forced, bool general)
 }
 
 struct Prepared {
-  int ny_pad = 0;      // plain width (0 = use the general kernel)
+  Walk walk = WALK_PLAIN;
+  int ny_pad = 0;      // kernel width in columns (packed walk: 2 x words)
   std::vector<int32_t> A, D;
 };
 
-void prepare_plain(const Problem &p, int ny_pad, Prepared &pp) {
-  pp.ny_pad = ny_pad;
-  const int W = ny_pad ? ny_pad : p.C;
+int64_t max_col_abs_sum(const Problem &p, int first_col) {
+  int64_t mx = 0;
+  for (int l = first_col; l < p.C; ++l) {
+    int64_t cs = 0;
+    for (int i = 0; i < p.R; ++i) cs += std::llabs((long long)p.A[(size_t)i * p.C + l]);
+    mx = std::max(mx, cs);
+  }
+  return mx;
+}
+
+void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
+  pp.walk = walk;
+  pp.ny_pad = walk == WALK_GENERAL ? 0 : ny_pad;
+  pp.D.clear();
+  if (walk == WALK_PLAIN16) {
+    // two columns per word, low half = even column
+    const int NW = ny_pad / 2;
+    pp.A.assign((size_t)p.R * NW, 0);
+    for (int i = 0; i < p.R; ++i)
+      for (int l = 0; l < p.C; ++l) {
+        const uint32_t v = (uint32_t)p.A[(size_t)i * p.C + l] & 0xFFFFu;
+        pp.A[(size_t)i * NW + l / 2] |= (int32_t)(v << (16 * (l & 1)));
+      }
+    return;
+  }
+  const int W = walk == WALK_PLAIN ? ny_pad : p.C;
   pp.A.assign((size_t)p.R * W, 0);
   for (int i = 0; i < p.R; ++i)
     for (int l = 0; l < p.C; ++l) pp.A[(size_t)i * W + l] = p.A[(size_t)i * p.C + l];
-  pp.D.clear();
-  if (ny_pad && p.n <= 2 && p.R > 1) {
+  if (walk == WALK_PLAIN && p.n <= 2 && p.R > 1) {
     // D[1][j] = +f A[j], D[0][j] = -f A[j]; f = 2 for L1/LM (y += 2A), 1 for L2
     const uint32_t f = p.n == 1 ? 2u : 1u;
     pp.D.assign((size_t)2 * (p.R - 1) * W, 0);
@@ -199,6 +226,15 @@ void prepare_plain(const Problem &p, int ny_pad, Prepared &pp) {
         pp.D[((size_t)(p.R - 1) + j) * W + l] = (int32_t)a;
         pp.D[(size_t)j * W + l] = (int32_t)(0u - a);
       }
+  }
+}
+
+hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const PlainArgs &a, hipStream_t st,
+                       int block) {
+  switch (pp.walk) {
+    case WALK_PLAIN: return launch_plain(p.kind, mode, a, st, block);
+    case WALK_PLAIN16: return launch_plain16(mode, a, st, block);
+    default: return launch_general(p.kind, mode, a, p.C, st);
   }
 }
 
@@ -238,8 +274,7 @@ int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometr
     a.gfirst = gfirst;
     a.state = (uint32_t *)cx.state.p;
     HIPCHK(hipEventRecord(cx.ev[4], st));
-    hipError_t e = pp.ny_pad ? launch_plain(p.kind, MODE_MAX, a, st, block)
-                             : launch_general(p.kind, MODE_MAX, a, p.C, st);
+    hipError_t e = launch_walk(p, pp, MODE_MAX, a, st, block);
     HIPCHK(hipEventRecord(cx.ev[5], st));
     if (e != hipSuccess) return fail(LNORM_E_HIP, "enumeration launch failed: %s", hipGetErrorString(e));
     HIPCHK(launch_first_unit(ub, n, gbest, gfirst, st));
@@ -262,7 +297,7 @@ int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometr
 // MODE_FIRST over the rank range [u * base^e, (u+1) * base^e).
 int run_walk_first(DevCtx &cx, const Problem &p, const Prepared &pp, uint64_t u, int e, int32_t target,
                    hipStream_t st, int block, uint64_t &rank) {
-  const Geometry g = geometry_for(p.base, e, 16, 0, pp.ny_pad == 0);
+  const Geometry g = geometry_for(p.base, e, 16, 0, pp.walk);
   unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
   HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
   PlainArgs a{};
@@ -279,8 +314,7 @@ int run_walk_first(DevCtx &cx, const Problem &p, const Prepared &pp, uint64_t u,
   a.gbest = (int32_t *)cx.scal.p;
   a.gfirst = gfirst;
   a.state = (uint32_t *)cx.state.p;
-  hipError_t err = pp.ny_pad ? launch_plain(p.kind, MODE_FIRST, a, st, block)
-                             : launch_general(p.kind, MODE_FIRST, a, p.C, st);
+  hipError_t err = launch_walk(p, pp, MODE_FIRST, a, st, block);
   if (err != hipSuccess) return fail(LNORM_E_HIP, "strategy launch failed: %s", hipGetErrorString(err));
   unsigned long long hf;
   HIPCHK(hipMemcpyAsync(&hf, gfirst, sizeof hf, hipMemcpyDeviceToHost, st));
@@ -373,7 +407,8 @@ bool blocked_plan(const Problem &p, const lnorm_hip_opts &o, BlockedPlan &bp, st
     if (nabs > 128) { why = "more than 128 |.| columns"; return false; }
     const int W = nabs <= 32 ? 1 : (nabs <= 64 ? 2 : 4);
     const int NC = ((nabs + W - 1) / W + 7) / 8 * 8;
-    if (P == 0) P = 4 * NC <= 128 ? 4 : 2;
+    // measured on 128 columns (W = 4): P = 2 and P = 4 within 1%, P = 1 30% slower
+    if (P == 0) P = W == 4 ? 2 : (4 * NC <= 128 ? 4 : 2);
     if (!blocked32_supported(NC, P, W, marg)) { why = "no 32-bit blocked instantiation for this width / P"; return false; }
     bp.W = W;
     bp.NC = NC;
@@ -553,18 +588,27 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
   int kernel = o.kernel;
   const bool can_block = (kernel == LNORM_KERNEL_AUTO || kernel == LNORM_KERNEL_BLOCKED) && blocked_plan(p, o, bp, why);
   const int ny_plain = plain_ny_pad(p.kind, p.n, p.C);
+  const bool fits16 = p.n >= 2 && max_col_abs_sum(p, 0) <= 32767;
+  const int w16 = fits16 ? plain16_words(p.n, p.C) : 0;
   if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
     return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
   if (kernel == LNORM_KERNEL_PLAIN && !ny_plain)
     return fail(LNORM_E_UNSUPPORTED, "no plain instantiation for %d columns at n = %d", p.C, p.n);
-  if (kernel == LNORM_KERNEL_AUTO) kernel = can_block ? LNORM_KERNEL_BLOCKED : (ny_plain ? LNORM_KERNEL_PLAIN : LNORM_KERNEL_GENERAL);
-  if (kernel != LNORM_KERNEL_PLAIN && kernel != LNORM_KERNEL_BLOCKED && kernel != LNORM_KERNEL_GENERAL)
+  if (kernel == LNORM_KERNEL_PLAIN16 && !w16)
+    return fail(LNORM_E_UNSUPPORTED, "packed 16-bit walk needs n >= 2, column sums <= 32767 and an instantiated "
+                "width (%d columns at n = %d)", p.C, p.n);
+  if (kernel == LNORM_KERNEL_AUTO)
+    kernel = can_block ? LNORM_KERNEL_BLOCKED
+                       : (w16 ? LNORM_KERNEL_PLAIN16 : (ny_plain ? LNORM_KERNEL_PLAIN : LNORM_KERNEL_GENERAL));
+  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_PLAIN16)
     return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
 
-  // the plain (or general) walk is needed for the enumeration or the strategy
+  // the walk that enumerates (non-blocked kernels) and recovers the strategy (all)
   Prepared pp;
-  prepare_plain(p, kernel == LNORM_KERNEL_GENERAL ? 0 : ny_plain, pp);
-  const bool general = pp.ny_pad == 0;
+  if (kernel == LNORM_KERNEL_PLAIN16) prepare_walk(p, WALK_PLAIN16, 2 * w16, pp);
+  else if (kernel == LNORM_KERNEL_GENERAL || !ny_plain) prepare_walk(p, WALK_GENERAL, 0, pp);
+  else prepare_walk(p, WALK_PLAIN, ny_plain, pp);
+  const bool general = pp.walk == WALK_GENERAL;
   const int nvec = (p.kind == KIND_L1 || p.kind == KIND_LM) ? 1 : p.n;
   if (int r = ensure(cx.A, pp.A.size() * 4)) return r;
   if (int r = ensure(cx.D, pp.D.size() * 4)) return r;
@@ -589,11 +633,13 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "blocked16<np=%d,P=%d,marg=%d,s=%d>", bp.np, bp.P,
                p.kind == KIND_LM, bp.s);
   } else {
-    const Geometry g = geometry_for(p.base, p.R - 1, 20, o.seg_log, general);
+    const Geometry g = geometry_for(p.base, p.R - 1, 20, o.seg_log, pp.walk);
     if (int r = run_walk_max(cx, p, pp, g, st, o.block, p1)) return r;
     unit_e = g.q;
     if (general)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "general<kind=%d,d=%d,q=%d>", (int)p.kind, p.n, g.q);
+    else if (pp.walk == WALK_PLAIN16)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain16<d=%d,words=%d,q=%d>", p.n, pp.ny_pad / 2, g.q);
     else
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain<kind=%d,d=%d,ny=%d,q=%d>", (int)p.kind, p.n,
                pp.ny_pad, g.q);

@@ -70,6 +70,9 @@ struct TableArgs {
 // Launchers (return hipSuccess or the launch error).  `grid_hint` <= 0 means
 // choose automatically.
 hipError_t launch_plain(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block);
+// Packed 16-bit d-ary walk (d = 2..10): A holds R x (ny_pad/2) packed words.
+hipError_t launch_plain16(Mode mode, const PlainArgs &a, hipStream_t st, int block);
+int plain16_words(int d, int ny);               // packed width instantiated for ny columns, 0 if none
 bool plain_supported(Kind kind, int d, int ny_pad);
 int plain_ny_pad(Kind kind, int d, int ny);     // smallest instantiated width >= ny, 0 if none
 

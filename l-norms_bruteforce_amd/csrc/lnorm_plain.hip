@@ -263,6 +263,205 @@ __global__ __launch_bounds__(256) void k_plain_dary(
 }
 
 // ---------------------------------------------------------------------------
+// Packed 16-bit d-ary family (d = 2..10): two columns per VGPR, each half
+// biased by 2^15.  A row move is one v_pk_sub_u16 and one v_pk_add_u16 per
+// two columns, a re-sum one v_sad_u16 per two columns: half the VALU work of
+// the 32-bit walk.  Exact whenever every column's absolute sum is <= 2^15 - 1
+// (the host checks), since each label sum t_c,l is bounded by it.  d = 2 is
+// L_2: the reflected binary code P = 0,1,1,0 is the BRGC (L_OpenMP.c:101).
+// ---------------------------------------------------------------------------
+typedef unsigned short u16x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ uint32_t pk_add(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) + __builtin_bit_cast(u16x2, b));
+}
+__device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) {
+  return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
+}
+
+template <int F, int T, int D, int NW>
+__device__ __forceinline__ void apply_move16(uint32_t (&t)[D][NW], uint32_t (&S)[D],
+                                             const uint32_t *__restrict__ row, uint32_t B) {
+  uint32_t sf = 0, st = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t a = row[w];
+    t[F][w] = pk_sub(t[F][w], a);
+    t[T][w] = pk_add(t[T][w], a);
+    sf = sad16(t[F][w], B, sf);
+    st = sad16(t[T][w], B, st);
+  }
+  S[F] = sf;
+  S[T] = st;
+}
+
+template <int C, int D, int NW>
+struct Mover16 {
+  __device__ __forceinline__ static void run(uint32_t (&t)[D][NW], uint32_t (&S)[D],
+                                             const uint32_t *__restrict__ row, int from, int up,
+                                             uint32_t B) {
+    if (from == C) {
+      if (up) {
+        if constexpr (C + 1 < D) apply_move16<C, C + 1, D, NW>(t, S, row, B);
+      } else {
+        if constexpr (C > 0) apply_move16<C, C - 1, D, NW>(t, S, row, B);
+      }
+    } else {
+      if constexpr (C + 1 < D) Mover16<C + 1, D, NW>::run(t, S, row, from, up, B);
+    }
+  }
+};
+
+// Ap: R x NW packed rows, word w = (A[i][2w+1] & 0xFFFF) << 16 | (A[i][2w] & 0xFFFF).
+template <int D, int NW, int MODE>
+__global__ __launch_bounds__(256) void k_plain16_dary(
+    const uint32_t *__restrict__ Ap, int R, int q, uint64_t seg0, uint64_t nseg, int32_t target,
+    int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest,
+    unsigned long long *__restrict__ gfirst, uint32_t B) {
+  const int lane = wave_lane();
+  const uint64_t w = global_wave();
+  if (((w >> 1) << 7) >= nseg) return;
+  const uint64_t local = lane_segment(w, lane);
+  const bool active = local < nseg;
+  const uint64_t seg = seg0 + local;
+  const uint32_t pseg = (uint32_t)__builtin_amdgcn_readfirstlane((int)((seg0 ^ w) & 1));
+
+  uint32_t par_mask;
+  if (D & 1) par_mask = pseg ? ((1u << q) - 1u) : 0u;
+  else par_mask = (q > 0 && pseg) ? (1u << (q - 1)) : 0u;
+
+  uint32_t t[D][NW];
+#pragma unroll
+  for (int c = 0; c < D; ++c)
+#pragma unroll
+    for (int l = 0; l < NW; ++l) t[c][l] = pk_add(kBias16x2, c == 0 ? Ap[(size_t)(R - 1) * NW + l] : 0u);
+  uint64_t x = seg;
+  for (int i = 0; i < R - 1; ++i) {
+    int lab;
+    if (i < q) {
+      lab = ((par_mask >> i) & 1) ? D - 1 : 0;
+    } else {
+      const uint32_t r = (uint32_t)(x % (uint64_t)(2 * D));
+      lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
+      x /= (uint64_t)D;
+    }
+    const uint32_t *row = Ap + (size_t)i * NW;
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+      for (int l = 0; l < NW; ++l) t[c][l] = pk_add(t[c][l], (lab == c) ? row[l] : 0u);
+  }
+  uint32_t S[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    uint32_t s = 0;
+#pragma unroll
+    for (int l = 0; l < NW; ++l) s = sad16(t[c][l], B, s);
+    S[c] = s;
+  }
+
+  int32_t best = INT32_MIN;
+  uint32_t first = 0xFFFFFFFFu;
+  auto eval = [&](uint32_t tt) {
+    uint32_t acc = 0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc += S[c];
+    if constexpr (MODE == MODE_MAX) {
+      best = (int32_t)acc > best ? (int32_t)acc : best;
+    } else {
+      const uint32_t cnd = ((int32_t)acc == target) ? tt : 0xFFFFFFFFu;
+      first = cnd < first ? cnd : first;
+    }
+  };
+
+  uint64_t biaspat = 0;
+  for (int i = 0; i < q; ++i) biaspat |= (uint64_t)(16 - D) << (4 * i);
+  uint64_t cnt = biaspat;
+  uint32_t pm = par_mask;
+  uint32_t nsteps = 1;
+  for (int i = 0; i < q; ++i) nsteps *= (uint32_t)D;
+
+  eval(0);
+  for (uint32_t tt = 1; tt < nsteps; ++tt) {
+    const int j = __builtin_ctzll(~cnt) >> 2;
+    const uint64_t low = (1ull << (4 * j)) - 1ull;
+    cnt = (cnt + 1ull) | (biaspat & low);
+    const int v = (int)((cnt >> (4 * j)) & 15ull) - (16 - D);
+    const int p = (pm >> j) & 1;
+    const int from = p ? D - v : v - 1;
+    pm ^= (1u << j) - 1u;
+    Mover16<0, D, NW>::run(t, S, Ap + (size_t)j * NW, from, p ? 0 : 1, B);
+    eval(tt);
+  }
+
+  if constexpr (MODE == MODE_MAX) {
+    if (active) unit_best[local] = best;
+    const int32_t m = wave_max_i32(active ? best : INT32_MIN);
+    if (lane == 0) atomicMax(gbest, m);
+  } else {
+    const uint64_t r = (active && first != 0xFFFFFFFFu) ? (seg * (uint64_t)nsteps + first) : ~0ull;
+    const uint64_t m = wave_min_u64(r);
+    if (lane == 0 && m != ~0ull) atomicMin(gfirst, (unsigned long long)m);
+  }
+}
+
+// largest packed width instantiated per d (d * NW <= 160 words of state)
+static constexpr int dary16_max_words(int d) {
+  return d == 2 ? 64 : d == 3 ? 48 : (d == 4 || d == 5) ? 32 : d == 6 ? 24 : (d >= 7 && d <= 10) ? 16 : 0;
+}
+static const int kPkWords[] = {4, 8, 12, 16, 20, 24, 28, 32, 40, 48, 64};
+
+int plain16_words(int d, int ny) {
+  const int need = (ny + 1) / 2;
+  const int mx = dary16_max_words(d);
+  for (int w : kPkWords) if (w >= need && w <= mx) return w;
+  return 0;
+}
+
+template <int D, int NW>
+static hipError_t launch_p16_nw(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
+  const uint64_t waves = 2 * ((a.nseg + 127) / 128);
+  const int wpb = block / 64;
+  const dim3 grid((unsigned)((waves + wpb - 1) / wpb)), blk(block);
+  const uint32_t *Ap = (const uint32_t *)a.A;
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_plain16_dary<D, NW, MODE_MAX>), grid, blk, 0, st, Ap, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst, kBias16x2);
+  else
+    hipLaunchKernelGGL((k_plain16_dary<D, NW, MODE_FIRST>), grid, blk, 0, st, Ap, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst, kBias16x2);
+  return hipGetLastError();
+}
+
+template <int D>
+static hipError_t launch_p16(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
+  switch (a.ny_pad / 2) {
+#define X(n) case n: if constexpr (n <= dary16_max_words(D)) return launch_p16_nw<D, n>(mode, a, st, block); break;
+    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(64)
+#undef X
+    default: break;
+  }
+  return hipErrorInvalidValue;
+}
+
+hipError_t launch_plain16(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
+  if (block <= 0 || block % 64 || block > 256) block = 256;
+  if (!plain16_words(a.d, a.ny_pad) || plain16_words(a.d, a.ny_pad) * 2 != a.ny_pad) return hipErrorInvalidValue;
+  switch (a.d) {
+    case 2: return launch_p16<2>(mode, a, st, block);
+    case 3: return launch_p16<3>(mode, a, st, block);
+    case 4: return launch_p16<4>(mode, a, st, block);
+    case 5: return launch_p16<5>(mode, a, st, block);
+    case 6: return launch_p16<6>(mode, a, st, block);
+    case 7: return launch_p16<7>(mode, a, st, block);
+    case 8: return launch_p16<8>(mode, a, st, block);
+    case 9: return launch_p16<9>(mode, a, st, block);
+    case 10: return launch_p16<10>(mode, a, st, block);
+    default: break;
+  }
+  return hipErrorInvalidValue;
+}
+
+// ---------------------------------------------------------------------------
 // instantiation tables and launchers
 // ---------------------------------------------------------------------------
 #define LN_BIN_WIDTHS(X) X(8) X(16) X(24) X(32) X(40) X(48) X(56) X(64) X(80) X(96) X(112) X(128)

@@ -43,11 +43,11 @@ def test_golden_vectors_all_kernels(pkg, golden):
         if p.R == 0:
             continue
         want_line = case["stdout"].strip().splitlines()[-1]
-        for kernel in ("auto", "plain", "general", "blocked"):
+        for kernel in ("auto", "plain", "general", "blocked", "plain16"):
             try:
                 norm, strat, res = pkg.calc_lnorm(A, p.n, p.marginal, kernel=kernel)
             except pkg.LnormError as e:
-                assert e.code == -5 and kernel in ("plain", "blocked"), (case["name"], kernel, str(e))
+                assert e.code == -5 and kernel in ("plain", "blocked", "plain16"), (case["name"], kernel, str(e))
                 continue
             line, fname, lines = ho.result_lines(p, norm, strat)
             assert line == want_line, (case["name"], kernel, res["kernel_name"])
@@ -67,8 +67,12 @@ def test_random_vs_oracle_plain_and_general(pkg, n, marg):
         A = _rand(rng, R, C, int(rng.choice([1, 2, 9, 100, 5000])))
         print("random n=%d marg=%d trial=%d shape=%s" % (n, marg, trial, A.shape), flush=True)
         _agree(pkg, A, n, marg, kernel="general")
-        if pkg.KERNELS and C <= {1: 128, 2: 64, 3: 40, 4: 32, 5: 24, 6: 24}.get(n, 16):
+        if C <= {1: 128, 2: 64, 3: 40, 4: 32, 5: 24, 6: 24}.get(n, 16):
             _agree(pkg, A, n, marg, kernel="plain")
+        words = {2: 64, 3: 48, 4: 32, 5: 32, 6: 24}.get(n, 16)
+        if n >= 2 and C <= 2 * words and int(np.abs(A).sum(axis=0).max()) <= 32767:
+            res = _agree(pkg, A, n, marg, kernel="plain16")
+            assert res["kernel_name"].startswith("plain16")
 
 
 @pytest.mark.parametrize("marg", [0, 1])
@@ -109,7 +113,7 @@ def test_blocked32_auto_for_int16_entries(pkg):
     # int16 entries: column sums beyond 2^15, AUTO takes the 32-bit blocked kernel
     A = gen.gen_matrix(128, 20, 30000, 105).T.copy()       # 20 x 128, 2^19 sign vectors
     res = _agree(pkg, A, 1, 0)
-    assert res["kernel_name"].startswith("blocked32<nc=32,P=4,W=4"), res["kernel_name"]
+    assert res["kernel_name"].startswith("blocked32<nc=32,P=2,W=4"), res["kernel_name"]
     res = _agree(pkg, A[:, :50].copy(), 1, 1)
     assert res["kernel_name"].startswith("blocked32"), res["kernel_name"]
 
@@ -176,8 +180,13 @@ def test_segment_size_independence(pkg):
     B = _rand(rng, 9, 7, 3)
     want3 = oracle.calc_lnorm(B, 3, 0, 8)
     for s in (1, 2, 5, 8):
-        norm, strat, res = pkg.calc_lnorm(B, 3, 0, kernel="plain", seg_log=s)
-        assert (norm, strat, res["best_rank"]) == want3
+        for k in ("plain", "plain16"):
+            norm, strat, res = pkg.calc_lnorm(B, 3, 0, kernel=k, seg_log=s)
+            assert (norm, strat, res["best_rank"]) == want3
+    want2 = oracle.calc_lnorm(A, 2, 0, 8)
+    for s in (1, 4, 9, 15):
+        norm, strat, res = pkg.calc_lnorm(A, 2, 0, kernel="plain16", seg_log=s)
+        assert (norm, strat, res["best_rank"]) == want2
 
 
 def test_errors(pkg):
@@ -220,10 +229,10 @@ def test_workloads_vs_oracle(pkg):
         _agree(pkg, A, 1, 0, kernel=k)
     # L_2 and L_3 of the 18 x 32 file (3^17 labelings at d = 3)
     B = gen.gen_matrix(18, 32, 50, 103)
-    print("cfg3 L2", flush=True)
-    _agree(pkg, B, 2, 0)
-    print("cfg3 L3", flush=True)
-    _agree(pkg, B, 3, 0)
+    for k in ("auto", "plain", "plain16"):
+        print("cfg3 L2/L3", k, flush=True)
+        _agree(pkg, B, 2, 0, kernel=k)
+        _agree(pkg, B, 3, 0, kernel=k)
 
 
 def test_blocked_equals_plain_large(pkg):
