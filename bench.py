@@ -133,6 +133,48 @@ def cpu_baseline(gen, rows: int = 64, r: int = 127, seed: int = 104, budget_cols
             "sample": sample + "; wall %.2f s; L1 = %d" % (dt, norm), "seconds": dt}
 
 
+def batch_line(pkg, gen, steps: int) -> dict:
+    """The mixed batch workload (4096 seeded files, m x n with m in [12, 30]
+    rows, n in [8, 64] columns, d = 1/2/3 at 60/25/15 %, a quarter with
+    duplicated rows/columns and 'y', a tenth of d = 1 with 'm'; d = 3 files
+    keep at most 20 rows) through the host library's reader and
+    preprocessing, then one lnorm_hip_calc_batch_ex call per step: one
+    persistent launch per kernel instantiation, replayed as a HIP graph.
+    Every replay must equal the eager (uncaptured) run, item for item."""
+    from lnorm_amd import host
+    work = gen.batch_workload(4096)
+    probs, per_d, alg_ops = [], {}, 0
+    for w in work:
+        p, _, exited = host.load(w["text"], w["d"], w["stat"], w["marginal"])
+        if exited:
+            raise RuntimeError("batch item %d rejected by the host pipeline" % w["seed"])
+        probs.append((p.matrix_laid_out(), p.n, p.marginal))
+        per_d[p.n] = per_d.get(p.n, 0) + p.total
+        # algorithmic int32 ops per row-update: 2 for L1/LM, d + 2 for L_d
+        alg_ops += p.total * p.C * (2 if p.n == 1 else p.n + 2)
+        host.free(p)
+    eager, st0 = pkg.calc_batch(probs, stats=True)
+    if any(s != 0 for _, _, s in eager):
+        raise RuntimeError("batch item failed")
+    times, dev = [], []
+    for _ in range(max(1, steps)):
+        t = time.perf_counter()
+        out, st = pkg.calc_batch(probs, stats=True, batch_graph=1)
+        times.append(time.perf_counter() - t)
+        dev.append(st["ms_device"])
+        if out != eager:
+            raise RuntimeError("graph replay differs from the eager batch")
+    lab = st["labelings"]
+    wall = sum(times) / len(times)
+    msd = sum(dev) / len(dev)
+    return {"items": len(probs), "groups": st["groups"], "batched": st["batched"], "labelings": lab,
+            "labelings_by_d": {str(k): v for k, v in sorted(per_d.items())},
+            "value": lab / wall, "unit": "labelings/s", "ms_per_step": 1e3 * wall, "ms_device": msd,
+            "device_labelings_per_s": lab / (msd / 1e3), "eager_ms_device": st0["ms_device"],
+            "alg_TOPs": alg_ops / (msd / 1e3) / 1e12, "alg_valu_frac": alg_ops / (msd / 1e3) / VALU_PEAK,
+            "graph_replayed": st["graph_replayed"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,6 +263,7 @@ def main():
         line["int16_128x40"] = {"blocked": s3["value"], "plain": s4["value"], "speedup": s3["value"] / s4["value"],
                                 "blocked_valu_frac": s3["valu_frac"], "plain_valu_frac": s4["valu_frac"],
                                 "kernel": s3["kernel_name"], "norm": s3["norm"]}
+        line["batch_4096"] = batch_line(pkg, gen, max(1, a.steps // 2))
     if not a.no_cpu:
         cb = cpu_baseline(gen)
         line["cpu_baseline"] = {k2: cb[k2] for k2 in ("value", "unit", "cores", "kind", "sample")}

@@ -50,6 +50,9 @@ typedef struct lnorm_hip_opts {
     int32_t device;             /* HIP device ordinal; -1 = current device */
     int32_t word_bits;          /* blocked: 16 (two halves per word) or 32; 0 = 16 when column sums allow */
     void *stream;               /* hipStream_t to run on; NULL = the library's own stream */
+    int32_t batch_graph;        /* batches: capture the launch sequence in a HIP graph and replay it
+                                   while the batch's structure (groups, sizes) stays the same */
+    int32_t autotune;           /* nonzero: use launch shapes chosen by lnorm_hip_autotune (cached) */
 } lnorm_hip_opts;
 
 typedef struct lnorm_hip_result {
@@ -101,8 +104,27 @@ typedef struct lnorm_hip_item {
 } lnorm_hip_item;
 
 /* Computes every item; returns LNORM_OK when all items succeeded, else the
- * first failing item's code (every item's own status is filled in). */
+ * first failing item's code (every item's own status is filled in).
+ * Items that fit the batched walks (at most 2^32 labelings; n = 1 with at
+ * most 128 columns; n >= 2 with column sums <= 32767) run together: one
+ * persistent launch per kernel
+ * instantiation for the enumeration and one for the strategies, with no host
+ * synchronisation in between (opts->batch_graph replays it as a HIP graph).
+ * Other items are computed one by one.  Results equal lnorm_hip_calc_Lnorm's. */
 int lnorm_hip_calc_batch(lnorm_hip_item *items, int count, const lnorm_hip_opts *opts);
+
+typedef struct lnorm_hip_batch_stats {
+    int32_t items, batched, fallback, groups;
+    int32_t graph_replayed;     /* 1 when an already captured graph was replayed */
+    int32_t reserved;
+    uint64_t labelings;         /* sum of total_num_to_calc over the batched items */
+    double ms_device;           /* HIP-event time of the batched launch sequence */
+    double ms_total;            /* host wall time of the call */
+} lnorm_hip_batch_stats;
+
+/* lnorm_hip_calc_batch with statistics (stats may be NULL). */
+int lnorm_hip_calc_batch_ex(lnorm_hip_item *items, int count, const lnorm_hip_opts *opts,
+                            lnorm_hip_batch_stats *stats);
 
 /* Fills opts with the defaults (AUTO kernel, strategy on, current device). */
 void lnorm_hip_default_opts(lnorm_hip_opts *opts);

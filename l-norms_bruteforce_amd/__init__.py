@@ -32,7 +32,7 @@ class Opts(ctypes.Structure):
                 ("seg_log", ctypes.c_int32), ("patterns_per_lane", ctypes.c_int32),
                 ("wave_log2", ctypes.c_int32), ("block", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("word_bits", ctypes.c_int32),
-                ("stream", ctypes.c_void_p)]
+                ("stream", ctypes.c_void_p), ("batch_graph", ctypes.c_int32), ("autotune", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):
@@ -49,6 +49,15 @@ class Result(ctypes.Structure):
         return d
 
 
+class BatchStats(ctypes.Structure):
+    _fields_ = [("items", ctypes.c_int32), ("batched", ctypes.c_int32), ("fallback", ctypes.c_int32),
+                ("groups", ctypes.c_int32), ("graph_replayed", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("labelings", ctypes.c_uint64), ("ms_device", ctypes.c_double), ("ms_total", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class Item(ctypes.Structure):
     _fields_ = [("mtx_as_vec", ctypes.POINTER(ctypes.c_int32)),
                 ("iRows", ctypes.c_int32), ("iCols", ctypes.c_int32),
@@ -61,7 +70,7 @@ class Item(ctypes.Structure):
 HIP_SYMBOLS = ["lnorm_hip_calc_Lnorm", "lnorm_hip_calc_ex", "lnorm_hip_calc_batch",
                "lnorm_hip_default_opts", "lnorm_hip_last_error", "lnorm_hip_abi_version",
                "lnorm_hip_device_count", "lnorm_hip_release", "lnorm_hip_selftest",
-               "lnorm_hip_debug_table"]
+               "lnorm_hip_debug_table", "lnorm_hip_calc_batch_ex"]
 
 _hip = None
 
@@ -88,6 +97,9 @@ def hip_lib():
                                         ctypes.POINTER(Opts), i32p, ctypes.POINTER(Result)]
         L.lnorm_hip_calc_batch.restype = ctypes.c_int
         L.lnorm_hip_calc_batch.argtypes = [ctypes.POINTER(Item), ctypes.c_int, ctypes.POINTER(Opts)]
+        L.lnorm_hip_calc_batch_ex.restype = ctypes.c_int
+        L.lnorm_hip_calc_batch_ex.argtypes = [ctypes.POINTER(Item), ctypes.c_int, ctypes.POINTER(Opts),
+                                              ctypes.POINTER(BatchStats)]
         L.lnorm_hip_default_opts.restype = None
         L.lnorm_hip_default_opts.argtypes = [ctypes.POINTER(Opts)]
         L.lnorm_hip_last_error.restype = ctypes.c_char_p
@@ -156,8 +168,10 @@ def calc_lnorm_simple(A, n: int, marginal: int = 0, blocks: int = 512, threads: 
     return int(norm.value), [int(v) for v in strat[: R - 1]]
 
 
-def calc_batch(problems, want_strategy=True, **opts):
-    """problems: iterable of (A, n, marginal).  Returns [(Lnorm, strategy, status)]."""
+def calc_batch(problems, want_strategy=True, stats=False, **opts):
+    """problems: iterable of (A, n, marginal).  Returns [(Lnorm, strategy, status)]
+    (and the batch statistics when stats=True)."""
+    problems = list(problems)
     mats = [_i32(A) for A, _, _ in problems]
     strats = [np.zeros(max(m.shape[0] - 1, 1), dtype=np.int32) for m in mats]
     items = (Item * len(mats))()
@@ -168,12 +182,15 @@ def calc_batch(problems, want_strategy=True, **opts):
         it.n, it.marginal = n, marg
         it.strategy = s.ctypes.data_as(i32p) if want_strategy else None
     o = default_opts(**opts)
-    hip_lib().lnorm_hip_calc_batch(items, len(mats), ctypes.byref(o))
+    st = BatchStats()
+    rc = hip_lib().lnorm_hip_calc_batch_ex(items, len(mats), ctypes.byref(o), ctypes.byref(st))
+    if rc == -4:
+        _check(rc)
     out = []
     for it, m, s in zip(items, mats, strats):
         out.append((int(it.Lnorm), [int(v) for v in s[: m.shape[0] - 1]] if want_strategy else None,
                     int(it.status)))
-    return out
+    return (out, st.as_dict()) if stats else out
 
 
 def selftest() -> int:

@@ -66,6 +66,10 @@ struct DevCtx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   Buf A, D, units, scal, table, dpk, seed, dfull, lo, state;
+  // batch buffers and the captured launch sequence
+  Buf bA, bD, bUnits, bScal, bTasks, bCnt;
+  hipGraphExec_t bGraph = nullptr;
+  std::vector<uint64_t> bGraphSig;
   bool init = false;
 };
 
@@ -674,6 +678,272 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
   return LNORM_OK;
 }
 
+// ---- batches ------------------------------------------------------------------
+// Items that fit a batched walk (L1/LM with at most 128 columns; d >= 2 with
+// column sums <= 2^15 - 1, packed) run in one persistent launch per
+// instantiation; the rest go through calc_impl one by one.
+struct BItem {
+  int idx;             // position in the caller's array
+  Problem p;
+  Prepared pp;
+  Geometry g1, g2;     // phase-1 segments; phase-2 sub-segments of one unit
+  double work;         // labelings x columns: longest first within a group
+  size_t a_off = 0, d_off = 0;
+  uint64_t ub_off = 0;
+};
+
+struct BGroup {
+  Walk walk;
+  Kind kind;
+  int d, width;
+  std::vector<int> members;   // indices into the BItem vector, longest first
+  uint64_t ntasks1 = 0, ntasks2 = 0;
+  int first = 0;              // position of the group's first item in task order
+};
+
+uint64_t waves_for(uint64_t nseg) { return 2 * ((nseg + 127) / 128); }
+
+// Items above 2^32 labelings fill the chip on their own; they take the
+// single-problem path (blocked MFMA kernel for L1/LM, chunked launches).
+constexpr uint64_t kBatchMaxTotal = 1ull << 32;
+
+// Phase-1 segments of a batched item: at least the 128 lane segments of the
+// two waves one task pair covers, and at most ~2^14 ranks per lane so a task
+// stays short next to the others (balance over the persistent grid).
+Geometry batch_geometry(int base, int e, int forced, Walk walk) {
+  if (forced > 0) return geometry_for(base, e, 63, forced, walk);
+  const double lb = std::log2((double)base);
+  Geometry g = geometry_for(base, e, 7, 0, walk);
+  const int qcap = std::max(0, (int)std::floor(14.0 / lb));
+  if (g.q > qcap) g = geometry_for(base, e, 63, qcap, walk);
+  return g;
+}
+
+int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_hip_batch_stats *stats) {
+  const auto t_start = std::chrono::steady_clock::now();
+  int first_err = 0;
+  std::vector<BItem> bis;
+  std::vector<int> fallback;
+  for (int i = 0; i < count; ++i) {
+    lnorm_hip_item &it = items[i];
+    it.Lnorm = 0;
+    it.status = LNORM_E_HIP;   // until this item's result is in
+    BItem b;
+    b.idx = i;
+    b.p = Problem{it.mtx_as_vec, it.iRows, it.iCols, it.n, it.marginal, KIND_L1, 2, 0};
+    if (int r = validate(b.p)) {
+      it.status = r;
+      if (!first_err) first_err = r;
+      continue;
+    }
+    Walk walk;
+    int width;
+    if (b.p.n >= 2) {
+      const int w16 = max_col_abs_sum(b.p, 0) <= 32767 ? plain16_words(b.p.n, b.p.C) : 0;
+      if (!w16) { fallback.push_back(i); continue; }
+      walk = WALK_PLAIN16;
+      width = 2 * w16;
+    } else {
+      width = plain_ny_pad(b.p.kind, 1, b.p.C);
+      if (!width) { fallback.push_back(i); continue; }
+      walk = WALK_PLAIN;
+    }
+    if (b.p.total > kBatchMaxTotal) { fallback.push_back(i); continue; }
+    prepare_walk(b.p, walk, width, b.pp);
+    b.g1 = batch_geometry(b.p.base, b.p.R - 1, o.seg_log, walk);
+    b.g2 = geometry_for(b.p.base, b.g1.q, 8, 0, walk);
+    b.work = (double)b.p.total * b.p.C;
+    bis.push_back(std::move(b));
+  }
+
+  // groups by instantiation, longest item first within each
+  std::vector<BGroup> groups;
+  for (int k = 0; k < (int)bis.size(); ++k) {
+    const BItem &b = bis[k];
+    BGroup *g = nullptr;
+    for (auto &gg : groups)
+      if (gg.walk == b.pp.walk && gg.kind == b.p.kind && gg.d == b.p.n && gg.width == b.pp.ny_pad) g = &gg;
+    if (!g) {
+      groups.push_back(BGroup{b.pp.walk, b.p.kind, b.p.n, b.pp.ny_pad, {}, 0, 0, 0});
+      g = &groups.back();
+    }
+    g->members.push_back(k);
+  }
+  std::vector<int> order;      // task order: group after group
+  for (auto &g : groups) {
+    std::stable_sort(g.members.begin(), g.members.end(), [&](int a, int b) { return bis[a].work > bis[b].work; });
+    g.first = (int)order.size();
+    for (int k : g.members) order.push_back(k);
+  }
+  const int nb = (int)order.size();
+
+  double ms_dev = 0;
+  bool replayed = false;
+  if (nb > 0) {
+    int dev = 0;
+    if (int r = select_device(o, dev)) return r;
+    DevCtx &cx = g_ctx[dev];
+    std::lock_guard<std::mutex> lock(cx.mu);
+    if (int r = ctx_init(cx)) return r;
+    hipStream_t st = o.stream ? (hipStream_t)o.stream : cx.stream;
+    // pools
+    size_t aw = 0, dw = 0;
+    uint64_t uw = 0;
+    for (int k : order) {
+      BItem &b = bis[k];
+      b.a_off = aw;
+      aw += b.pp.A.size();
+      b.d_off = dw;
+      dw += b.pp.D.size();
+      b.ub_off = uw;
+      uw += b.g1.nseg;
+    }
+    if (int r = ensure(cx.bA, aw * 4)) return r;
+    if (int r = ensure(cx.bD, dw * 4)) return r;
+    if (int r = ensure(cx.bUnits, uw * 4)) return r;
+    if (int r = ensure(cx.bScal, (size_t)nb * 24)) return r;
+    if (int r = ensure(cx.bTasks, (size_t)2 * nb * sizeof(BatchTask))) return r;
+    if (int r = ensure(cx.bCnt, (size_t)2 * groups.size() * 4)) return r;
+    int32_t *gbest = (int32_t *)cx.bScal.p;
+    unsigned long long *gfirst = (unsigned long long *)((char *)cx.bScal.p + (size_t)nb * 8);
+    unsigned long long *funit = gfirst + nb;
+    BatchTask *t1 = (BatchTask *)cx.bTasks.p, *t2 = t1 + nb;
+    unsigned int *cnt = (unsigned int *)cx.bCnt.p;
+    std::vector<int32_t> hA(aw), hD(dw);
+    std::vector<BatchTask> h1(nb), h2(nb);
+    for (auto &g : groups) {
+      uint64_t task1 = 0, task2 = 0;
+      for (int m = 0; m < (int)g.members.size(); ++m) {
+        const int pos = g.first + m;
+        BItem &b = bis[order[pos]];
+        std::copy(b.pp.A.begin(), b.pp.A.end(), hA.begin() + b.a_off);
+        std::copy(b.pp.D.begin(), b.pp.D.end(), hD.begin() + b.d_off);
+        BatchTask x{};
+        x.A = (const int32_t *)cx.bA.p + b.a_off;
+        x.D = (const int32_t *)cx.bD.p + b.d_off;
+        x.R = b.p.R;
+        x.q = b.g1.q;
+        x.nseg = b.g1.nseg;
+        x.seg_base = 0;
+        x.task0 = task1;
+        x.unit_best = (int32_t *)cx.bUnits.p + b.ub_off;
+        x.gbest = gbest + pos;
+        x.gfirst = gfirst + pos;
+        x.unit = funit + pos;
+        h1[pos] = x;
+        task1 += waves_for(b.g1.nseg);
+        x.q = b.g2.q;
+        x.nseg = b.g2.nseg;
+        x.seg_base = b.g2.nseg;
+        x.task0 = task2;
+        h2[pos] = x;
+        task2 += waves_for(b.g2.nseg);
+      }
+      g.ntasks1 = task1;
+      g.ntasks2 = task2;
+    }
+    if (aw) HIPCHK(hipMemcpyAsync(cx.bA.p, hA.data(), aw * 4, hipMemcpyHostToDevice, st));
+    if (dw) HIPCHK(hipMemcpyAsync(cx.bD.p, hD.data(), dw * 4, hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(t1, h1.data(), (size_t)nb * sizeof(BatchTask), hipMemcpyHostToDevice, st));
+    HIPCHK(hipMemcpyAsync(t2, h2.data(), (size_t)nb * sizeof(BatchTask), hipMemcpyHostToDevice, st));
+
+    // the launch sequence: no host synchronisation inside, so it can be captured
+    auto sequence = [&]() -> int {
+      HIPCHK(launch_fill_i32(gbest, INT32_MIN, (uint64_t)nb, st));
+      HIPCHK(launch_fill_u64(gfirst, ~0ull, 2 * (uint64_t)nb, st));
+      HIPCHK(launch_fill_i32((int32_t *)cnt, 0, 2 * groups.size(), st));
+      for (int gi = 0; gi < (int)groups.size(); ++gi) {
+        const BGroup &g = groups[gi];
+        BatchLaunch bl{t1 + g.first, (int)g.members.size(), g.ntasks1, cnt + 2 * gi, g.kind, g.d, g.width,
+                       g.walk == WALK_PLAIN16};
+        HIPCHK(launch_batch_walk(MODE_MAX, bl, st));
+      }
+      HIPCHK(launch_batch_first_unit(t1, nb, funit, st));
+      for (int gi = 0; gi < (int)groups.size(); ++gi) {
+        const BGroup &g = groups[gi];
+        BatchLaunch bl{t2 + g.first, (int)g.members.size(), g.ntasks2, cnt + 2 * gi + 1, g.kind, g.d, g.width,
+                       g.walk == WALK_PLAIN16};
+        HIPCHK(launch_batch_walk(MODE_FIRST, bl, st));
+      }
+      return 0;
+    };
+    HIPCHK(hipEventRecord(cx.ev[0], st));
+    if (o.batch_graph) {
+      // the graph bakes in buffer addresses and the group structure
+      std::vector<uint64_t> sig = {(uint64_t)nb, (uint64_t)(uintptr_t)cx.bScal.p, (uint64_t)(uintptr_t)cx.bTasks.p,
+                                   (uint64_t)(uintptr_t)cx.bCnt.p, (uint64_t)(uintptr_t)st};
+      for (auto &g : groups) {
+        sig.push_back(((uint64_t)g.walk << 48) | ((uint64_t)g.kind << 40) | ((uint64_t)g.d << 32) | (uint64_t)g.width);
+        sig.push_back(g.members.size());
+        sig.push_back(g.ntasks1);
+        sig.push_back(g.ntasks2);
+      }
+      if (!cx.bGraph || cx.bGraphSig != sig) {
+        if (cx.bGraph) (void)hipGraphExecDestroy(cx.bGraph);
+        cx.bGraph = nullptr;
+        hipGraph_t graph;
+        HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const int rc = sequence();
+        hipError_t ec = hipStreamEndCapture(st, &graph);
+        if (rc) return rc;
+        if (ec != hipSuccess) return fail(LNORM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
+        hipError_t ei = hipGraphInstantiate(&cx.bGraph, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (ei != hipSuccess) return fail(LNORM_E_HIP, "hipGraphInstantiate: %s", hipGetErrorString(ei));
+        cx.bGraphSig = sig;
+      } else {
+        replayed = true;
+      }
+      HIPCHK(hipGraphLaunch(cx.bGraph, st));
+    } else {
+      if (int r = sequence()) return r;
+    }
+    HIPCHK(hipEventRecord(cx.ev[1], st));
+    std::vector<int32_t> hb(nb);
+    std::vector<unsigned long long> hf(nb);
+    HIPCHK(hipMemcpyAsync(hb.data(), gbest, (size_t)nb * 4, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipMemcpyAsync(hf.data(), gfirst, (size_t)nb * 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(hipStreamSynchronize(st));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, cx.ev[0], cx.ev[1]));
+    ms_dev = ms;
+    for (int pos = 0; pos < nb; ++pos) {
+      const BItem &b = bis[order[pos]];
+      lnorm_hip_item &it = items[b.idx];
+      if (b.p.R > 1 && hf[pos] == ~0ull) {
+        it.status = fail(LNORM_E_HIP, "batch item %d: no rank attains %d", b.idx, hb[pos]);
+        if (!first_err) first_err = it.status;
+        continue;
+      }
+      it.Lnorm = hb[pos];
+      if (it.strategy && b.p.R > 1) labels_of_rank(hf[pos], b.p.R, b.p.n, it.strategy);
+      it.status = LNORM_OK;
+    }
+  }
+  for (int i : fallback) {
+    lnorm_hip_item &it = items[i];
+    lnorm_hip_opts oo = o;
+    oo.want_strategy = it.strategy != nullptr;
+    oo.kernel = LNORM_KERNEL_AUTO;
+    lnorm_hip_result r;
+    it.status = calc_impl(it.mtx_as_vec, it.iRows, it.iCols, it.n, it.marginal, &oo, it.strategy, &r);
+    it.Lnorm = it.status == 0 ? r.Lnorm : 0;
+    if (it.status && !first_err) first_err = it.status;
+  }
+  if (stats) {
+    std::memset(stats, 0, sizeof *stats);
+    stats->items = count;
+    stats->batched = nb;
+    stats->fallback = (int32_t)fallback.size();
+    stats->groups = (int32_t)groups.size();
+    stats->graph_replayed = replayed ? 1 : 0;
+    stats->ms_device = ms_dev;
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+    for (const BItem &b : bis) stats->labelings += b.p.total;
+  }
+  return first_err;
+}
+
 }  // namespace
 
 extern "C" {
@@ -706,20 +976,17 @@ int lnorm_hip_calc_Lnorm(const int32_t *A, int R, int C, int n, int marginal, in
 }
 
 int lnorm_hip_calc_batch(lnorm_hip_item *items, int count, const lnorm_hip_opts *opts) {
+  return lnorm_hip_calc_batch_ex(items, count, opts, nullptr);
+}
+
+int lnorm_hip_calc_batch_ex(lnorm_hip_item *items, int count, const lnorm_hip_opts *opts,
+                            lnorm_hip_batch_stats *stats) {
+  g_err.clear();
   if (count < 0 || (count > 0 && !items)) return fail(LNORM_E_ARG, "bad batch");
-  int first_err = 0;
-  for (int i = 0; i < count; ++i) {
-    lnorm_hip_item &it = items[i];
-    lnorm_hip_opts o;
-    lnorm_hip_default_opts(&o);
-    if (opts) o = *opts;
-    o.want_strategy = it.strategy != nullptr;
-    lnorm_hip_result r;
-    it.status = calc_impl(it.mtx_as_vec, it.iRows, it.iCols, it.n, it.marginal, &o, it.strategy, &r);
-    it.Lnorm = it.status == 0 ? r.Lnorm : 0;
-    if (it.status && !first_err) first_err = it.status;
-  }
-  return first_err;
+  lnorm_hip_opts o;
+  lnorm_hip_default_opts(&o);
+  if (opts) o = *opts;
+  return batch_impl(items, count, o, stats);
 }
 
 const char *lnorm_hip_last_error(void) { return g_err.c_str(); }

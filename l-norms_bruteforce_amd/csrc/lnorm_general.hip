@@ -101,12 +101,13 @@ __global__ __launch_bounds__(64) void k_general(
       }
       eval(N);
     }
+    // best and first are wave-uniform (built from wave sums): every lane
+    // stores and updates with the same value, so no branch depends on the
+    // lane index inside the segment loop
     if (MODE == MODE_MAX) {
-      if (lane == 0) {
-        unit_best[sidx] = best;
-        atomicMax(gbest, best);
-      }
-    } else if (lane == 0 && first != ~0ull) {
+      unit_best[sidx] = best;
+      atomicMax(gbest, best);
+    } else if (first != ~0ull) {
       atomicMin(gfirst, (unsigned long long)first);
     }
   }

@@ -69,6 +69,41 @@ struct TableArgs {
 
 // Launchers (return hipSuccess or the launch error).  `grid_hint` <= 0 means
 // choose automatically.
+// largest packed width (words of two columns) instantiated per d: d * NW <= 160
+constexpr int dary16_max_words(int d) {
+  return d == 2 ? 64 : d == 3 ? 48 : (d == 4 || d == 5) ? 32 : d == 6 ? 24 : (d >= 7 && d <= 10) ? 16 : 0;
+}
+
+// One item of a batch for one phase (device memory, read by the persistent
+// batch kernels).  Tasks [task0, task0 + 2*ceil(nseg/128)) are its waves.
+struct BatchTask {
+  const int32_t *A;            // binary walk: R x NY; packed walk: R x NW words
+  const int32_t *D;            // binary walk: [2][R-1][NY] deltas
+  int R, q;                    // rows; segment = base^q ranks
+  uint64_t nseg;               // segments of this item in this phase
+  uint64_t seg_base;           // MODE_FIRST: segments per unit (seg0 = *unit * seg_base)
+  uint64_t task0;              // first task of this item
+  int32_t *unit_best;          // MODE_MAX: this item's unit maxima (nseg entries)
+  int32_t *gbest;              // this item's maximum
+  unsigned long long *gfirst;  // MODE_FIRST: this item's smallest maximising rank
+  const unsigned long long *unit;  // MODE_FIRST: the first unit attaining the maximum
+};
+
+// One persistent launch over the items of a group.
+struct BatchLaunch {
+  const BatchTask *tasks;      // nitems descriptors of this group (device)
+  int nitems;
+  uint64_t ntasks;
+  unsigned int *counter;       // zeroed before the launch
+  Kind kind;
+  int d;
+  int width;                   // binary: NY; packed: 2 * NW
+  bool packed;
+};
+
+hipError_t launch_batch_walk(Mode mode, const BatchLaunch &b, hipStream_t st);
+hipError_t launch_batch_first_unit(const BatchTask *tasks, int nitems, unsigned long long *first, hipStream_t st);
+
 hipError_t launch_plain(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block);
 // Packed 16-bit d-ary walk (d = 2..10): A holds R x (ny_pad/2) packed words.
 hipError_t launch_plain16(Mode mode, const PlainArgs &a, hipStream_t st, int block);

@@ -75,3 +75,44 @@ def matrix_text(M) -> str:
 def write_matrix(path, M) -> None:
     with open(path, "w") as f:
         f.write(matrix_text(M))
+
+
+def _draw(seed: int, k: int, lo: int, hi: int) -> int:
+    """The k-th parameter of an item: one draw from sub-stream (seed, k)."""
+    return int(uniform_ints(((seed << 8) | k) & _M64, 1, lo, hi)[0])
+
+
+def batch_item(i: int, seed0: int = 1_000_000, d3_max_rows: int = 20, r: int = 100) -> dict:
+    """Item i of the mixed batch workload: an m x n file (m rows uniform in
+    [12, 30], n columns uniform in [8, 64]), d = 1, 2, 3 with probability
+    60/25/15 %, entries uniform in [-r, r].  A quarter of the items get 1-3
+    duplicated rows and 1-3 duplicated columns and the preprocessing flag 'y';
+    a tenth of the d = 1 items use marginals ('m').  d = 3 items keep at most
+    d3_max_rows rows (3^19 = 1.2e9 labelings at the default 20), which bounds
+    the batch at about 1e12 labelings in total.
+
+    Returns {"seed", "text" (file bytes), "d", "stat", "marginal"}: the
+    arguments the reference CLI takes after the file name."""
+    seed = seed0 + i
+    cols = _draw(seed, 0, 8, 64)
+    rows = _draw(seed, 1, 12, 30)
+    u = _draw(seed, 2, 0, 99)
+    d = 1 if u < 60 else (2 if u < 85 else 3)
+    if d == 3:
+        rows = min(rows, d3_max_rows)
+    dup = _draw(seed, 3, 0, 3) == 0
+    marginal = 1 if d == 1 and _draw(seed, 4, 0, 9) == 0 else 0
+    M = gen_matrix(rows, cols, r, seed)
+    if dup:
+        nr, nc = _draw(seed, 5, 1, 3), _draw(seed, 6, 1, 3)
+        src_r = uniform_ints(((seed << 8) | 7) & _M64, nr, 1, rows - 1)
+        src_c = uniform_ints(((seed << 8) | 8) & _M64, nc, 1, cols - 1)
+        M = np.vstack([M, M[src_r]])
+        M = np.hstack([M, M[:, src_c]])
+    return {"seed": seed, "text": matrix_text(M).encode(), "d": d, "stat": "y" if dup else "n",
+            "marginal": marginal}
+
+
+def batch_workload(count: int = 4096, seed0: int = 1_000_000, d3_max_rows: int = 20, r: int = 100) -> list:
+    """Items 0..count-1 of the mixed batch workload (see batch_item)."""
+    return [batch_item(i, seed0, d3_max_rows, r) for i in range(count)]
