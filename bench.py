@@ -30,6 +30,12 @@ PKG_DIR = os.path.join(ROOT, "l-norms_bruteforce_amd")
 METRIC = "exact L_1 sign-vectors/s on 1 MI355X (int8-MFMA vs plain Gray), bit-exact"
 VALU_PEAK = 256 * 4 * 16 * 2.4e9        # int32 lane-ops/s: 256 CUs x 4 SIMD16 x 2.4 GHz
 MFMA_I8_PEAK = 5.0e15                   # dense int8 ops/s
+# HBM bytes per launch of the headline instantiation, measured with
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE on `bench.py --case blocked`
+# (207.6 KiB and 20480 KiB), each scaled by the counter's reading for a 1 GiB
+# device-to-device copy (x2.0 for FETCH_SIZE, x1.0 for WRITE_SIZE).  Only the
+# per-wave maxima reach HBM; the matrix and the pattern table stay on chip.
+PROFILED_TRAFFIC = {"blocked16<np=32,P=4,marg=0": 425216 + 20971520}
 
 
 def load_pkg():
@@ -203,6 +209,19 @@ def batch_line(pkg, gen, steps: int) -> dict:
             "graph_replayed": st["graph_replayed"]}
 
 
+def run_case(pkg, gen, case: str, steps: int, warmup: int) -> int:
+    """One workload alone: the headline matrix on the blocked or the plain
+    kernel, or the 4096-item batch."""
+    if case == "batch":
+        print(json.dumps({"case": case, **batch_line(pkg, gen, steps)}), flush=True)
+        return 0
+    A = gen.gen_matrix(64, 40, 127, 104).T.copy()
+    k, ny = A.shape
+    r = run_steps(pkg, A, case, steps, warmup)
+    print(json.dumps({"case": case, **summarize(r, 2 ** (k - 1), ny, case)}), flush=True)
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -210,13 +229,30 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the plain kernel and the 40 x 40 line")
+    ap.add_argument("--case", choices=["blocked", "plain", "batch"],
+                    help="run only this workload (for profiler passes) and print its figures")
+    ap.add_argument("--copy-calibration", type=int, metavar="MIB",
+                    help="run one device-to-device copy of MIB MiB (torch) and exit: the reference "
+                         "kernel for scaling the profiler's FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the enumeration shards over GPUs by Gray-rank ranges "
               "(one max/argmin exchange), but no multi-GPU run is part of this build.", file=sys.stderr)
         return 2
+    if a.copy_calibration:
+        import torch
+        n = a.copy_calibration * (1 << 20) // 4
+        src = torch.arange(n, dtype=torch.int32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
+                          "ok": bool(torch.equal(dst[-16:].cpu(), src[-16:].cpu()))}), flush=True)
+        return 0
     pkg = load_pkg()
     from lnorm_amd import gen
+    if a.case:
+        return run_case(pkg, gen, a.case, a.steps, a.warmup)
 
     A = gen.gen_matrix(64, 40, 127, 104).T.copy()       # laid out 40 x 64 as the reference does
     k, ny = A.shape
@@ -251,7 +287,7 @@ def main():
             "peak": VALU_PEAK / 1e12,
             "unit": "TOP/s",
             "frac": sb["valu_frac"],
-            "traffic": None,
+            "traffic": next((v for k2, v in PROFILED_TRAFFIC.items() if sb["kernel_name"].startswith(k2)), None),
             "ops_per_sign_vector": sb["valu_ops_per_sign_vector"],
             "kernel_ms": sb["kernel_ms"],
         },
