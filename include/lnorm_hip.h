@@ -53,6 +53,8 @@ typedef struct lnorm_hip_opts {
     int32_t batch_graph;        /* batches: capture the launch sequence in a HIP graph and replay it
                                    while the batch's structure (groups, sizes) stays the same */
     int32_t autotune;           /* nonzero: use launch shapes chosen by lnorm_hip_autotune (cached) */
+    int32_t batch_streams;      /* batches: streams the kernel groups spread over, 1..4; 0 = auto */
+    int32_t batch_cap_log;      /* batches: at most ~2^cap ranks per lane segment, 8..20; 0 = auto */
 } lnorm_hip_opts;
 
 typedef struct lnorm_hip_result {

@@ -32,7 +32,8 @@ class Opts(ctypes.Structure):
                 ("seg_log", ctypes.c_int32), ("patterns_per_lane", ctypes.c_int32),
                 ("wave_log2", ctypes.c_int32), ("block", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("word_bits", ctypes.c_int32),
-                ("stream", ctypes.c_void_p), ("batch_graph", ctypes.c_int32), ("autotune", ctypes.c_int32)]
+                ("stream", ctypes.c_void_p), ("batch_graph", ctypes.c_int32), ("autotune", ctypes.c_int32),
+                ("batch_streams", ctypes.c_int32), ("batch_cap_log", ctypes.c_int32)]
 
 
 class Result(ctypes.Structure):

@@ -55,6 +55,9 @@ int fail(int code, const char *fmt, ...) {
 constexpr int kMaxLabels = 10;      // RANK_OF_NORM, functions.h:3
 constexpr int kMaxCols = 4096;      // length, functions.h:2
 constexpr uint64_t kUnitsPerChunk = 1ull << 20;
+// Streams besides the main one over which the kernel groups of a batch run
+// concurrently (each stream at the default priority).
+constexpr int kSideStreams = 3;
 
 struct Buf {
   void *p = nullptr;
@@ -66,8 +69,11 @@ struct DevCtx {
   hipStream_t stream = nullptr;
   hipEvent_t ev[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
   Buf A, D, units, scal, table, dpk, seed, dfull, lo, state;
-  // batch buffers and the captured launch sequence
+  // batch buffers, the side streams the groups of a batch spread over, and
+  // the captured launch sequence
   Buf bA, bD, bUnits, bScal, bTasks, bCnt;
+  hipStream_t side[kSideStreams] = {};
+  hipEvent_t bFork[2] = {}, bJoin[2][kSideStreams] = {};
   hipGraphExec_t bGraph = nullptr;
   std::vector<uint64_t> bGraphSig;
   bool init = false;
@@ -92,6 +98,11 @@ int ctx_init(DevCtx &c) {
   if (c.init) return 0;
   HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   for (auto &e : c.ev) HIPCHK(hipEventCreate(&e));
+  for (auto &st : c.side) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+  for (int ph = 0; ph < 2; ++ph) {
+    HIPCHK(hipEventCreateWithFlags(&c.bFork[ph], hipEventDisableTiming));
+    for (auto &e : c.bJoin[ph]) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+  }
   c.init = true;
   return 0;
 }
@@ -708,13 +719,16 @@ uint64_t waves_for(uint64_t nseg) { return 2 * ((nseg + 127) / 128); }
 constexpr uint64_t kBatchMaxTotal = 1ull << 32;
 
 // Phase-1 segments of a batched item: at least the 128 lane segments of the
-// two waves one task pair covers, and at most ~2^14 ranks per lane so a task
-// stays short next to the others (balance over the persistent grid).
-Geometry batch_geometry(int base, int e, int forced, Walk walk) {
+// two waves one task pair covers, and at most ~2^cap_log ranks per lane so a
+// task stays short next to the others (balance over the persistent grid and
+// a short tail at the end of each group).
+constexpr int kBatchCapLog = 14;
+constexpr int kBatchStreams = 4;
+Geometry batch_geometry(int base, int e, int forced, Walk walk, int cap_log) {
   if (forced > 0) return geometry_for(base, e, 63, forced, walk);
   const double lb = std::log2((double)base);
   Geometry g = geometry_for(base, e, 7, 0, walk);
-  const int qcap = std::max(0, (int)std::floor(14.0 / lb));
+  const int qcap = std::max(0, (int)std::floor(cap_log / lb));
   if (g.q > qcap) g = geometry_for(base, e, 63, qcap, walk);
   return g;
 }
@@ -724,6 +738,12 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
   int first_err = 0;
   std::vector<BItem> bis;
   std::vector<int> fallback;
+  if (o.batch_streams < 0 || o.batch_streams > kSideStreams + 1)
+    return fail(LNORM_E_ARG, "batch_streams must be 0..%d", kSideStreams + 1);
+  if (o.batch_cap_log != 0 && (o.batch_cap_log < 8 || o.batch_cap_log > 20))
+    return fail(LNORM_E_ARG, "batch_cap_log must be 0 or 8..20");
+  const int cap_log = o.batch_cap_log ? o.batch_cap_log : kBatchCapLog;
+  const int nlanes = o.batch_streams ? o.batch_streams : kBatchStreams;
   for (int i = 0; i < count; ++i) {
     lnorm_hip_item &it = items[i];
     it.Lnorm = 0;
@@ -750,7 +770,7 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
     }
     if (b.p.total > kBatchMaxTotal) { fallback.push_back(i); continue; }
     prepare_walk(b.p, walk, width, b.pp);
-    b.g1 = batch_geometry(b.p.base, b.p.R - 1, o.seg_log, walk);
+    b.g1 = batch_geometry(b.p.base, b.p.R - 1, o.seg_log, walk, cap_log);
     b.g2 = geometry_for(b.p.base, b.g1.q, 8, 0, walk);
     b.work = (double)b.p.total * b.p.C;
     bis.push_back(std::move(b));
@@ -803,7 +823,7 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
     if (int r = ensure(cx.bUnits, uw * 4)) return r;
     if (int r = ensure(cx.bScal, (size_t)nb * 24)) return r;
     if (int r = ensure(cx.bTasks, (size_t)2 * nb * sizeof(BatchTask))) return r;
-    if (int r = ensure(cx.bCnt, (size_t)2 * groups.size() * 4)) return r;
+    if (int r = ensure(cx.bCnt, (size_t)2 * groups.size() * kCounterShards * kCounterStride * 4)) return r;
     int32_t *gbest = (int32_t *)cx.bScal.p;
     unsigned long long *gfirst = (unsigned long long *)((char *)cx.bScal.p + (size_t)nb * 8);
     unsigned long long *funit = gfirst + nb;
@@ -839,6 +859,8 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
         h2[pos] = x;
         task2 += waves_for(b.g2.nseg);
       }
+      if (task1 >= (1ull << 31) || task2 >= (1ull << 31))   // t = k * 8 + shard stays in uint32
+        return fail(LNORM_E_SIZE, "batch group of %llu tasks: split the batch", (unsigned long long)task1);
       g.ntasks1 = task1;
       g.ntasks2 = task2;
     }
@@ -847,25 +869,53 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
     HIPCHK(hipMemcpyAsync(t1, h1.data(), (size_t)nb * sizeof(BatchTask), hipMemcpyHostToDevice, st));
     HIPCHK(hipMemcpyAsync(t2, h2.data(), (size_t)nb * sizeof(BatchTask), hipMemcpyHostToDevice, st));
 
-    // the launch sequence: no host synchronisation inside, so it can be captured
+    // Groups spread over the main and the side streams, heaviest first onto
+    // the least loaded stream, so that one group's last tasks overlap the
+    // others' work instead of leaving the chip idle.
+    const int kLanes = nlanes;
+    std::vector<int> gorder(groups.size());
+    for (int gi = 0; gi < (int)groups.size(); ++gi) gorder[gi] = gi;
+    std::vector<double> gwork(groups.size(), 0.0);
+    for (int gi = 0; gi < (int)groups.size(); ++gi)
+      for (int k : groups[gi].members) gwork[gi] += bis[k].work;
+    std::stable_sort(gorder.begin(), gorder.end(), [&](int a, int b) { return gwork[a] > gwork[b]; });
+    std::vector<int> glane(groups.size(), 0);
+    {
+      double load[kSideStreams + 1] = {};
+      for (int gi : gorder) {
+        int best = 0;
+        for (int l = 1; l < kLanes; ++l) if (load[l] < load[best]) best = l;
+        glane[gi] = best;
+        load[best] += gwork[gi];
+      }
+    }
+    auto lane_stream = [&](int l) { return l == 0 ? st : cx.side[l - 1]; };
+
+    // the launch sequence: no host synchronisation inside, so it can be
+    // captured (the side streams join the capture through the fork event)
+    auto phase = [&](int ph, Mode mode, BatchTask *tasks) -> int {
+      HIPCHK(hipEventRecord(cx.bFork[ph], st));
+      for (int l = 1; l < kLanes; ++l) HIPCHK(hipStreamWaitEvent(lane_stream(l), cx.bFork[ph], 0));
+      for (int gi : gorder) {
+        const BGroup &g = groups[gi];
+        BatchLaunch bl{tasks + g.first, (int)g.members.size(), ph == 0 ? g.ntasks1 : g.ntasks2,
+                       cnt + (size_t)(2 * gi + ph) * kCounterShards * kCounterStride,
+                       g.kind, g.d, g.width, g.walk == WALK_PLAIN16};
+        HIPCHK(launch_batch_walk(mode, bl, lane_stream(glane[gi])));
+      }
+      for (int l = 1; l < kLanes; ++l) {
+        HIPCHK(hipEventRecord(cx.bJoin[ph][l - 1], lane_stream(l)));
+        HIPCHK(hipStreamWaitEvent(st, cx.bJoin[ph][l - 1], 0));
+      }
+      return 0;
+    };
     auto sequence = [&]() -> int {
       HIPCHK(launch_fill_i32(gbest, INT32_MIN, (uint64_t)nb, st));
       HIPCHK(launch_fill_u64(gfirst, ~0ull, 2 * (uint64_t)nb, st));
-      HIPCHK(launch_fill_i32((int32_t *)cnt, 0, 2 * groups.size(), st));
-      for (int gi = 0; gi < (int)groups.size(); ++gi) {
-        const BGroup &g = groups[gi];
-        BatchLaunch bl{t1 + g.first, (int)g.members.size(), g.ntasks1, cnt + 2 * gi, g.kind, g.d, g.width,
-                       g.walk == WALK_PLAIN16};
-        HIPCHK(launch_batch_walk(MODE_MAX, bl, st));
-      }
+      HIPCHK(launch_fill_i32((int32_t *)cnt, 0, 2 * groups.size() * kCounterShards * kCounterStride, st));
+      if (int r = phase(0, MODE_MAX, t1)) return r;
       HIPCHK(launch_batch_first_unit(t1, nb, funit, st));
-      for (int gi = 0; gi < (int)groups.size(); ++gi) {
-        const BGroup &g = groups[gi];
-        BatchLaunch bl{t2 + g.first, (int)g.members.size(), g.ntasks2, cnt + 2 * gi + 1, g.kind, g.d, g.width,
-                       g.walk == WALK_PLAIN16};
-        HIPCHK(launch_batch_walk(MODE_FIRST, bl, st));
-      }
-      return 0;
+      return phase(1, MODE_FIRST, t2);
     };
     HIPCHK(hipEventRecord(cx.ev[0], st));
     if (o.batch_graph) {
@@ -878,14 +928,20 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
         sig.push_back(g.ntasks1);
         sig.push_back(g.ntasks2);
       }
+      sig.push_back((uint64_t)nlanes);
+      for (int l : glane) sig.push_back((uint64_t)l);
       if (!cx.bGraph || cx.bGraphSig != sig) {
         if (cx.bGraph) (void)hipGraphExecDestroy(cx.bGraph);
         cx.bGraph = nullptr;
         hipGraph_t graph;
         HIPCHK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        graph = nullptr;
         const int rc = sequence();
         hipError_t ec = hipStreamEndCapture(st, &graph);
-        if (rc) return rc;
+        if (rc) {
+          if (ec == hipSuccess && graph) (void)hipGraphDestroy(graph);
+          return rc;
+        }
         if (ec != hipSuccess) return fail(LNORM_E_HIP, "hipStreamEndCapture: %s", hipGetErrorString(ec));
         hipError_t ei = hipGraphInstantiate(&cx.bGraph, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
@@ -1005,13 +1061,22 @@ void lnorm_hip_release(void) {
   if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return;
   DevCtx &c = g_ctx[dev];
   std::lock_guard<std::mutex> lock(c.mu);
-  for (Buf *b : {&c.A, &c.D, &c.units, &c.scal, &c.table, &c.dpk, &c.seed, &c.dfull, &c.lo, &c.state}) {
+  if (c.bGraph) (void)hipGraphExecDestroy(c.bGraph);
+  c.bGraph = nullptr;
+  c.bGraphSig.clear();
+  for (Buf *b : {&c.A, &c.D, &c.units, &c.scal, &c.table, &c.dpk, &c.seed, &c.dfull, &c.lo, &c.state, &c.bA, &c.bD,
+                 &c.bUnits, &c.bScal, &c.bTasks, &c.bCnt}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->cap = 0;
   }
   if (c.init) {
     for (auto &e : c.ev) (void)hipEventDestroy(e);
+    for (int ph = 0; ph < 2; ++ph) {
+      (void)hipEventDestroy(c.bFork[ph]);
+      for (auto &e : c.bJoin[ph]) (void)hipEventDestroy(e);
+    }
+    for (auto &st : c.side) (void)hipStreamDestroy(st);
     (void)hipStreamDestroy(c.stream);
     c.init = false;
   }

@@ -19,24 +19,40 @@
 
 namespace lnorm {
 
-// The wave's next task: one atomic increment per wave, its old value
-// broadcast from lane 0.  Every lane executes the atomic (lane 0 adds 1, the
-// others 0) so that no control flow depends on the lane index: a
-// `if (lane == 0)` here, loop-invariant but lane-divergent, let the compiler
-// split the persistent loop by lane, and lanes 1..63 then looped on task 0
-// forever.
-__device__ __forceinline__ uint32_t next_task(unsigned int *counter) {
-  const uint32_t old = atomicAdd(counter, wave_lane() == 0 ? 1u : 0u);
-  return (uint32_t)__builtin_amdgcn_readfirstlane((int)old);
+// The wave's next task.  A wave starts on the counter shard of its XCD
+// (workgroups are dispatched to the 8 XCDs round-robin by index) and moves
+// to the next shard when that one is exhausted, so every task is handed out
+// once and the waves of an XCD that finishes early help the others.  Every
+// lane executes the atomic (lane 0 adds 1, the others 0): no control flow
+// depends on the lane index.  A `if (lane == 0)` here, loop-invariant but
+// lane-divergent, let the compiler split the persistent loop by lane, and
+// lanes 1..63 then looped on task 0 forever.
+__device__ __forceinline__ bool next_task(unsigned int *counters, uint64_t ntasks, int &shard, int &left,
+                                          uint32_t &t) {
+  while (left > 0) {
+    // readfirstlane: the shard is wave-uniform, and the compiler must see it
+    // so; with a uniform address it issues one atomic per wave (the lane
+    // addends are summed first) instead of 64 same-address atomics.
+    const int us = __builtin_amdgcn_readfirstlane(shard);
+    const uint32_t k = atomicAdd(counters + (size_t)us * kCounterStride, wave_lane() == 0 ? 1u : 0u);
+    const uint64_t cand = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)k) * kCounterShards + shard;
+    if (cand < ntasks) {
+      t = (uint32_t)cand;
+      return true;
+    }
+    shard = (shard + 1) & (kCounterShards - 1);
+    --left;
+  }
+  return false;
 }
 
 template <int KIND, int NY, int MODE>
 __global__ __launch_bounds__(256) void k_batch_bin(const BatchTask *__restrict__ tasks, int nitems,
                                                    uint64_t ntasks, unsigned int *__restrict__ counter,
                                                    uint32_t B) {
-  for (;;) {
-    const uint32_t t = next_task(counter);
-    if (t >= ntasks) break;                      // every wave ends here
+  int shard = (int)(blockIdx.x & (kCounterShards - 1)), left = kCounterShards;
+  uint32_t t;
+  while (next_task(counter, ntasks, shard, left, t)) {   // every wave ends when all shards are empty
     int lo = 0, hi = nitems - 1;                 // last item with task0 <= t
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;
@@ -59,9 +75,9 @@ template <int D, int NW, int MODE>
 __global__ __launch_bounds__(256) void k_batch_p16(const BatchTask *__restrict__ tasks, int nitems,
                                                    uint64_t ntasks, unsigned int *__restrict__ counter,
                                                    uint32_t B) {
-  for (;;) {
-    const uint32_t t = next_task(counter);
-    if (t >= ntasks) break;
+  int shard = (int)(blockIdx.x & (kCounterShards - 1)), left = kCounterShards;
+  uint32_t t;
+  while (next_task(counter, ntasks, shard, left, t)) {
     int lo = 0, hi = nitems - 1;
     while (lo < hi) {
       const int mid = (lo + hi + 1) >> 1;

@@ -28,13 +28,22 @@ __device__ __forceinline__ uint64_t global_wave() {
   return (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)w);
 }
 
+// Butterfly reductions over the 64 lanes.  Every lane ends with the same
+// value; readfirstlane states that to the compiler, so the result lives in
+// an SGPR and an atomic on it is issued once per wave, not once per lane.
+__device__ __forceinline__ uint64_t uniform_u64(uint64_t v) {
+  const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)v);
+  const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)(v >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
 __device__ __forceinline__ int32_t wave_max_i32(int32_t v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) {
     int32_t o = __shfl_xor(v, off, 64);
     v = v > o ? v : o;
   }
-  return v;
+  return __builtin_amdgcn_readfirstlane(v);
 }
 
 __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
@@ -43,13 +52,13 @@ __device__ __forceinline__ uint64_t wave_min_u64(uint64_t v) {
     uint64_t o = __shfl_xor(v, off, 64);
     v = v < o ? v : o;
   }
-  return v;
+  return uniform_u64(v);
 }
 
 __device__ __forceinline__ uint32_t wave_sum_u32(uint32_t v) {
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-  return v;
+  return (uint32_t)__builtin_amdgcn_readfirstlane((int)v);
 }
 
 // Segment of a lane in the lane-per-segment walks.  Lanes of one wave get

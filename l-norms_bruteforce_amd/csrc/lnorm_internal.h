@@ -89,12 +89,18 @@ struct BatchTask {
   const unsigned long long *unit;  // MODE_FIRST: the first unit attaining the maximum
 };
 
+// The task counter of a persistent launch is sharded by XCD: shard x hands
+// out tasks x, x + 8, x + 16, ... and lives in its own 4 KiB block, so the
+// device-scope atomics of different XCDs do not queue on one address.
+constexpr int kCounterShards = 8;
+constexpr int kCounterStride = 1024;   // uint32 words between shards
+
 // One persistent launch over the items of a group.
 struct BatchLaunch {
   const BatchTask *tasks;      // nitems descriptors of this group (device)
   int nitems;
-  uint64_t ntasks;
-  unsigned int *counter;       // zeroed before the launch
+  uint64_t ntasks;             // < 2^31
+  unsigned int *counter;       // kCounterShards shards, kCounterStride apart, zeroed before the launch
   Kind kind;
   int d;
   int width;                   // binary: NY; packed: 2 * NW

@@ -118,7 +118,13 @@ __device__ __forceinline__ void walk_bin(
   if constexpr (MODE == MODE_MAX) {
     if (active) unit_best[local] = best;
     const int32_t m = wave_max_i32(active ? best : INT32_MIN);
-    atomicMax(gbest, m);                 // every lane: no branch on the lane index
+    // gbest only grows, so any value read from it is a lower bound of the
+    // current one: skip the (device-scope, serialising) atomic when it
+    // cannot raise the maximum.  The value read is made wave-uniform
+    // (readfirstlane) so the branch is uniform: a lane-divergent branch here
+    // made the compiler structurise the whole walk loop again, with one
+    // register copy per state word and step.
+    if (m > __builtin_amdgcn_readfirstlane(*(volatile int32_t *)gbest)) atomicMax(gbest, m);
   } else {
     const uint64_t r = (active && first != 0xFFFFFFFFu) ? ((seg << q) + first) : ~0ull;
     const uint64_t m = wave_min_u64(r);
@@ -257,7 +263,13 @@ __device__ __forceinline__ void walk_dary(
   if constexpr (MODE == MODE_MAX) {
     if (active) unit_best[local] = best;
     const int32_t m = wave_max_i32(active ? best : INT32_MIN);
-    atomicMax(gbest, m);                 // every lane: no branch on the lane index
+    // gbest only grows, so any value read from it is a lower bound of the
+    // current one: skip the (device-scope, serialising) atomic when it
+    // cannot raise the maximum.  The value read is made wave-uniform
+    // (readfirstlane) so the branch is uniform: a lane-divergent branch here
+    // made the compiler structurise the whole walk loop again, with one
+    // register copy per state word and step.
+    if (m > __builtin_amdgcn_readfirstlane(*(volatile int32_t *)gbest)) atomicMax(gbest, m);
   } else {
     const uint64_t r = (active && first != 0xFFFFFFFFu) ? (seg * (uint64_t)nsteps + first) : ~0ull;
     const uint64_t m = wave_min_u64(r);
@@ -399,7 +411,13 @@ __device__ __forceinline__ void walk_p16(
   if constexpr (MODE == MODE_MAX) {
     if (active) unit_best[local] = best;
     const int32_t m = wave_max_i32(active ? best : INT32_MIN);
-    atomicMax(gbest, m);                 // every lane: no branch on the lane index
+    // gbest only grows, so any value read from it is a lower bound of the
+    // current one: skip the (device-scope, serialising) atomic when it
+    // cannot raise the maximum.  The value read is made wave-uniform
+    // (readfirstlane) so the branch is uniform: a lane-divergent branch here
+    // made the compiler structurise the whole walk loop again, with one
+    // register copy per state word and step.
+    if (m > __builtin_amdgcn_readfirstlane(*(volatile int32_t *)gbest)) atomicMax(gbest, m);
   } else {
     const uint64_t r = (active && first != 0xFFFFFFFFu) ? (seg * (uint64_t)nsteps + first) : ~0ull;
     const uint64_t m = wave_min_u64(r);
