@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define LNORM_HIP_ABI_VERSION 1
+#define LNORM_HIP_ABI_VERSION 2
 
 /* error codes */
 #define LNORM_OK 0
@@ -127,6 +127,54 @@ typedef struct lnorm_hip_batch_stats {
 /* lnorm_hip_calc_batch with statistics (stats may be NULL). */
 int lnorm_hip_calc_batch_ex(lnorm_hip_item *items, int count, const lnorm_hip_opts *opts,
                             lnorm_hip_batch_stats *stats);
+
+/* One timed candidate of the autotuner. */
+typedef struct lnorm_hip_tune_entry {
+    int32_t kernel;             /* LNORM_KERNEL_* of the candidate */
+    int32_t patterns_per_lane;  /* blocked: P */
+    int32_t wave_log2;          /* blocked: 2^wave_log2 high ranks per wave */
+    int32_t block;              /* plain kernels: threads per workgroup */
+    int32_t batch_streams;      /* batches: streams the groups spread over */
+    int32_t batch_cap_log;      /* batches: lane-segment cap */
+    double ms;                  /* HIP-event time of the probe */
+    double labelings_per_s;     /* labelings the probe covered per second */
+} lnorm_hip_tune_entry;
+
+/*
+ * Launch-shape autotuner.  No counterpart in the reference, whose launch
+ * shape is the CLI's blocks x threads (L_CUDA.cu:361-412) and does not
+ * change results here either.
+ *
+ * Times the candidate shapes of the kernel opts->kernel (AUTO: the one AUTO
+ * picks) would run this problem with:
+ *   blocked: patterns per lane 1/2/4/8 x 2^10..2^14 high ranks per wave;
+ *   plain, plain16: 64/128/256 threads per workgroup.
+ * Each probe is that kernel over a prefix of this problem's own units.
+ * The fastest shape is stored in the library's per-device cache, keyed by
+ * (kernel, kind, d, padded width, marginal, word size), and replaces any
+ * earlier entry.  Calls with opts->autotune != 0 use the cache (tuning on a
+ * miss); options the caller sets explicitly win.
+ *   best    : out (may be NULL), opts with the tuned fields filled in.
+ *   entries : out (may be NULL), up to max_entries timed candidates.
+ * Returns the number of candidates timed (0 for kernels without tunable
+ * shapes), or a negative LNORM_E_* code.
+ */
+int lnorm_hip_autotune(const int32_t *mtx_as_vec, int iRows, int iCols, int n, int marginal,
+                       const lnorm_hip_opts *opts, lnorm_hip_opts *best,
+                       lnorm_hip_tune_entry *entries, int max_entries);
+
+/*
+ * Batch tuner: times batch_streams 1/2/4 x batch_cap_log 10/12/14/16 on
+ * every 8th item of the batch (all items when fewer than 64), through the
+ * batched path without strategies.  The fastest pair is cached under the
+ * batch's shape (the multiset of item sizes, orders and marginal flags) and
+ * used by lnorm_hip_calc_batch(_ex) when opts->autotune != 0 and the caller
+ * leaves batch_streams and batch_cap_log at 0.  best and entries as above.
+ * Returns the number of candidates timed or a negative LNORM_E_* code.
+ * Items are not modified.
+ */
+int lnorm_hip_autotune_batch(const lnorm_hip_item *items, int count, const lnorm_hip_opts *opts,
+                             lnorm_hip_opts *best, lnorm_hip_tune_entry *entries, int max_entries);
 
 /* Fills opts with the defaults (AUTO kernel, strategy on, current device). */
 void lnorm_hip_default_opts(lnorm_hip_opts *opts);

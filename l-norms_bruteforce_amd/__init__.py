@@ -59,6 +59,16 @@ class BatchStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
 
 
+class TuneEntry(ctypes.Structure):
+    _fields_ = [("kernel", ctypes.c_int32), ("patterns_per_lane", ctypes.c_int32),
+                ("wave_log2", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("batch_streams", ctypes.c_int32), ("batch_cap_log", ctypes.c_int32),
+                ("ms", ctypes.c_double), ("labelings_per_s", ctypes.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class Item(ctypes.Structure):
     _fields_ = [("mtx_as_vec", ctypes.POINTER(ctypes.c_int32)),
                 ("iRows", ctypes.c_int32), ("iCols", ctypes.c_int32),
@@ -71,7 +81,8 @@ class Item(ctypes.Structure):
 HIP_SYMBOLS = ["lnorm_hip_calc_Lnorm", "lnorm_hip_calc_ex", "lnorm_hip_calc_batch",
                "lnorm_hip_default_opts", "lnorm_hip_last_error", "lnorm_hip_abi_version",
                "lnorm_hip_device_count", "lnorm_hip_release", "lnorm_hip_selftest",
-               "lnorm_hip_debug_table", "lnorm_hip_calc_batch_ex"]
+               "lnorm_hip_debug_table", "lnorm_hip_calc_batch_ex", "lnorm_hip_autotune",
+               "lnorm_hip_autotune_batch"]
 
 _hip = None
 
@@ -101,6 +112,13 @@ def hip_lib():
         L.lnorm_hip_calc_batch_ex.restype = ctypes.c_int
         L.lnorm_hip_calc_batch_ex.argtypes = [ctypes.POINTER(Item), ctypes.c_int, ctypes.POINTER(Opts),
                                               ctypes.POINTER(BatchStats)]
+        L.lnorm_hip_autotune.restype = ctypes.c_int
+        L.lnorm_hip_autotune.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.POINTER(Opts), ctypes.POINTER(Opts), ctypes.POINTER(TuneEntry),
+                                         ctypes.c_int]
+        L.lnorm_hip_autotune_batch.restype = ctypes.c_int
+        L.lnorm_hip_autotune_batch.argtypes = [ctypes.POINTER(Item), ctypes.c_int, ctypes.POINTER(Opts),
+                                               ctypes.POINTER(Opts), ctypes.POINTER(TuneEntry), ctypes.c_int]
         L.lnorm_hip_default_opts.restype = None
         L.lnorm_hip_default_opts.argtypes = [ctypes.POINTER(Opts)]
         L.lnorm_hip_last_error.restype = ctypes.c_char_p
@@ -172,16 +190,7 @@ def calc_lnorm_simple(A, n: int, marginal: int = 0, blocks: int = 512, threads: 
 def calc_batch(problems, want_strategy=True, stats=False, **opts):
     """problems: iterable of (A, n, marginal).  Returns [(Lnorm, strategy, status)]
     (and the batch statistics when stats=True)."""
-    problems = list(problems)
-    mats = [_i32(A) for A, _, _ in problems]
-    strats = [np.zeros(max(m.shape[0] - 1, 1), dtype=np.int32) for m in mats]
-    items = (Item * len(mats))()
-    i32p = ctypes.POINTER(ctypes.c_int32)
-    for it, m, s, (_, n, marg) in zip(items, mats, strats, problems):
-        it.mtx_as_vec = m.ctypes.data_as(i32p)
-        it.iRows, it.iCols = m.shape
-        it.n, it.marginal = n, marg
-        it.strategy = s.ctypes.data_as(i32p) if want_strategy else None
+    items, mats, strats = _items(problems, want_strategy)
     o = default_opts(**opts)
     st = BatchStats()
     rc = hip_lib().lnorm_hip_calc_batch_ex(items, len(mats), ctypes.byref(o), ctypes.byref(st))
@@ -192,6 +201,52 @@ def calc_batch(problems, want_strategy=True, stats=False, **opts):
         out.append((int(it.Lnorm), [int(v) for v in s[: m.shape[0] - 1]] if want_strategy else None,
                     int(it.status)))
     return (out, st.as_dict()) if stats else out
+
+
+def _opts_dict(o: Opts) -> dict:
+    return {k: getattr(o, k) for k, _ in o._fields_ if k != "stream"}
+
+
+def autotune(A, n: int, marginal: int = 0, **opts):
+    """lnorm_hip_autotune: times the launch-shape candidates of the kernel this
+    problem runs on and caches the fastest.  Returns (best options dict,
+    [candidate dicts])."""
+    a = _i32(A)
+    R, C = a.shape
+    o = default_opts(**opts)
+    best = Opts()
+    ent = (TuneEntry * 64)()
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    n_ent = _check(hip_lib().lnorm_hip_autotune(a.ctypes.data_as(i32p), R, C, n, marginal, ctypes.byref(o),
+                                                ctypes.byref(best), ent, 64))
+    return _opts_dict(best), [ent[i].as_dict() for i in range(min(n_ent, 64))]
+
+
+def _items(problems, want_strategy):
+    problems = list(problems)
+    mats = [_i32(A) for A, _, _ in problems]
+    strats = [np.zeros(max(m.shape[0] - 1, 1), dtype=np.int32) for m in mats]
+    items = (Item * len(mats))()
+    i32p = ctypes.POINTER(ctypes.c_int32)
+    for it, m, s, (_, n, marg) in zip(items, mats, strats, problems):
+        it.mtx_as_vec = m.ctypes.data_as(i32p)
+        it.iRows, it.iCols = m.shape
+        it.n, it.marginal = n, marg
+        it.strategy = s.ctypes.data_as(i32p) if want_strategy else None
+    return items, mats, strats
+
+
+def autotune_batch(problems, **opts):
+    """lnorm_hip_autotune_batch: times batch_streams x batch_cap_log on a
+    sample of the batch and caches the fastest for this batch shape.
+    Returns (best options dict, [candidate dicts])."""
+    items, mats, _ = _items(problems, False)
+    o = default_opts(**opts)
+    best = Opts()
+    ent = (TuneEntry * 64)()
+    n_ent = _check(hip_lib().lnorm_hip_autotune_batch(items, len(mats), ctypes.byref(o), ctypes.byref(best),
+                                                      ent, 64))
+    return _opts_dict(best), [ent[i].as_dict() for i in range(min(n_ent, 64))]
 
 
 def selftest() -> int:

@@ -17,13 +17,16 @@
 // (L_OpenMP.c:140,204; functions.h:350-359).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/lnorm_hip.h"
@@ -92,6 +95,14 @@ int ensure(Buf &b, size_t bytes) {
   if (e != hipSuccess) return fail(LNORM_E_HIP, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
   b.cap = bytes;
   return 0;
+}
+
+// The per-unit maxima (one chunk of units) and the scalar results that every
+// enumeration writes.  Each run_* function makes sure of them itself: the
+// autotuner's probes run before a call's own uploads.
+int ensure_result_bufs(DevCtx &c) {
+  if (int r = ensure(c.units, kUnitsPerChunk * 4)) return r;
+  return ensure(c.scal, 256);
 }
 
 int ctx_init(DevCtx &c) {
@@ -262,17 +273,21 @@ struct Phase1Out {
   double ms_kernel = 0;   // HIP-event time of the enumeration kernel launches alone
 };
 
+// max_units > 0 limits the walk to the first max_units units (the
+// autotuner's probes); the result is then a partial maximum.
 int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometry &g, hipStream_t st,
-                 int block, Phase1Out &out) {
+                 int block, Phase1Out &out, uint64_t max_units = 0) {
+  if (int r = ensure_result_bufs(cx)) return r;
   const uint64_t chunk = kUnitsPerChunk;
-  out.units = g.nseg;
+  const uint64_t nunits = max_units ? std::min(max_units, g.nseg) : g.nseg;
+  out.units = nunits;
   out.best = INT32_MIN;
   out.first_unit = ~0ull;
   int32_t *ub = (int32_t *)cx.units.p;
   int32_t *gbest = (int32_t *)cx.scal.p;
   unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
-  for (uint64_t c0 = 0; c0 < g.nseg; c0 += chunk) {
-    const uint64_t n = std::min(chunk, g.nseg - c0);
+  for (uint64_t c0 = 0; c0 < nunits; c0 += chunk) {
+    const uint64_t n = std::min(chunk, nunits - c0);
     HIPCHK(launch_fill_i32(gbest, INT32_MIN, 1, st));
     HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
     PlainArgs a{};
@@ -312,6 +327,7 @@ int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometr
 // MODE_FIRST over the rank range [u * base^e, (u+1) * base^e).
 int run_walk_first(DevCtx &cx, const Problem &p, const Prepared &pp, uint64_t u, int e, int32_t target,
                    hipStream_t st, int block, uint64_t &rank) {
+  if (int r = ensure_result_bufs(cx)) return r;
   const Geometry g = geometry_for(p.base, e, 16, 0, pp.walk);
   unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
   HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
@@ -452,7 +468,9 @@ bool blocked_plan(const Problem &p, const lnorm_hip_opts &o, BlockedPlan &bp, st
   return true;
 }
 
-int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t st, Phase1Out &out) {
+int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t st, Phase1Out &out,
+                uint64_t max_units = 0) {
+  if (int r = ensure_result_bufs(cx)) return r;
   const int marg = p.kind == KIND_LM ? 1 : 0;
   const int NW = bp.NW, kl = bp.kl, nh = bp.nh;
   auto Aat = [&](int i, int l) -> int64_t { return l < 0 ? 0 : (int64_t)p.A[(size_t)i * p.C + l]; };
@@ -521,11 +539,12 @@ int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t
   int32_t *ub = (int32_t *)cx.units.p;
   int32_t *gbest = (int32_t *)cx.scal.p;
   unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
-  out.units = bp.nunits;
+  const uint64_t nunits = max_units ? std::min(max_units, bp.nunits) : bp.nunits;
+  out.units = nunits;
   out.best = INT32_MIN;
   out.first_unit = ~0ull;
-  for (uint64_t c0 = 0; c0 < bp.nunits; c0 += kUnitsPerChunk) {
-    const uint64_t n = std::min(kUnitsPerChunk, bp.nunits - c0);
+  for (uint64_t c0 = 0; c0 < nunits; c0 += kUnitsPerChunk) {
+    const uint64_t n = std::min(kUnitsPerChunk, nunits - c0);
     HIPCHK(launch_fill_i32(gbest, INT32_MIN, 1, st));
     HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
     BlockedArgs a{};
@@ -576,6 +595,151 @@ int select_device(const lnorm_hip_opts &o, int &dev) {
   return 0;
 }
 
+// ---- kernel choice ------------------------------------------------------------
+struct Choice {
+  int kernel = LNORM_KERNEL_AUTO;
+  BlockedPlan bp;
+  int ny_plain = 0, w16 = 0;
+};
+
+int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
+  std::string why;
+  int kernel = o.kernel;
+  const bool can_block =
+      (kernel == LNORM_KERNEL_AUTO || kernel == LNORM_KERNEL_BLOCKED) && blocked_plan(p, o, c.bp, why);
+  c.ny_plain = plain_ny_pad(p.kind, p.n, p.C);
+  const bool fits16 = p.n >= 2 && max_col_abs_sum(p, 0) <= 32767;
+  c.w16 = fits16 ? plain16_words(p.n, p.C) : 0;
+  if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
+    return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
+  if (kernel == LNORM_KERNEL_PLAIN && !c.ny_plain)
+    return fail(LNORM_E_UNSUPPORTED, "no plain instantiation for %d columns at n = %d", p.C, p.n);
+  if (kernel == LNORM_KERNEL_PLAIN16 && !c.w16)
+    return fail(LNORM_E_UNSUPPORTED, "packed 16-bit walk needs n >= 2, column sums <= 32767 and an instantiated "
+                "width (%d columns at n = %d)", p.C, p.n);
+  if (kernel == LNORM_KERNEL_AUTO)
+    kernel = can_block ? LNORM_KERNEL_BLOCKED
+                       : (c.w16 ? LNORM_KERNEL_PLAIN16 : (c.ny_plain ? LNORM_KERNEL_PLAIN : LNORM_KERNEL_GENERAL));
+  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_PLAIN16)
+    return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
+  c.kernel = kernel;
+  return 0;
+}
+
+void prepare_for(const Problem &p, const Choice &c, Prepared &pp) {
+  if (c.kernel == LNORM_KERNEL_PLAIN16) prepare_walk(p, WALK_PLAIN16, 2 * c.w16, pp);
+  else if (c.kernel == LNORM_KERNEL_GENERAL || !c.ny_plain) prepare_walk(p, WALK_GENERAL, 0, pp);
+  else prepare_walk(p, WALK_PLAIN, c.ny_plain, pp);
+}
+
+int upload_walk(DevCtx &cx, const Problem &p, const Prepared &pp, hipStream_t st) {
+  const int nvec = (p.kind == KIND_L1 || p.kind == KIND_LM) ? 1 : p.n;
+  if (int r = ensure(cx.A, pp.A.size() * 4)) return r;
+  if (int r = ensure(cx.D, pp.D.size() * 4)) return r;
+  if (int r = ensure(cx.units, kUnitsPerChunk * 4)) return r;
+  if (int r = ensure(cx.scal, 256)) return r;
+  if (pp.walk == WALK_GENERAL)
+    if (int r = ensure(cx.state, (size_t)kGeneralSlices * nvec * p.C * 4)) return r;
+  HIPCHK(hipMemcpyAsync(cx.A.p, pp.A.data(), pp.A.size() * 4, hipMemcpyHostToDevice, st));
+  if (!pp.D.empty()) HIPCHK(hipMemcpyAsync(cx.D.p, pp.D.data(), pp.D.size() * 4, hipMemcpyHostToDevice, st));
+  return 0;
+}
+
+// ---- autotuner ------------------------------------------------------------------
+// Launch shapes are timed on probes: the real kernel over a prefix of the
+// problem's own units, sized to a few full rounds of the chip (about 2^35
+// labelings), so per-unit work and occupancy are those of the full run.
+//   blocked: patterns per lane P (1, 2, 4, 8) x high ranks per wave 2^s
+//            (s = 10..14), wherever an instantiation exists;
+//   plain, plain16: threads per workgroup (64, 128, 256).
+// The fastest shape is cached per device and kernel shape (kernel, kind, d,
+// padded width, marginal, word size); calls with opts->autotune use it.
+struct TuneKey {
+  int dev, kernel, kind, n, width, marg, wide;
+  bool operator<(const TuneKey &o) const {
+    return std::tie(dev, kernel, kind, n, width, marg, wide) <
+           std::tie(o.dev, o.kernel, o.kind, o.n, o.width, o.marg, o.wide);
+  }
+};
+struct TuneVal {
+  int P = 0, s = 0, block = 0;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune;
+
+TuneKey tune_key(int dev, const Problem &p, const Choice &c) {
+  TuneKey k{dev, c.kernel, (int)p.kind, p.n, 0, p.kind == KIND_LM ? 1 : 0, 0};
+  if (c.kernel == LNORM_KERNEL_BLOCKED) {
+    k.width = c.bp.NW;
+    k.wide = c.bp.wide ? 1 : 0;
+  } else if (c.kernel == LNORM_KERNEL_PLAIN16) {
+    k.width = c.w16;
+  } else {
+    k.width = c.ny_plain;
+  }
+  return k;
+}
+
+// Times the candidates on the locked context; fills `best` (the tuned
+// fields of o) and appends one entry per candidate to `log` when given.
+int tune_single(DevCtx &cx, const Problem &p, const Choice &c, const lnorm_hip_opts &o, hipStream_t st,
+                TuneVal &best, std::vector<lnorm_hip_tune_entry> *log) {
+  double best_rate = -1;
+  auto note = [&](const lnorm_hip_tune_entry &e) {
+    if (log) log->push_back(e);
+    if (e.labelings_per_s > best_rate) {
+      best_rate = e.labelings_per_s;
+      best.P = e.patterns_per_lane;
+      best.s = e.wave_log2;
+      best.block = e.block;
+    }
+  };
+  if (c.kernel == LNORM_KERNEL_BLOCKED) {
+    for (int P : {1, 2, 4, 8}) {
+      for (int s = 10; s <= 14; ++s) {
+        lnorm_hip_opts oc = o;
+        oc.patterns_per_lane = P;
+        oc.wave_log2 = s;
+        BlockedPlan bp;
+        std::string why;
+        if (!blocked_plan(p, oc, bp, why) || bp.s != s) continue;
+        const uint64_t per_wave = (uint64_t)64 * P << s;
+        const uint64_t probe = std::max<uint64_t>(1ull << 14, std::min<uint64_t>(1ull << 17, (1ull << 35) / per_wave));
+        Phase1Out out;
+        if (int r = run_blocked(cx, p, bp, st, out, probe)) return r;
+        lnorm_hip_tune_entry e{};
+        e.kernel = LNORM_KERNEL_BLOCKED;
+        e.patterns_per_lane = P;
+        e.wave_log2 = s;
+        e.ms = out.ms_kernel;
+        e.labelings_per_s = (double)(out.units * per_wave) / (out.ms_kernel / 1e3);
+        note(e);
+      }
+    }
+  } else if (c.kernel == LNORM_KERNEL_PLAIN || c.kernel == LNORM_KERNEL_PLAIN16) {
+    Prepared pp;
+    prepare_for(p, c, pp);
+    if (int r = upload_walk(cx, p, pp, st)) return r;
+    const double lb = std::log2((double)p.base);
+    const Geometry full = geometry_for(p.base, p.R - 1, 20, o.seg_log, pp.walk);
+    // segments of at most ~2^14 ranks, 2^20 of them: the probe's shape
+    const int qp = std::min(full.q, (int)std::floor(14.0 / lb));
+    const Geometry g = geometry_for(p.base, p.R - 1, 63, std::max(qp, 0), pp.walk);
+    for (int block : {64, 128, 256}) {
+      Phase1Out out;
+      if (qp <= 0) break;
+      if (int r = run_walk_max(cx, p, pp, g, st, block, out, 1ull << 20)) return r;
+      lnorm_hip_tune_entry e{};
+      e.kernel = c.kernel;
+      e.block = block;
+      e.ms = out.ms_kernel;
+      e.labelings_per_s = (double)(out.units * g.seglen) / (out.ms_kernel / 1e3);
+      note(e);
+    }
+  }
+  return 0;
+}
+
 int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *optsp,
               int32_t *strategy, lnorm_hip_result *res) {
   const auto t_start = std::chrono::steady_clock::now();
@@ -597,42 +761,42 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
   std::memset(&rr, 0, sizeof rr);
   rr.total = p.total;
 
-  // choose the kernel
-  BlockedPlan bp;
-  std::string why;
-  int kernel = o.kernel;
-  const bool can_block = (kernel == LNORM_KERNEL_AUTO || kernel == LNORM_KERNEL_BLOCKED) && blocked_plan(p, o, bp, why);
-  const int ny_plain = plain_ny_pad(p.kind, p.n, p.C);
-  const bool fits16 = p.n >= 2 && max_col_abs_sum(p, 0) <= 32767;
-  const int w16 = fits16 ? plain16_words(p.n, p.C) : 0;
-  if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
-    return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
-  if (kernel == LNORM_KERNEL_PLAIN && !ny_plain)
-    return fail(LNORM_E_UNSUPPORTED, "no plain instantiation for %d columns at n = %d", p.C, p.n);
-  if (kernel == LNORM_KERNEL_PLAIN16 && !w16)
-    return fail(LNORM_E_UNSUPPORTED, "packed 16-bit walk needs n >= 2, column sums <= 32767 and an instantiated "
-                "width (%d columns at n = %d)", p.C, p.n);
-  if (kernel == LNORM_KERNEL_AUTO)
-    kernel = can_block ? LNORM_KERNEL_BLOCKED
-                       : (w16 ? LNORM_KERNEL_PLAIN16 : (ny_plain ? LNORM_KERNEL_PLAIN : LNORM_KERNEL_GENERAL));
-  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_PLAIN16)
-    return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
+  Choice ch;
+  if (int r = choose_kernel(p, o, ch)) return r;
+  if (o.autotune && (ch.kernel == LNORM_KERNEL_BLOCKED || ch.kernel == LNORM_KERNEL_PLAIN ||
+                     ch.kernel == LNORM_KERNEL_PLAIN16)) {
+    // launch shapes from the cache (tuned on the first miss); explicit
+    // options of the caller win
+    const TuneKey key = tune_key(dev, p, ch);
+    TuneVal tv;
+    bool hit;
+    {
+      std::lock_guard<std::mutex> tl(g_tune_mu);
+      auto it = g_tune.find(key);
+      hit = it != g_tune.end();
+      if (hit) tv = it->second;
+    }
+    if (!hit) {
+      if (int r = tune_single(cx, p, ch, o, st, tv, nullptr)) return r;
+      std::lock_guard<std::mutex> tl(g_tune_mu);
+      g_tune[key] = tv;
+    }
+    if (!o.patterns_per_lane && tv.P) o.patterns_per_lane = tv.P;
+    if (!o.wave_log2 && tv.s) o.wave_log2 = tv.s;
+    if (!o.block && tv.block) o.block = tv.block;
+    if (ch.kernel == LNORM_KERNEL_BLOCKED) {
+      std::string why;
+      if (!blocked_plan(p, o, ch.bp, why)) return fail(LNORM_E_UNSUPPORTED, "tuned blocked shape: %s", why.c_str());
+    }
+  }
+  const int kernel = ch.kernel;
+  const BlockedPlan &bp = ch.bp;
 
   // the walk that enumerates (non-blocked kernels) and recovers the strategy (all)
   Prepared pp;
-  if (kernel == LNORM_KERNEL_PLAIN16) prepare_walk(p, WALK_PLAIN16, 2 * w16, pp);
-  else if (kernel == LNORM_KERNEL_GENERAL || !ny_plain) prepare_walk(p, WALK_GENERAL, 0, pp);
-  else prepare_walk(p, WALK_PLAIN, ny_plain, pp);
+  prepare_for(p, ch, pp);
   const bool general = pp.walk == WALK_GENERAL;
-  const int nvec = (p.kind == KIND_L1 || p.kind == KIND_LM) ? 1 : p.n;
-  if (int r = ensure(cx.A, pp.A.size() * 4)) return r;
-  if (int r = ensure(cx.D, pp.D.size() * 4)) return r;
-  if (int r = ensure(cx.units, kUnitsPerChunk * 4)) return r;
-  if (int r = ensure(cx.scal, 256)) return r;
-  if (general)
-    if (int r = ensure(cx.state, (size_t)kGeneralSlices * nvec * p.C * 4)) return r;
-  HIPCHK(hipMemcpyAsync(cx.A.p, pp.A.data(), pp.A.size() * 4, hipMemcpyHostToDevice, st));
-  if (!pp.D.empty()) HIPCHK(hipMemcpyAsync(cx.D.p, pp.D.data(), pp.D.size() * 4, hipMemcpyHostToDevice, st));
+  if (int r = upload_walk(cx, p, pp, st)) return r;
 
   Phase1Out p1;
   int unit_e;   // log_base of the ranks per unit
@@ -733,6 +897,69 @@ Geometry batch_geometry(int base, int e, int forced, Walk walk, int cap_log) {
   return g;
 }
 
+// Batch shape for the tuner's cache: FNV-1a over the sorted item sizes.
+uint64_t batch_shape_key(const lnorm_hip_item *items, int count) {
+  std::vector<uint64_t> v((size_t)count);
+  for (int i = 0; i < count; ++i)
+    v[i] = ((uint64_t)(uint32_t)items[i].iRows << 40) ^ ((uint64_t)(uint32_t)items[i].iCols << 16) ^
+           ((uint64_t)(uint32_t)items[i].n << 4) ^ (uint64_t)(items[i].marginal & 1);
+  std::sort(v.begin(), v.end());
+  uint64_t h = 1469598103934665603ull;
+  for (uint64_t x : v)
+    for (int b = 0; b < 8; ++b) {
+      h ^= (x >> (8 * b)) & 0xFF;
+      h *= 1099511628211ull;
+    }
+  return h ^ (uint64_t)count;
+}
+std::map<uint64_t, std::pair<int, int>> g_tune_batch;   // shape -> (streams, cap_log), under g_tune_mu
+
+int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_hip_batch_stats *stats);
+
+// Times (streams, cap) candidates on a sample of the batch; stores the best.
+int tune_batch(const lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_hip_opts &best,
+               std::vector<lnorm_hip_tune_entry> *log) {
+  const int stride = count >= 64 ? 8 : 1;
+  std::vector<lnorm_hip_item> sample;
+  for (int i = 0; i < count; i += stride) {
+    lnorm_hip_item it = items[i];
+    it.strategy = nullptr;
+    sample.push_back(it);
+  }
+  best = o;
+  double best_ms = -1;
+  int ntimed = 0;
+  for (int streams : {1, 2, 4}) {
+    for (int cap : {10, 12, 14, 16}) {
+      lnorm_hip_opts oc = o;
+      oc.autotune = 0;
+      oc.batch_graph = 0;
+      oc.batch_streams = streams;
+      oc.batch_cap_log = cap;
+      lnorm_hip_batch_stats st;
+      std::vector<lnorm_hip_item> work = sample;
+      const int r = batch_impl(work.data(), (int)work.size(), oc, &st);
+      if (r == LNORM_E_HIP || r == LNORM_E_ARG) return r;
+      if (st.batched == 0) return 0;          // nothing in this batch runs batched
+      lnorm_hip_tune_entry e{};
+      e.batch_streams = streams;
+      e.batch_cap_log = cap;
+      e.ms = st.ms_device;
+      e.labelings_per_s = st.ms_device > 0 ? (double)st.labelings / (st.ms_device / 1e3) : 0;
+      if (log) log->push_back(e);
+      ++ntimed;
+      if (best_ms < 0 || st.ms_device < best_ms) {
+        best_ms = st.ms_device;
+        best.batch_streams = streams;
+        best.batch_cap_log = cap;
+      }
+    }
+  }
+  std::lock_guard<std::mutex> tl(g_tune_mu);
+  g_tune_batch[batch_shape_key(items, count)] = {best.batch_streams, best.batch_cap_log};
+  return ntimed;
+}
+
 int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_hip_batch_stats *stats) {
   const auto t_start = std::chrono::steady_clock::now();
   int first_err = 0;
@@ -742,8 +969,26 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
     return fail(LNORM_E_ARG, "batch_streams must be 0..%d", kSideStreams + 1);
   if (o.batch_cap_log != 0 && (o.batch_cap_log < 8 || o.batch_cap_log > 20))
     return fail(LNORM_E_ARG, "batch_cap_log must be 0 or 8..20");
-  const int cap_log = o.batch_cap_log ? o.batch_cap_log : kBatchCapLog;
-  const int nlanes = o.batch_streams ? o.batch_streams : kBatchStreams;
+  int cap_log = o.batch_cap_log ? o.batch_cap_log : kBatchCapLog;
+  int nlanes = o.batch_streams ? o.batch_streams : kBatchStreams;
+  if (o.autotune && !o.batch_cap_log && !o.batch_streams && count > 0) {
+    const uint64_t key = batch_shape_key(items, count);
+    std::pair<int, int> tuned{0, 0};
+    {
+      std::lock_guard<std::mutex> tl(g_tune_mu);
+      auto it = g_tune_batch.find(key);
+      if (it != g_tune_batch.end()) tuned = it->second;
+    }
+    if (!tuned.first) {
+      lnorm_hip_opts bestopt;
+      if (int r = tune_batch(items, count, o, bestopt, nullptr); r < 0) return r;
+      tuned = {bestopt.batch_streams, bestopt.batch_cap_log};
+    }
+    if (tuned.first) {
+      nlanes = tuned.first;
+      cap_log = tuned.second;
+    }
+  }
   for (int i = 0; i < count; ++i) {
     lnorm_hip_item &it = items[i];
     it.Lnorm = 0;
@@ -1043,6 +1288,58 @@ int lnorm_hip_calc_batch_ex(lnorm_hip_item *items, int count, const lnorm_hip_op
   lnorm_hip_default_opts(&o);
   if (opts) o = *opts;
   return batch_impl(items, count, o, stats);
+}
+
+int lnorm_hip_autotune(const int32_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *opts,
+                       lnorm_hip_opts *best, lnorm_hip_tune_entry *entries, int max_entries) {
+  g_err.clear();
+  lnorm_hip_opts o;
+  lnorm_hip_default_opts(&o);
+  if (opts) o = *opts;
+  if (max_entries < 0 || (max_entries > 0 && !entries)) return fail(LNORM_E_ARG, "bad entries buffer");
+  Problem p{A, R, C, n, marginal, KIND_L1, 2, 0};
+  if (int r = validate(p)) return r;
+  int dev = 0;
+  if (int r = select_device(o, dev)) return r;
+  DevCtx &cx = g_ctx[dev];
+  std::lock_guard<std::mutex> lock(cx.mu);
+  if (int r = ctx_init(cx)) return r;
+  hipStream_t st = o.stream ? (hipStream_t)o.stream : cx.stream;
+  Choice ch;
+  if (int r = choose_kernel(p, o, ch)) return r;
+  std::vector<lnorm_hip_tune_entry> log;
+  TuneVal tv;
+  if (int r = tune_single(cx, p, ch, o, st, tv, &log)) return r;
+  if (!log.empty()) {
+    std::lock_guard<std::mutex> tl(g_tune_mu);
+    g_tune[tune_key(dev, p, ch)] = tv;
+  }
+  if (best) {
+    *best = o;
+    best->kernel = ch.kernel;
+    if (tv.P) best->patterns_per_lane = tv.P;
+    if (tv.s) best->wave_log2 = tv.s;
+    if (tv.block) best->block = tv.block;
+  }
+  for (int i = 0; i < (int)log.size() && i < max_entries; ++i) entries[i] = log[i];
+  return (int)log.size();
+}
+
+int lnorm_hip_autotune_batch(const lnorm_hip_item *items, int count, const lnorm_hip_opts *opts,
+                             lnorm_hip_opts *best, lnorm_hip_tune_entry *entries, int max_entries) {
+  g_err.clear();
+  if (count < 0 || (count > 0 && !items)) return fail(LNORM_E_ARG, "bad batch");
+  if (max_entries < 0 || (max_entries > 0 && !entries)) return fail(LNORM_E_ARG, "bad entries buffer");
+  lnorm_hip_opts o;
+  lnorm_hip_default_opts(&o);
+  if (opts) o = *opts;
+  std::vector<lnorm_hip_tune_entry> log;
+  lnorm_hip_opts b;
+  const int r = tune_batch(items, count, o, b, &log);
+  if (r < 0) return r;
+  if (best) *best = b;
+  for (int i = 0; i < (int)log.size() && i < max_entries; ++i) entries[i] = log[i];
+  return (int)log.size();
 }
 
 const char *lnorm_hip_last_error(void) { return g_err.c_str(); }

@@ -175,6 +175,7 @@ __global__ __launch_bounds__(64) void k_table_mfma(const int8_t *__restrict__ lo
 }
 
 hipError_t launch_table(const TableArgs &a, hipStream_t st) {
+  if (!a.lo || !a.out) return hipErrorInvalidValue;
   const int patterns = 64 * a.P;
   const int mslots = (a.wide ? a.np : 2 * a.np) + (a.marginal ? 1 : 0);
   const dim3 grid((unsigned)(patterns / 32), (unsigned)((mslots + 31) / 32)), blk(64);
@@ -350,6 +351,7 @@ static hipError_t launch_b32n(int nc, int P, int W, const BlockedArgs &a, hipStr
 }
 
 hipError_t launch_blocked32(int nc, int P, int W, int marginal, const BlockedArgs &a, hipStream_t st) {
+  if (!a.table || !a.dpk || !a.seed || !a.dfull || !a.unit_best || !a.gbest) return hipErrorInvalidValue;
   if (!blocked32_supported(nc, P, W, marginal)) return hipErrorInvalidValue;
   return marginal ? launch_b32n<1>(nc, P, W, a, st) : launch_b32n<0>(nc, P, W, a, st);
 }
@@ -393,6 +395,7 @@ static hipError_t launch_bp(int np, const BlockedArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_blocked(int np, int P, int marginal, const BlockedArgs &a, hipStream_t st) {
+  if (!a.table || !a.dpk || !a.seed || !a.dfull || !a.unit_best || !a.gbest) return hipErrorInvalidValue;
   if (!blocked_supported(np, P, marginal)) return hipErrorInvalidValue;
   if (marginal) {
     switch (P) {

@@ -114,6 +114,7 @@ __global__ __launch_bounds__(64) void k_general(
 }
 
 hipError_t launch_general(Kind kind, Mode mode, const PlainArgs &a, int ny, hipStream_t st) {
+  if (!a.A || !a.state || !a.unit_best || !a.gbest || !a.gfirst) return hipErrorInvalidValue;
   // one wave per segment; the state scratch holds grid * nvec * ny words
   uint64_t seglen = 1;
   const int base = a.d < 2 ? 2 : a.d;
@@ -176,15 +177,18 @@ static unsigned grid_for(uint64_t n, int block) {
 }
 
 hipError_t launch_fill_i32(int32_t *p, int32_t v, uint64_t n, hipStream_t st) {
+  if (!p && n) return hipErrorInvalidValue;   // an unallocated buffer: an error, never a launch
   hipLaunchKernelGGL(k_fill_i32, dim3(grid_for(n, 256)), dim3(256), 0, st, p, v, n);
   return hipGetLastError();
 }
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t n, hipStream_t st) {
+  if (!p && n) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(n, 256)), dim3(256), 0, st, p, v, n);
   return hipGetLastError();
 }
 hipError_t launch_first_unit(const int32_t *ub, uint64_t n, const int32_t *gbest, unsigned long long *first,
                              hipStream_t st) {
+  if (!ub || !gbest || !first) return hipErrorInvalidValue;
   hipLaunchKernelGGL(k_first_unit, dim3(grid_for(n, 256)), dim3(256), 0, st, ub, n, gbest, first);
   return hipGetLastError();
 }

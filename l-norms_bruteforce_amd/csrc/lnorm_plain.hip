@@ -86,6 +86,7 @@ static hipError_t launch_p16(Mode mode, const PlainArgs &a, hipStream_t st, int 
 }
 
 hipError_t launch_plain16(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
+  if (!a.A || !a.unit_best || !a.gbest || !a.gfirst) return hipErrorInvalidValue;
   if (block <= 0 || block % 64 || block > 256) block = 256;
   if (!plain16_words(a.d, a.ny_pad) || plain16_words(a.d, a.ny_pad) * 2 != a.ny_pad) return hipErrorInvalidValue;
   switch (a.d) {
@@ -193,6 +194,7 @@ static hipError_t launch_dary(Mode mode, const PlainArgs &a, hipStream_t st, int
 }
 
 hipError_t launch_plain(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block) {
+  if (!a.A || (kind != KIND_LD && !a.D) || !a.unit_best || !a.gbest || !a.gfirst) return hipErrorInvalidValue;
   // the kernels are compiled for at most 256 threads (__launch_bounds__)
   if (block <= 0 || block % 64 || block > 256) block = 256;
   if (!plain_supported(kind, a.d, a.ny_pad)) return hipErrorInvalidValue;

@@ -22,7 +22,32 @@ def test_hip_library_exports_header(pkg):
     missing = [n for n in names if not hasattr(lib, n)]
     assert not missing, missing
     assert set(names) == set(pkg.HIP_SYMBOLS)
-    assert lib.lnorm_hip_abi_version() == 1
+    version = int(re.search(r"#define LNORM_HIP_ABI_VERSION (\d+)",
+                            open(os.path.join(ROOT, "include", "lnorm_hip.h")).read()).group(1))
+    assert lib.lnorm_hip_abi_version() == version
+
+
+def _struct_fields(header, name):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    body = re.search(r"typedef struct %s \{(.*?)\} %s;" % (name, name), text, flags=re.S).group(1)
+    fields = []
+    for decl in body.split(";"):
+        decl = decl.strip()
+        if not decl:
+            continue
+        names = decl.split(None, 1)[1] if not decl.startswith("const") else decl.split(None, 2)[2]
+        for nm in names.split(","):
+            fields.append(re.sub(r"\[.*?\]|[*\s]", "", nm))
+    return fields
+
+
+@pytest.mark.parametrize("cname,pyname", [("lnorm_hip_opts", "Opts"), ("lnorm_hip_result", "Result"),
+                                          ("lnorm_hip_item", "Item"), ("lnorm_hip_batch_stats", "BatchStats"),
+                                          ("lnorm_hip_tune_entry", "TuneEntry")])
+def test_ctypes_structs_match_header(pkg, cname, pyname):
+    """The Python views of the ABI structs list the header's fields in order."""
+    assert [f for f, _ in getattr(pkg, pyname)._fields_] == _struct_fields("lnorm_hip.h", cname)
 
 
 def test_host_library_exports_header(pkg):
