@@ -30,12 +30,16 @@ PKG_DIR = os.path.join(ROOT, "l-norms_bruteforce_amd")
 METRIC = "exact L_1 sign-vectors/s on 1 MI355X (int8-MFMA vs plain Gray), bit-exact"
 VALU_PEAK = 256 * 4 * 16 * 2.4e9        # int32 lane-ops/s: 256 CUs x 4 SIMD16 x 2.4 GHz
 MFMA_I8_PEAK = 5.0e15                   # dense int8 ops/s
-# HBM bytes per launch of the headline instantiation, measured with
+# HBM bytes per launch of the headline instantiations, measured with
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE on `bench.py --case blocked`
-# (207.6 KiB and 20480 KiB), each scaled by the counter's reading for a 1 GiB
-# device-to-device copy (x2.0 for FETCH_SIZE, x1.0 for WRITE_SIZE).  Only the
-# per-wave maxima reach HBM; the matrix and the pattern table stay on chip.
-PROFILED_TRAFFIC = {"blocked16<np=32,P=4,marg=0": 425216 + 20971520}
+# (with and without --no-autotune), each scaled by the counter's reading for a
+# 1 GiB device-to-device copy (x2.0 for FETCH_SIZE, x1.0 for WRITE_SIZE):
+#   P=2, s=11 (autotuned): 128.5 KiB and 40960 KiB per launch of 2^20 waves;
+#   P=4, s=12 (default):   207.6 KiB and 20480 KiB per launch of 2^19 waves.
+# Only the per-wave maxima reach HBM; the matrix and the pattern table stay
+# on chip.
+PROFILED_TRAFFIC = {"blocked16<np=32,P=2,marg=0,s=11>": 263168 + 41943040,
+                    "blocked16<np=32,P=4,marg=0,s=12>": 425216 + 20971520}
 
 
 def load_pkg():
@@ -62,16 +66,27 @@ def valu_ops_per_unit(kernel: str, ny: int, marginal: int) -> int:
     return 2 * ny + 1
 
 
+AUTOTUNE = True    # bench.py --no-autotune clears it
+
+
 def run_steps(pkg, A, kernel, steps, warmup, P=0):
+    """`steps` timed calls (norm + strategy) after `warmup` untimed ones.
+    With autotuning, the launch shape is tuned once before any of them
+    (lnorm_hip_autotune, cached) and every call uses the cached shape."""
     res = None
+    tuned = None
+    if AUTOTUNE:
+        best, _ = pkg.autotune(A, 1, 0, kernel=kernel)
+        tuned = {k: best[k] for k in ("patterns_per_lane", "wave_log2", "block")}
+    kw = {"kernel": kernel, "patterns_per_lane": P, "autotune": 1 if AUTOTUNE else 0}
     for _ in range(warmup):
-        pkg.calc_lnorm(A, 1, 0, kernel=kernel, patterns_per_lane=P)
+        pkg.calc_lnorm(A, 1, 0, **kw)
     times, kms = [], []
     out = None
     t0 = time.perf_counter()
     for _ in range(steps):
         t = time.perf_counter()
-        norm, strat, res = pkg.calc_lnorm(A, 1, 0, kernel=kernel, patterns_per_lane=P)
+        norm, strat, res = pkg.calc_lnorm(A, 1, 0, **kw)
         times.append(time.perf_counter() - t)
         kms.append(res["ms_kernel"])
         if out is None:
@@ -80,7 +95,7 @@ def run_steps(pkg, A, kernel, steps, warmup, P=0):
             raise RuntimeError("result changed between steps")
     wall = time.perf_counter() - t0
     return {"norm": out[0], "best_rank": out[2], "strategy": out[1], "wall_s": wall, "step_s": times,
-            "kernel_ms": kms, "res": res}
+            "kernel_ms": kms, "res": res, "tuned": tuned}
 
 
 def summarize(r, total, ny, kernel, marginal=0):
@@ -189,10 +204,14 @@ def batch_line(pkg, gen, steps: int) -> dict:
     eager, st0 = pkg.calc_batch(probs, stats=True)
     if any(s != 0 for _, _, s in eager):
         raise RuntimeError("batch item failed")
+    tuned = None
+    if AUTOTUNE:
+        best, _ = pkg.autotune_batch(probs)
+        tuned = {k: best[k] for k in ("batch_streams", "batch_cap_log")}
     times, dev = [], []
     for _ in range(max(1, steps)):
         t = time.perf_counter()
-        out, st = pkg.calc_batch(probs, stats=True, batch_graph=1)
+        out, st = pkg.calc_batch(probs, stats=True, batch_graph=1, autotune=1 if AUTOTUNE else 0)
         times.append(time.perf_counter() - t)
         dev.append(st["ms_device"])
         if out != eager:
@@ -206,7 +225,35 @@ def batch_line(pkg, gen, steps: int) -> dict:
             "device_labelings_per_s": lab / (msd / 1e3), "eager_ms_device": st0["ms_device"],
             "row_updates_per_s": row_updates / (msd / 1e3),
             "valu_TOPs": ops / (msd / 1e3) / 1e12, "valu_frac": ops / (msd / 1e3) / VALU_PEAK,
-            "graph_replayed": st["graph_replayed"]}
+            "graph_replayed": st["graph_replayed"], "autotuned": tuned}
+
+
+def tune_sweep(pkg, gen) -> dict:
+    """Every autotuner candidate on the bench workloads: labelings/s of each
+    probe and the shape the tuner keeps (lnorm_hip_autotune and
+    lnorm_hip_autotune_batch)."""
+    from lnorm_amd import host
+    out = {}
+    cases = {
+        "L1_64x40_int8": (gen.gen_matrix(64, 40, 127, 104).T.copy(), 1, 0, {}),
+        "L1_40x40": (gen.gen_matrix(40, 40, 100, 106), 1, 0, {}),
+        "L1_128x40_int16": (gen.gen_matrix(128, 40, 30000, 105).T.copy(), 1, 0, {}),
+        "LM_36x36": (gen.gen_matrix(36, 36, 100, 102), 1, 1, {}),
+        "L1_64x40_plain": (gen.gen_matrix(64, 40, 127, 104).T.copy(), 1, 0, {"kernel": "plain"}),
+        "L3_18x32": (gen.gen_matrix(18, 32, 50, 103), 3, 0, {}),
+    }
+    for name, (A, n, m, kw) in cases.items():
+        best, entries = pkg.autotune(A, n, m, **kw)
+        out[name] = {"best": {k: best[k] for k in ("kernel", "patterns_per_lane", "wave_log2", "block")},
+                     "candidates": entries}
+    probs = []
+    for w in gen.batch_workload(4096):
+        p, _, exited = host.load(w["text"], w["d"], w["stat"], w["marginal"])
+        probs.append((p.matrix_laid_out(), p.n, p.marginal))
+        host.free(p)
+    best, entries = pkg.autotune_batch(probs)
+    out["batch_4096"] = {"best": {k: best[k] for k in ("batch_streams", "batch_cap_log")}, "candidates": entries}
+    return out
 
 
 def run_case(pkg, gen, case: str, steps: int, warmup: int) -> int:
@@ -229,6 +276,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-extra", action="store_true", help="skip the plain kernel and the 40 x 40 line")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="run the default launch shapes instead of the autotuner's (tuned before timing)")
+    ap.add_argument("--tune", action="store_true",
+                    help="sweep the autotuner's candidates on the bench workloads and print them as JSON")
     ap.add_argument("--case", choices=["blocked", "plain", "batch"],
                     help="run only this workload (for profiler passes) and print its figures")
     ap.add_argument("--copy-calibration", type=int, metavar="MIB",
@@ -249,10 +300,15 @@ def main():
         print(json.dumps({"copy_bytes_read": n * 4, "copy_bytes_written": n * 4,
                           "ok": bool(torch.equal(dst[-16:].cpu(), src[-16:].cpu()))}), flush=True)
         return 0
+    global AUTOTUNE
+    AUTOTUNE = not a.no_autotune
     pkg = load_pkg()
     from lnorm_amd import gen
     if a.case:
         return run_case(pkg, gen, a.case, a.steps, a.warmup)
+    if a.tune:
+        print(json.dumps(tune_sweep(pkg, gen)), flush=True)
+        return 0
 
     A = gen.gen_matrix(64, 40, 127, 104).T.copy()       # laid out 40 x 64 as the reference does
     k, ny = A.shape
@@ -277,6 +333,7 @@ def main():
                         "per step, norm + strategy",
             "k": k, "n_y": ny, "sign_vectors_per_step": total,
             "kernel": sb["kernel_name"],
+            "autotuned": blk["tuned"],
             "arith": "int8 MFMA (int32 accumulate) builds the low-row table; v_sad_u16 on biased 16-bit halves "
                      "into uint32 sums; exact",
             "parallelism": "single GPU",
@@ -287,7 +344,8 @@ def main():
             "peak": VALU_PEAK / 1e12,
             "unit": "TOP/s",
             "frac": sb["valu_frac"],
-            "traffic": next((v for k2, v in PROFILED_TRAFFIC.items() if sb["kernel_name"].startswith(k2)), None),
+            "traffic": PROFILED_TRAFFIC.get(sb["kernel_name"]),
+            "traffic_per": "launch (2^20 waves at most per launch)",
             "ops_per_sign_vector": sb["valu_ops_per_sign_vector"],
             "kernel_ms": sb["kernel_ms"],
         },
