@@ -57,6 +57,14 @@ def lib():
         L.lo_brute.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p, u64p]
         L.lo_objective.restype = ctypes.c_int32
         L.lo_objective.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.lo_calc_lnorm64.restype = ctypes.c_int
+        L.lo_calc_lnorm64.argtypes = [i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, i64p, i32p, u64p]
+        L.lo_brute64.restype = ctypes.c_int
+        L.lo_brute64.argtypes = [i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i64p, u64p]
+        L.lo_objective64.restype = ctypes.c_int64
+        L.lo_objective64.argtypes = [i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p]
         _lib = L
     return _lib
 
@@ -127,3 +135,41 @@ def objective(A, n: int, marginal: int, labels) -> int:
     R, C = a.shape
     lab = np.ascontiguousarray(np.asarray(list(labels) + [0], dtype=np.int32))
     return int(lib().lo_objective(p, R, C, n, marginal, lab.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))
+
+
+# ---- the int_type = long long variant (README.md:100-105) -------------------
+
+def _arr64(A):
+    a = np.ascontiguousarray(np.asarray(A, dtype=np.int64))
+    return a, a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64))
+
+
+def calc_lnorm64(A, n: int, marginal: int = 0, threads: int = 8):
+    """(norm, strategy, rank) in 64-bit arithmetic for an int64 matrix."""
+    a, p = _arr64(A)
+    R, C = a.shape
+    norm = ctypes.c_int64()
+    rank = ctypes.c_uint64()
+    strat = np.zeros(max(R - 1, 1), dtype=np.int32)
+    rc = lib().lo_calc_lnorm64(p, R, C, n, marginal, threads, ctypes.byref(norm),
+                               strat.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)), ctypes.byref(rank))
+    if rc != 0:
+        raise RuntimeError("lo_calc_lnorm64 failed")
+    return int(norm.value), [int(v) for v in strat[: R - 1]], int(rank.value)
+
+
+def brute64(A, n: int, marginal: int = 0):
+    a, p = _arr64(A)
+    R, C = a.shape
+    norm = ctypes.c_int64()
+    rank = ctypes.c_uint64()
+    if lib().lo_brute64(p, R, C, n, marginal, ctypes.byref(norm), ctypes.byref(rank)) != 0:
+        raise RuntimeError("lo_brute64 failed")
+    return int(norm.value), int(rank.value)
+
+
+def objective64(A, n: int, marginal: int, labels) -> int:
+    a, p = _arr64(A)
+    R, C = a.shape
+    lab = np.ascontiguousarray(np.asarray(list(labels) + [0], dtype=np.int32))
+    return int(lib().lo_objective64(p, R, C, n, marginal, lab.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))))

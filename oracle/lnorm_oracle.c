@@ -13,11 +13,19 @@
  *   - lo_brute: closed forms and hand-checked tiny cases; it shares no code
  *     path with the Gray walks (direct digit decoding, full recompute).
  *
- * Integer arithmetic: the reference computes in int (int_type,
- * functions.h:5).  The engine's contract requires sum|A| <= INT32_MAX, under
- * which every partial sum and objective fits in int32; here updates are
- * done in uint32 (wrapping, so 2*A[i] never invokes signed-overflow UB) and
- * objectives are summed in int64, which is identical inside the contract.
+ * Integer arithmetic: the reference computes in int_type (functions.h:5),
+ * int by default and long long in its documented variant (README.md:100-105,
+ * with abs -> llabs).  The walks here run in 64-bit two's complement
+ * (uint64 updates, so 2*A[i] never invokes signed-overflow UB): that is the
+ * long long variant exactly whenever sum|A| <= INT64_MAX, and the int
+ * version exactly whenever sum|A| <= INT32_MAX (every partial sum and
+ * objective then fits in int32).  The int32 entry points widen the matrix and
+ * narrow the results; the *64 entry points are the long long variant.
+ *
+ * Extra pins of the 64-bit arithmetic (tests/test_oracle.py): the scaling
+ * identity L(c*A) = |c| L(A) with |c| * L(A) > 2^31, the rank-one closed form
+ * with entries beyond int32, and lo_brute64 against lo_calc_lnorm64 on
+ * matrices with entries up to 2^40.
  */
 #include "lnorm_oracle.h"
 
@@ -29,8 +37,6 @@
 #endif
 
 #define LO_MAX_LABELS 10   /* RANK_OF_NORM, functions.h:3 */
-
-static inline int32_t iabs32(int32_t v) { return v < 0 ? -v : v; }
 
 uint64_t lo_total(int R, int n)
 {
@@ -71,107 +77,109 @@ int lo_digit(uint64_t N, int i, int n)
     return r < (uint64_t)n ? (int)r : (int)(2 * n - 1 - r);
 }
 
-int32_t lo_objective(const int32_t *A, int R, int C, int n, int marginal, const int32_t *labels)
+static inline int64_t iabs64(int64_t v) { return v < 0 ? -v : v; }
+
+int64_t lo_objective64(const int64_t *A, int R, int C, int n, int marginal, const int32_t *labels)
 {
     int64_t total = 0;
     if (n == 1) {
         for (int l = 0; l < C; ++l) {
-            uint32_t y = (uint32_t)A[(size_t)(R - 1) * C + l];
+            uint64_t y = (uint64_t)A[(size_t)(R - 1) * C + l];
             for (int i = 0; i < R - 1; ++i) {
-                uint32_t a = (uint32_t)A[(size_t)i * C + l];
+                uint64_t a = (uint64_t)A[(size_t)i * C + l];
                 y = labels[i] ? y + a : y - a;
             }
-            int32_t v = (int32_t)y;
-            total += (marginal && l == 0) ? v : iabs32(v);
+            int64_t v = (int64_t)y;
+            total += (marginal && l == 0) ? v : iabs64(v);
         }
-        return (int32_t)total;
+        return total;
     }
     for (int c = 0; c < n; ++c) {
         for (int l = 0; l < C; ++l) {
-            uint32_t t = c == 0 ? (uint32_t)A[(size_t)(R - 1) * C + l] : 0u;
+            uint64_t t = c == 0 ? (uint64_t)A[(size_t)(R - 1) * C + l] : 0u;
             for (int i = 0; i < R - 1; ++i)
-                if (labels[i] == c) t += (uint32_t)A[(size_t)i * C + l];
-            total += iabs32((int32_t)t);
+                if (labels[i] == c) t += (uint64_t)A[(size_t)i * C + l];
+            total += iabs64((int64_t)t);
         }
     }
-    return (int32_t)total;
+    return total;
 }
 
 /* ---- the Gray walks (L_OpenMP.c:15-239) ---------------------------------- */
 
-static int32_t sum_abs_l1(const uint32_t *y, int C, int marginal)
+static int64_t sum_abs_l1(const uint64_t *y, int C, int marginal)
 {
     int64_t s = 0;
     for (int l = 0; l < C; ++l) {
-        int32_t v = (int32_t)y[l];
-        s += (marginal && l == 0) ? v : iabs32(v);
+        int64_t v = (int64_t)y[l];
+        s += (marginal && l == 0) ? v : iabs64(v);
     }
-    return (int32_t)s;
+    return s;
 }
 
-static int32_t sum_abs_labels(const uint32_t *t, int n, int C)
+static int64_t sum_abs_labels(const uint64_t *t, int n, int C)
 {
     int64_t s = 0;
     for (int c = 0; c < n; ++c)
-        for (int l = 0; l < C; ++l) s += iabs32((int32_t)t[(size_t)c * C + l]);
-    return (int32_t)s;
+        for (int l = 0; l < C; ++l) s += iabs64((int64_t)t[(size_t)c * C + l]);
+    return s;
 }
 
-int lo_walk(const int32_t *A, int R, int C, int n, int marginal,
-            uint64_t jmin, uint64_t jmax,
-            int32_t *best_out, uint64_t *best_rank, int32_t *strategy)
+int lo_walk64(const int64_t *A, int R, int C, int n, int marginal,
+              uint64_t jmin, uint64_t jmax,
+              int64_t *best_out, uint64_t *best_rank, int32_t *strategy)
 {
     if (R < 1 || C < 1 || n < 1 || n > LO_MAX_LABELS || jmax < jmin) return -1;
     const int nv = R - 1;
     const size_t Cs = (size_t)C;
     int32_t vect[64];
-    int32_t best;
+    int64_t best;
     uint64_t brank = jmin;
 
     if (n <= 2) {
         /* binary reflected Gray code: L1/LM (L_OpenMP.c:15-85), L2 (87-124) */
-        uint32_t *t = (uint32_t *)calloc(2 * Cs, sizeof(uint32_t));
+        uint64_t *t = (uint64_t *)calloc(2 * Cs, sizeof(uint64_t));
         if (!t) return -1;
-        uint32_t *t0 = t, *t1 = t + Cs;  /* L1: t0 is y; L2: t0,t1 */
-        for (int l = 0; l < C; ++l) t0[l] = (uint32_t)A[(size_t)nv * Cs + l];
+        uint64_t *t0 = t, *t1 = t + Cs;  /* L1: t0 is y; L2: t0,t1 */
+        for (int l = 0; l < C; ++l) t0[l] = (uint64_t)A[(size_t)nv * Cs + l];
         for (int i = 0; i < nv; ++i) {
             vect[i] = lo_digit(jmin, i, 2);
-            const int32_t *row = A + (size_t)i * Cs;
+            const int64_t *row = A + (size_t)i * Cs;
             for (int l = 0; l < C; ++l) {
-                if (n == 1) t0[l] = vect[i] ? t0[l] + (uint32_t)row[l] : t0[l] - (uint32_t)row[l];
-                else if (vect[i]) t1[l] += (uint32_t)row[l];
-                else t0[l] += (uint32_t)row[l];
+                if (n == 1) t0[l] = vect[i] ? t0[l] + (uint64_t)row[l] : t0[l] - (uint64_t)row[l];
+                else if (vect[i]) t1[l] += (uint64_t)row[l];
+                else t0[l] += (uint64_t)row[l];
             }
         }
         best = n == 1 ? sum_abs_l1(t0, C, marginal) : sum_abs_labels(t, 2, C);
         if (strategy) memcpy(strategy, vect, sizeof(int32_t) * (size_t)nv);
         for (uint64_t N = jmin + 1; N <= jmax && N != 0; ++N) {
             int i = __builtin_ctzll(N);
-            const uint32_t *row = (const uint32_t *)A + (size_t)i * Cs;
+            const uint64_t *row = (const uint64_t *)A + (size_t)i * Cs;
             int up = vect[i] == 0;
             vect[i] = up;
             int64_t prod = 0;
             if (n == 1) {
                 /* L_OpenMP.c:77-78 (LM: 40-41, column 0 without abs) */
-                uint32_t m = up ? 2u : (uint32_t)-2;
+                uint64_t m = up ? 2u : (uint64_t)-2;
                 int l0 = 0;
-                if (marginal) { t0[0] += m * row[0]; prod += (int32_t)t0[0]; l0 = 1; }
+                if (marginal) { t0[0] += m * row[0]; prod += (int64_t)t0[0]; l0 = 1; }
                 for (int l = l0; l < C; ++l) {
                     t0[l] += m * row[l];
-                    prod += iabs32((int32_t)t0[l]);
+                    prod += iabs64((int64_t)t0[l]);
                 }
             } else {
                 /* L_OpenMP.c:116-117 */
-                uint32_t m = up ? 1u : (uint32_t)-1;
+                uint64_t m = up ? 1u : (uint64_t)-1;
                 for (int l = 0; l < C; ++l) {
-                    uint32_t a = m * row[l];
+                    uint64_t a = m * row[l];
                     t1[l] += a;
                     t0[l] -= a;
-                    prod += iabs32((int32_t)t0[l]) + iabs32((int32_t)t1[l]);
+                    prod += iabs64((int64_t)t0[l]) + iabs64((int64_t)t1[l]);
                 }
             }
-            if ((int32_t)prod > best) {
-                best = (int32_t)prod;
+            if (prod > best) {
+                best = prod;
                 brank = N;
                 if (strategy) memcpy(strategy, vect, sizeof(int32_t) * (size_t)nv);
             }
@@ -183,13 +191,13 @@ int lo_walk(const int32_t *A, int R, int C, int n, int marginal,
         uint64_t pw[64];
         pw[0] = 1;
         for (int i = 1; i < 64; ++i) pw[i] = pw[i - 1] * (uint64_t)n;
-        uint32_t *t = (uint32_t *)calloc((size_t)n * Cs, sizeof(uint32_t));
+        uint64_t *t = (uint64_t *)calloc((size_t)n * Cs, sizeof(uint64_t));
         if (!t) return -1;
-        for (int l = 0; l < C; ++l) t[l] = (uint32_t)A[(size_t)nv * Cs + l];
+        for (int l = 0; l < C; ++l) t[l] = (uint64_t)A[(size_t)nv * Cs + l];
         for (int i = 0; i < nv; ++i) {
             vect[i] = lo_digit(jmin, i, n);
-            const int32_t *row = A + (size_t)i * Cs;
-            for (int l = 0; l < C; ++l) t[(size_t)vect[i] * Cs + l] += (uint32_t)row[l];
+            const int64_t *row = A + (size_t)i * Cs;
+            for (int l = 0; l < C; ++l) t[(size_t)vect[i] * Cs + l] += (uint64_t)row[l];
         }
         best = sum_abs_labels(t, n, C);
         if (strategy) memcpy(strategy, vect, sizeof(int32_t) * (size_t)nv);
@@ -199,14 +207,14 @@ int lo_walk(const int32_t *A, int R, int C, int n, int marginal,
                 if (q % (uint64_t)n) {
                     uint64_t r = q % (uint64_t)(2 * n);
                     int nw = r < (uint64_t)n ? (int)r : (int)(2 * n - 1 - r);
-                    const int32_t *row = A + (size_t)i * Cs;
-                    uint32_t *from = t + (size_t)vect[i] * Cs, *to = t + (size_t)nw * Cs;
-                    for (int l = 0; l < C; ++l) { from[l] -= (uint32_t)row[l]; to[l] += (uint32_t)row[l]; }
+                    const int64_t *row = A + (size_t)i * Cs;
+                    uint64_t *from = t + (size_t)vect[i] * Cs, *to = t + (size_t)nw * Cs;
+                    for (int l = 0; l < C; ++l) { from[l] -= (uint64_t)row[l]; to[l] += (uint64_t)row[l]; }
                     vect[i] = nw;
                     break;
                 }
             }
-            int32_t prod = sum_abs_labels(t, n, C);
+            int64_t prod = sum_abs_labels(t, n, C);
             if (prod > best) {
                 best = prod;
                 brank = N;
@@ -221,15 +229,15 @@ int lo_walk(const int32_t *A, int R, int C, int n, int marginal,
     return 0;
 }
 
-int lo_calc_lnorm(const int32_t *A, int R, int C, int n, int marginal, int threads,
-                  int32_t *norm, int32_t *strategy, uint64_t *rank_out)
+int lo_calc_lnorm64(const int64_t *A, int R, int C, int n, int marginal, int threads,
+                    int64_t *norm, int32_t *strategy, uint64_t *rank_out)
 {
     uint64_t total = lo_total(R, n);
     if (total == 0 || threads < 1) return -1;
     uint64_t copy = (uint64_t)threads > total ? total : (uint64_t)threads;  /* L_OpenMP.c:368 */
     uint64_t steps = total / copy, rem = total % copy;                       /* :369-370 */
     int nv = R - 1 > 0 ? R - 1 : 0;
-    int32_t *vals = (int32_t *)malloc(sizeof(int32_t) * copy);
+    int64_t *vals = (int64_t *)malloc(sizeof(int64_t) * copy);
     uint64_t *ranks = (uint64_t *)malloc(sizeof(uint64_t) * copy);
     int32_t *strat = (int32_t *)calloc(copy * (size_t)(nv ? nv : 1), sizeof(int32_t));
     int err = 0;
@@ -238,8 +246,8 @@ int lo_calc_lnorm(const int32_t *A, int R, int C, int n, int marginal, int threa
     for (long long w = 0; w < (long long)copy; ++w) {
         uint64_t jmin, jmax;
         lo_segment((int)w, steps, rem, &jmin, &jmax);
-        err |= lo_walk(A, R, C, n, marginal, jmin, jmax, &vals[w], &ranks[w],
-                       strat + (size_t)w * (size_t)nv);
+        err |= lo_walk64(A, R, C, n, marginal, jmin, jmax, &vals[w], &ranks[w],
+                         strat + (size_t)w * (size_t)nv);
     }
     if (!err) {
         /* L_OpenMP.c:291-294: first maximum wins */
@@ -254,13 +262,13 @@ int lo_calc_lnorm(const int32_t *A, int R, int C, int n, int marginal, int threa
     return err ? -1 : 0;
 }
 
-int lo_brute(const int32_t *A, int R, int C, int n, int marginal, int32_t *norm, uint64_t *rank_out)
+int lo_brute64(const int64_t *A, int R, int C, int n, int marginal, int64_t *norm, uint64_t *rank_out)
 {
     uint64_t total = lo_total(R, n);
     if (total == 0 || n > LO_MAX_LABELS) return -1;
     int nv = R - 1;
     int32_t labels[64];
-    int32_t best = 0;
+    int64_t best = 0;
     uint64_t brank = 0;
     for (uint64_t N = 0; N < total; ++N) {
         for (int i = 0; i < nv; ++i) {
@@ -275,10 +283,66 @@ int lo_brute(const int32_t *A, int R, int C, int n, int marginal, int32_t *norm,
                 labels[i] = (int32_t)((blk & 1) ? (uint64_t)(n - 1) - r : r);
             }
         }
-        int32_t v = lo_objective(A, R, C, n, marginal, labels);
+        int64_t v = lo_objective64(A, R, C, n, marginal, labels);
         if (N == 0 || v > best) { best = v; brank = N; }
     }
     *norm = best;
     if (rank_out) *rank_out = brank;
     return 0;
+}
+
+/* ---- int (the reference default): widen, walk in 64 bits, narrow -------- */
+
+static int64_t *widen(const int32_t *A, int R, int C)
+{
+    if (R < 1 || C < 1) return NULL;
+    int64_t *w = (int64_t *)malloc(sizeof(int64_t) * (size_t)R * (size_t)C);
+    if (!w) return NULL;
+    for (size_t i = 0; i < (size_t)R * (size_t)C; ++i) w[i] = A[i];
+    return w;
+}
+
+int32_t lo_objective(const int32_t *A, int R, int C, int n, int marginal, const int32_t *labels)
+{
+    int64_t *w = widen(A, R, C);
+    if (!w) return 0;
+    int64_t v = lo_objective64(w, R, C, n, marginal, labels);
+    free(w);
+    return (int32_t)v;
+}
+
+int lo_walk(const int32_t *A, int R, int C, int n, int marginal,
+            uint64_t jmin, uint64_t jmax,
+            int32_t *best_out, uint64_t *best_rank, int32_t *strategy)
+{
+    int64_t *w = widen(A, R, C);
+    if (!w) return -1;
+    int64_t b = 0;
+    int r = lo_walk64(w, R, C, n, marginal, jmin, jmax, &b, best_rank, strategy);
+    free(w);
+    if (!r) *best_out = (int32_t)b;
+    return r;
+}
+
+int lo_calc_lnorm(const int32_t *A, int R, int C, int n, int marginal, int threads,
+                  int32_t *norm, int32_t *strategy, uint64_t *rank_out)
+{
+    int64_t *w = widen(A, R, C);
+    if (!w) return -1;
+    int64_t b = 0;
+    int r = lo_calc_lnorm64(w, R, C, n, marginal, threads, &b, strategy, rank_out);
+    free(w);
+    if (!r) *norm = (int32_t)b;
+    return r;
+}
+
+int lo_brute(const int32_t *A, int R, int C, int n, int marginal, int32_t *norm, uint64_t *rank_out)
+{
+    int64_t *w = widen(A, R, C);
+    if (!w) return -1;
+    int64_t b = 0;
+    int r = lo_brute64(w, R, C, n, marginal, &b, rank_out);
+    free(w);
+    if (!r) *norm = (int32_t)b;
+    return r;
 }

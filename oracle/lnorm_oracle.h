@@ -7,6 +7,7 @@
  * the L_HIP host) never links it.
  *
  * Matrix convention: A is the reference's mtx_as_vec, row-major R x C int32
+ * (int64 for the *64 functions)
  * (functions.h:291-329).  Row R-1 carries the fixed label (+1 for d=1,
  * label 0 for d>=2); rows 0..R-2 are enumerated.  n is the effective order
  * (item_calc.n after calc_Parameters, L_OpenMP.c:346-350).
@@ -56,6 +57,19 @@ int lo_brute(const int32_t *A, int R, int C, int n, int marginal,
  * sum_c sum_l |t_{c,l}| (n>=2). */
 int32_t lo_objective(const int32_t *A, int R, int C, int n, int marginal,
                      const int32_t *labels);
+
+/* The int_type = long long variant (README.md:100-105, abs -> llabs):
+ * the same functions on an int64 matrix with int64 objectives.  The int32
+ * entry points above widen the matrix and call these. */
+int lo_walk64(const int64_t *A, int R, int C, int n, int marginal,
+              uint64_t jmin, uint64_t jmax,
+              int64_t *best, uint64_t *best_rank, int32_t *strategy);
+int lo_calc_lnorm64(const int64_t *A, int R, int C, int n, int marginal, int threads,
+                    int64_t *norm, int32_t *strategy, uint64_t *rank_out);
+int lo_brute64(const int64_t *A, int R, int C, int n, int marginal,
+               int64_t *norm, uint64_t *rank_out);
+int64_t lo_objective64(const int64_t *A, int R, int C, int n, int marginal,
+                       const int32_t *labels);
 
 #ifdef __cplusplus
 }

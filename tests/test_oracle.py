@@ -205,3 +205,49 @@ def test_host_preprocessing_preserves_norm():
                     A = np.array(p.A, dtype=np.int32).reshape(p.R, p.C)
                     norms.add(oracle.calc_lnorm(A, p.n, p.marginal, 4)[0])
                 assert len(norms) == 1, (M.tolist(), d, marg, norms)
+
+
+# ---- the int_type = long long variant -------------------------------------
+
+@pytest.mark.parametrize("n,marg", [(1, 0), (1, 1), (2, 0), (3, 0), (4, 0)])
+def test_int64_scaling_beyond_int32(n, marg):
+    """L(c*A) = c*L(A) for c > 0 (same maximising rank) with c*L(A) far
+    beyond int32: a narrowing anywhere in the 64-bit walk breaks it."""
+    rng = np.random.default_rng(40 + n + marg)
+    R = {1: 9, 2: 8, 3: 6, 4: 5}[n]
+    A = rng.integers(-50, 51, size=(R, 7)).astype(np.int64)
+    norm, strat, rank = oracle.calc_lnorm64(A, n, marg, 3)
+    assert (norm, strat, rank) == oracle.calc_lnorm(A.astype(np.int32), n, marg, 3)
+    for c in (2 ** 24 + 3, 2 ** 33 - 1):
+        n2, s2, r2 = oracle.calc_lnorm64(A * c, n, marg, 3)
+        assert n2 == c * norm and n2 > 2 ** 31
+        assert (s2, r2) == (strat, rank)
+    if not marg:
+        # a negative factor: |c| scaling, same maximising rank
+        n3, s3, r3 = oracle.calc_lnorm64(A * -(2 ** 30 + 7), n, marg, 3)
+        assert (n3, s3, r3) == ((2 ** 30 + 7) * norm, strat, rank)
+
+
+def test_int64_rank_one_closed_form():
+    """L1 of u v^T is ||u||_1 ||v||_1, here about 2^44."""
+    rng = np.random.default_rng(7)
+    u = rng.integers(-(2 ** 20), 2 ** 20, size=8).astype(np.int64)
+    v = rng.integers(-(2 ** 20), 2 ** 20, size=5).astype(np.int64)
+    M = np.outer(u, v)
+    want = int(np.abs(u).sum()) * int(np.abs(v).sum())
+    assert want > 2 ** 40
+    assert oracle.calc_lnorm64(M, 1, 0, 4)[0] == want
+    assert oracle.brute64(M, 1, 0)[0] == want
+
+
+@pytest.mark.parametrize("n,marg", [(1, 0), (1, 1), (2, 0), (3, 0), (5, 0)])
+def test_int64_walk_matches_bruteforce(n, marg):
+    """The incremental 64-bit walks against direct decoding and full
+    recompute, entries up to 2^40."""
+    rng = np.random.default_rng(100 + n + marg)
+    R = {1: 10, 2: 9, 3: 6, 5: 4}[n]
+    for _ in range(3):
+        A = rng.integers(-(2 ** 40), 2 ** 40, size=(R, int(rng.integers(1, 6)))).astype(np.int64)
+        norm, strat, rank = oracle.calc_lnorm64(A, n, marg, 4)
+        assert (norm, rank) == oracle.brute64(A, n, marg)
+        assert oracle.objective64(A, n, marg, strat) == norm
