@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define LNORM_HIP_ABI_VERSION 2
+#define LNORM_HIP_ABI_VERSION 3
 
 /* error codes */
 #define LNORM_OK 0
@@ -95,6 +95,22 @@ int lnorm_hip_calc_Lnorm(const int32_t *mtx_as_vec, int iRows, int iCols, int n,
  * strategy may be NULL when opts->want_strategy == 0; res may be NULL). */
 int lnorm_hip_calc_ex(const int32_t *mtx_as_vec, int iRows, int iCols, int n, int marginal,
                       const lnorm_hip_opts *opts, int32_t *strategy, lnorm_hip_result *res);
+
+/*
+ * The int_type = long long variant of calc_Lnorm (README.md:100-105:
+ * functions.h:5 typedef long long int_type, abs -> llabs): entries and the
+ * norm in int64, every sum exact while sum |A| <= INT64_MAX (else
+ * LNORM_E_BOUND).  Problems inside the int32 contract run on the int32
+ * kernels (the same result); the rest on 64-bit walks (no blocked or packed
+ * 16-bit counterpart: opts->kernel may be AUTO, PLAIN or GENERAL).
+ * res->Lnorm holds the norm only when it fits int32 (else 0); *Lnorm always.
+ */
+int lnorm_hip_calc_Lnorm64(const int64_t *mtx_as_vec, int iRows, int iCols, int n, int marginal,
+                           int num_of_blocks, int num_of_threads_per_block,
+                           int64_t *Lnorm, int32_t *strategy);
+int lnorm_hip_calc_ex64(const int64_t *mtx_as_vec, int iRows, int iCols, int n, int marginal,
+                        const lnorm_hip_opts *opts, int32_t *strategy, int64_t *Lnorm,
+                        lnorm_hip_result *res);
 
 /* One problem of a batch (cfg: many matrices of uneven shape and d). */
 typedef struct lnorm_hip_item {

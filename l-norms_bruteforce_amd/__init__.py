@@ -82,7 +82,7 @@ HIP_SYMBOLS = ["lnorm_hip_calc_Lnorm", "lnorm_hip_calc_ex", "lnorm_hip_calc_batc
                "lnorm_hip_default_opts", "lnorm_hip_last_error", "lnorm_hip_abi_version",
                "lnorm_hip_device_count", "lnorm_hip_release", "lnorm_hip_selftest",
                "lnorm_hip_debug_table", "lnorm_hip_calc_batch_ex", "lnorm_hip_autotune",
-               "lnorm_hip_autotune_batch"]
+               "lnorm_hip_autotune_batch", "lnorm_hip_calc_Lnorm64", "lnorm_hip_calc_ex64"]
 
 _hip = None
 
@@ -119,6 +119,13 @@ def hip_lib():
         L.lnorm_hip_autotune_batch.restype = ctypes.c_int
         L.lnorm_hip_autotune_batch.argtypes = [ctypes.POINTER(Item), ctypes.c_int, ctypes.POINTER(Opts),
                                                ctypes.POINTER(Opts), ctypes.POINTER(TuneEntry), ctypes.c_int]
+        i64p = ctypes.POINTER(ctypes.c_int64)
+        L.lnorm_hip_calc_ex64.restype = ctypes.c_int
+        L.lnorm_hip_calc_ex64.argtypes = [i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.POINTER(Opts), i32p, i64p, ctypes.POINTER(Result)]
+        L.lnorm_hip_calc_Lnorm64.restype = ctypes.c_int
+        L.lnorm_hip_calc_Lnorm64.argtypes = [i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_int, i64p, i32p]
         L.lnorm_hip_default_opts.restype = None
         L.lnorm_hip_default_opts.argtypes = [ctypes.POINTER(Opts)]
         L.lnorm_hip_last_error.restype = ctypes.c_char_p
@@ -173,6 +180,23 @@ def calc_lnorm(A, n: int, marginal: int = 0, want_strategy: bool = True, **opts)
     _check(hip_lib().lnorm_hip_calc_ex(a.ctypes.data_as(i32p), R, C, n, marginal, ctypes.byref(o),
                                        strat.ctypes.data_as(i32p), ctypes.byref(res)))
     return int(res.Lnorm), [int(v) for v in strat[: R - 1]] if want_strategy else None, res.as_dict()
+
+
+def calc_lnorm64(A, n: int, marginal: int = 0, want_strategy: bool = True, **opts):
+    """The int_type = long long variant (lnorm_hip_calc_ex64): int64 entries
+    and norm.  Returns (Lnorm, strategy list, result dict)."""
+    a = np.ascontiguousarray(np.asarray(A, dtype=np.int64))
+    if a.ndim != 2:
+        raise ValueError("matrix must be 2-D")
+    R, C = a.shape
+    o = default_opts(want_strategy=int(want_strategy), **opts)
+    strat = np.zeros(max(R - 1, 1), dtype=np.int32)
+    res = Result()
+    norm = ctypes.c_int64()
+    _check(hip_lib().lnorm_hip_calc_ex64(a.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)), R, C, n, marginal,
+                                         ctypes.byref(o), strat.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)),
+                                         ctypes.byref(norm), ctypes.byref(res)))
+    return int(norm.value), [int(v) for v in strat[: R - 1]] if want_strategy else None, res.as_dict()
 
 
 def calc_lnorm_simple(A, n: int, marginal: int = 0, blocks: int = 512, threads: int = 32):

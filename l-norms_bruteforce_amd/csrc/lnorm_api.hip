@@ -75,6 +75,8 @@ struct DevCtx {
   // batch buffers, the side streams the groups of a batch spread over, and
   // the captured launch sequence
   Buf bA, bD, bUnits, bScal, bTasks, bCnt;
+  // the int_type = long long variant
+  Buf A64, D64, units64, state64;
   hipStream_t side[kSideStreams] = {};
   hipEvent_t bFork[2] = {}, bJoin[2][kSideStreams] = {};
   hipGraphExec_t bGraph = nullptr;
@@ -1245,6 +1247,193 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
   return first_err;
 }
 
+
+// ---- the int_type = long long variant (README.md:100-105) -------------------
+// Problems inside the int32 contract run on the int32 kernels (identical
+// results); the rest on the 64-bit walks of lnorm_int64.hip.
+int run64_phase(DevCtx &cx, Kind kind, int d, int ny, bool general, Mode mode, const Geometry &g, uint64_t seg0,
+                uint64_t nseg, int R, int64_t target, hipStream_t st, double &ms_kernel, int64_t &best,
+                uint64_t &first) {
+  long long *gbest = (long long *)cx.scal.p;
+  unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
+  HIPCHK(launch_fill_i64(gbest, INT64_MIN, 1, st));
+  HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
+  Plain64Args a{};
+  a.A = (const int64_t *)cx.A64.p;
+  a.D = (const int64_t *)cx.D64.p;
+  a.R = R;
+  a.ny_pad = ny;
+  a.d = d;
+  a.q = g.q;
+  a.seg0 = seg0;
+  a.nseg = nseg;
+  a.target = target;
+  a.unit_best = (int64_t *)cx.units64.p;
+  a.gbest = gbest;
+  a.gfirst = gfirst;
+  a.state = (uint64_t *)cx.state64.p;
+  HIPCHK(hipEventRecord(cx.ev[4], st));
+  hipError_t e = general ? launch_general64(kind, mode, a, ny, st) : launch_plain64(kind, mode, a, st);
+  HIPCHK(hipEventRecord(cx.ev[5], st));
+  if (e != hipSuccess) return fail(LNORM_E_HIP, "64-bit launch failed: %s", hipGetErrorString(e));
+  if (mode == MODE_MAX) HIPCHK(launch_first_unit64(a.unit_best, nseg, gbest, gfirst, st));
+  long long hb = 0;
+  unsigned long long hf = 0;
+  HIPCHK(hipMemcpyAsync(&hb, gbest, sizeof hb, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipMemcpyAsync(&hf, gfirst, sizeof hf, hipMemcpyDeviceToHost, st));
+  HIPCHK(hipStreamSynchronize(st));
+  float kms = 0;
+  HIPCHK(hipEventElapsedTime(&kms, cx.ev[4], cx.ev[5]));
+  ms_kernel += kms;
+  best = hb;
+  first = hf;
+  return 0;
+}
+
+int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *optsp,
+                int32_t *strategy, int64_t *Lnorm, lnorm_hip_result *res) {
+  const auto t_start = std::chrono::steady_clock::now();
+  g_err.clear();
+  lnorm_hip_opts o;
+  lnorm_hip_default_opts(&o);
+  if (optsp) o = *optsp;
+  if (!A || !Lnorm) return fail(LNORM_E_ARG, "mtx_as_vec or Lnorm is NULL");
+  if (R < 1 || C < 1) return fail(LNORM_E_ARG, "matrix must be at least 1 x 1 (got %d x %d)", R, C);
+  // sum |A| <= INT64_MAX keeps every partial sum and objective of the
+  // long long variant in range (the int32 contract, one level up)
+  unsigned __int128 s = 0;
+  bool fits32 = true;
+  for (size_t i = 0; i < (size_t)R * C; ++i) {
+    const int64_t v = A[i];
+    const unsigned __int128 av = v < 0 ? (unsigned __int128)(-(v + 1)) + 1 : (unsigned __int128)v;
+    s += av;
+    if (v < INT32_MIN || v > INT32_MAX) fits32 = false;
+  }
+  if (s > (unsigned __int128)INT64_MAX)
+    return fail(LNORM_E_BOUND, "sum |A| exceeds INT64_MAX; the long long variant would overflow");
+  if (fits32 && s <= (unsigned __int128)INT32_MAX) {
+    std::vector<int32_t> a32((size_t)R * C);
+    for (size_t i = 0; i < a32.size(); ++i) a32[i] = (int32_t)A[i];
+    lnorm_hip_result r;
+    if (int rc = calc_impl(a32.data(), R, C, n, marginal, &o, strategy, &r)) return rc;
+    *Lnorm = r.Lnorm;
+    if (res) *res = r;
+    return LNORM_OK;
+  }
+  // the checks of validate() on the shape
+  std::vector<int32_t> dummy(1, 0);
+  Problem p{dummy.data(), R, C, n, marginal, KIND_L1, 2, 0};
+  if (p.n < 1 || p.n > kMaxLabels) return fail(LNORM_E_SIZE, "order %d outside 1..%d", p.n, kMaxLabels);
+  if (p.marginal != 0 && p.marginal != 1) return fail(LNORM_E_ARG, "marginal must be 0 or 1");
+  if (p.marginal && p.n != 1) return fail(LNORM_E_ARG, "marginals are defined for n = 1 only");
+  if (p.R > max_rows(p.n)) return fail(LNORM_E_SIZE, "%d rows exceed maxRows %d for n = %d", p.R, max_rows(p.n), p.n);
+  if (p.C > kMaxCols) return fail(LNORM_E_SIZE, "%d columns exceed length %d", p.C, kMaxCols);
+  p.kind = p.n == 1 ? (p.marginal ? KIND_LM : KIND_L1) : (p.n == 2 ? KIND_L2 : KIND_LD);
+  p.base = p.n <= 2 ? 2 : p.n;
+  if (p.n <= 2) {
+    p.total = 1ull << (p.R - 1);
+  } else {
+    const double dt = std::pow((double)p.n, (double)(p.R - 1));
+    if (dt >= 9007199254740992.0)
+      return fail(LNORM_E_SIZE, "%d^%d labelings exceed 2^53, where the reference's pow() count is inexact", p.n,
+                  p.R - 1);
+    p.total = ipow((uint64_t)p.n, p.R - 1);
+  }
+  if (o.want_strategy && R > 1 && !strategy) return fail(LNORM_E_ARG, "strategy output is NULL");
+  const int ny_plain = plain64_ny_pad(p.kind, p.n, C);
+  int kernel = o.kernel;
+  if (kernel == LNORM_KERNEL_BLOCKED || kernel == LNORM_KERNEL_PLAIN16)
+    return fail(LNORM_E_UNSUPPORTED, "the blocked and packed 16-bit kernels have no 64-bit counterpart");
+  if (kernel == LNORM_KERNEL_PLAIN && !ny_plain)
+    return fail(LNORM_E_UNSUPPORTED, "no 64-bit plain instantiation for %d columns at n = %d", C, n);
+  if (kernel == LNORM_KERNEL_AUTO) kernel = ny_plain ? LNORM_KERNEL_PLAIN : LNORM_KERNEL_GENERAL;
+  if (kernel != LNORM_KERNEL_PLAIN && kernel != LNORM_KERNEL_GENERAL)
+    return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
+  const bool general = kernel == LNORM_KERNEL_GENERAL;
+  const int ny = general ? C : ny_plain;
+
+  // matrix padded to the kernel width and the binary delta table
+  std::vector<int64_t> hA((size_t)R * ny, 0), hD;
+  for (int i = 0; i < R; ++i)
+    for (int l = 0; l < C; ++l) hA[(size_t)i * ny + l] = A[(size_t)i * C + l];
+  if (!general && p.n <= 2) {
+    hD.assign((size_t)2 * std::max(R - 1, 1) * ny, 0);
+    const int64_t f = p.n == 1 ? 2 : 1;
+    for (int j = 0; j < R - 1; ++j)
+      for (int l = 0; l < ny; ++l) {
+        const int64_t v = (int64_t)((uint64_t)f * (uint64_t)hA[(size_t)j * ny + l]);
+        hD[((size_t)(R - 1) + j) * ny + l] = v;                     // up: +f A
+        hD[(size_t)j * ny + l] = (int64_t)(0ull - (uint64_t)v);       // down: -f A
+      }
+  }
+
+  int dev = 0;
+  if (int r = select_device(o, dev)) return r;
+  DevCtx &cx = g_ctx[dev];
+  std::lock_guard<std::mutex> lock(cx.mu);
+  if (int r = ctx_init(cx)) return r;
+  hipStream_t st = o.stream ? (hipStream_t)o.stream : cx.stream;
+  const int nvec = (p.kind == KIND_L1 || p.kind == KIND_LM) ? 1 : p.n;
+  if (int r = ensure(cx.A64, hA.size() * 8)) return r;
+  if (int r = ensure(cx.D64, std::max<size_t>(hD.size(), 1) * 8)) return r;
+  if (int r = ensure(cx.units64, kUnitsPerChunk * 8)) return r;
+  if (int r = ensure(cx.scal, 256)) return r;
+  if (general)
+    if (int r = ensure(cx.state64, (size_t)kGeneralSlices * nvec * C * 8)) return r;
+  HIPCHK(hipMemcpyAsync(cx.A64.p, hA.data(), hA.size() * 8, hipMemcpyHostToDevice, st));
+  if (!hD.empty()) HIPCHK(hipMemcpyAsync(cx.D64.p, hD.data(), hD.size() * 8, hipMemcpyHostToDevice, st));
+
+  lnorm_hip_result rr;
+  std::memset(&rr, 0, sizeof rr);
+  rr.total = p.total;
+  const Walk walk = general ? WALK_GENERAL : WALK_PLAIN;
+  const Geometry g = geometry_for(p.base, R - 1, 20, o.seg_log, walk);
+  double ms_kernel = 0;
+  int64_t best = INT64_MIN;
+  uint64_t first_unit = ~0ull;
+  HIPCHK(hipEventRecord(cx.ev[0], st));
+  for (uint64_t c0 = 0; c0 < g.nseg; c0 += kUnitsPerChunk) {
+    const uint64_t nch = std::min(kUnitsPerChunk, g.nseg - c0);
+    int64_t b;
+    uint64_t f;
+    if (int r = run64_phase(cx, p.kind, p.n, ny, general, MODE_MAX, g, c0, nch, R, 0, st, ms_kernel, b, f)) return r;
+    if (first_unit == ~0ull || b > best) {
+      best = b;
+      first_unit = c0 + f;
+    }
+  }
+  HIPCHK(hipEventRecord(cx.ev[1], st));
+  rr.units = g.nseg;
+  rr.unit_log = g.q;
+  rr.kernel = kernel;
+  if (o.want_strategy && R > 1) {
+    const Geometry g2 = geometry_for(p.base, g.q, 16, 0, walk);
+    double ms2 = 0;
+    int64_t b;
+    uint64_t rank;
+    if (int r = run64_phase(cx, p.kind, p.n, ny, general, MODE_FIRST, g2, first_unit * g2.nseg, g2.nseg, R, best, st,
+                            ms2, b, rank))
+      return r;
+    if (rank == ~0ull) return fail(LNORM_E_HIP, "no rank attains the maximum %lld", (long long)best);
+    rr.ms_strategy = ms2;
+    rr.best_rank = rank;
+    labels_of_rank(rank, R, n, strategy);
+  } else {
+    rr.best_rank = R == 1 ? 0 : ~0ull;
+  }
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, cx.ev[0], cx.ev[1]));
+  rr.ms_enumerate = ms;
+  rr.ms_kernel = ms_kernel;
+  rr.Lnorm = (best >= INT32_MIN && best <= INT32_MAX) ? (int32_t)best : 0;
+  snprintf(rr.kernel_name, sizeof rr.kernel_name, "%s64<kind=%d,d=%d,ny=%d,q=%d>", general ? "general" : "plain",
+           (int)p.kind, p.n, ny, g.q);
+  rr.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
+  *Lnorm = best;
+  if (res) *res = rr;
+  return LNORM_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -1260,6 +1449,21 @@ void lnorm_hip_default_opts(lnorm_hip_opts *o) {
 int lnorm_hip_calc_ex(const int32_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *opts,
                       int32_t *strategy, lnorm_hip_result *res) {
   return calc_impl(A, R, C, n, marginal, opts, strategy, res);
+}
+
+int lnorm_hip_calc_ex64(const int64_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *opts,
+                        int32_t *strategy, int64_t *Lnorm, lnorm_hip_result *res) {
+  return calc_impl64(A, R, C, n, marginal, opts, strategy, Lnorm, res);
+}
+
+int lnorm_hip_calc_Lnorm64(const int64_t *A, int R, int C, int n, int marginal, int num_of_blocks,
+                           int num_of_threads_per_block, int64_t *Lnorm, int32_t *strategy) {
+  (void)num_of_blocks;
+  (void)num_of_threads_per_block;
+  lnorm_hip_opts o;
+  lnorm_hip_default_opts(&o);
+  o.want_strategy = strategy != nullptr;
+  return calc_impl64(A, R, C, n, marginal, &o, strategy, Lnorm, nullptr);
 }
 
 int lnorm_hip_calc_Lnorm(const int32_t *A, int R, int C, int n, int marginal, int num_of_blocks,
@@ -1362,7 +1566,7 @@ void lnorm_hip_release(void) {
   c.bGraph = nullptr;
   c.bGraphSig.clear();
   for (Buf *b : {&c.A, &c.D, &c.units, &c.scal, &c.table, &c.dpk, &c.seed, &c.dfull, &c.lo, &c.state, &c.bA, &c.bD,
-                 &c.bUnits, &c.bScal, &c.bTasks, &c.bCnt}) {
+                 &c.bUnits, &c.bScal, &c.bTasks, &c.bCnt, &c.A64, &c.D64, &c.units64, &c.state64}) {
     if (b->p) (void)hipFree(b->p);
     b->p = nullptr;
     b->cap = 0;

@@ -38,6 +38,19 @@ struct PlainArgs {
   uint32_t *state;         // general kernel: per-wave scratch, kGeneralSlices * nvec * ny words
 };
 
+// The int_type = long long variant: the same geometry with 64-bit values.
+struct Plain64Args {
+  const int64_t *A;        // R x ny_pad (general kernel: R x ny)
+  const int64_t *D;        // binary: [2][R-1][ny_pad] signed row deltas
+  int R, ny_pad, d, q;
+  uint64_t seg0, nseg;
+  int64_t target;          // FIRST mode: value to find
+  int64_t *unit_best;      // MAX mode: best per segment [nseg]
+  long long *gbest;        // MAX mode: atomic max
+  unsigned long long *gfirst;
+  uint64_t *state;         // general kernel: per-wave scratch, kGeneralSlices * nvec * ny words
+};
+
 // The general kernel runs at most this many waves at once (its scratch size).
 constexpr uint64_t kGeneralSlices = 2048;
 
@@ -131,6 +144,12 @@ hipError_t launch_table(const TableArgs &a, hipStream_t st);
 hipError_t launch_general(Kind kind, Mode mode, const PlainArgs &a, int ny, hipStream_t st);
 
 hipError_t launch_fill_i32(int32_t *p, int32_t v, uint64_t n, hipStream_t st);
+hipError_t launch_fill_i64(long long *p, long long v, uint64_t n, hipStream_t st);
+hipError_t launch_first_unit64(const int64_t *unit_best, uint64_t n, const long long *gbest,
+                               unsigned long long *first, hipStream_t st);
+int plain64_ny_pad(Kind kind, int d, int ny);   // 0: no register-resident 64-bit width
+hipError_t launch_plain64(Kind kind, Mode mode, const Plain64Args &a, hipStream_t st);
+hipError_t launch_general64(Kind kind, Mode mode, const Plain64Args &a, int ny, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t n, hipStream_t st);
 hipError_t launch_first_unit(const int32_t *unit_best, uint64_t n, const int32_t *gbest,
                              unsigned long long *first, hipStream_t st);

@@ -1,0 +1,84 @@
+"""The int_type = long long variant (lnorm_hip_calc_ex64): 64-bit matrices
+whose sums exceed int32, against the oracle's 64-bit walks (norm, strategy
+and maximising rank), the scaling identity on the GPU, delegation to the
+int32 kernels inside the int32 bound, and error codes.  GPU only."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def _rand64(rng, R, C, bits):
+    return rng.integers(-(2 ** bits), 2 ** bits, size=(R, C)).astype(np.int64)
+
+
+# (n, marginal, rows, column counts): the column counts cover the
+# register-resident 64-bit widths and, beyond them, the general kernel
+CASES = [
+    (1, 0, 14, [1, 7, 33, 64, 70]),
+    (1, 1, 13, [2, 9, 48, 65]),
+    (2, 0, 12, [3, 17, 32, 40]),
+    (3, 0, 8, [5, 24, 30]),
+    (4, 0, 7, [4, 16, 20]),
+    (5, 0, 6, [3, 8, 12]),
+    (7, 0, 5, [6, 9]),
+    (10, 0, 4, [8, 10]),
+]
+
+
+@pytest.mark.parametrize("n,marg,R,cols", CASES)
+def test_int64_vs_oracle(pkg, n, marg, R, cols):
+    rng = np.random.default_rng(500 + 10 * n + marg)
+    for C in cols:
+        for bits in (33, 40):
+            A = _rand64(rng, R, C, bits)
+            print("int64", n, marg, A.shape, bits, flush=True)
+            want = oracle.calc_lnorm64(A, n, marg, 8)
+            norm, strat, res = pkg.calc_lnorm64(A, n, marg)
+            assert (norm, strat, res["best_rank"]) == want, (n, marg, A.shape, bits, res["kernel_name"])
+            assert "64<" in res["kernel_name"]
+
+
+def test_int64_general_equals_plain(pkg):
+    rng = np.random.default_rng(9)
+    for n, R, C in ((1, 16, 20), (2, 12, 10), (3, 9, 8)):
+        A = _rand64(rng, R, C, 36)
+        p = pkg.calc_lnorm64(A, n, 0, kernel="plain")
+        g = pkg.calc_lnorm64(A, n, 0, kernel="general")
+        assert p[:2] == g[:2] and p[2]["best_rank"] == g[2]["best_rank"]
+        assert p[2]["kernel_name"].startswith("plain64") and g[2]["kernel_name"].startswith("general64")
+
+
+def test_int64_scaling_on_gpu(pkg):
+    """L(c*A) = c * L(A) with the same maximising rank: a 2^29-vector L1
+    and a 3^13-labeling L3, scaled far beyond int32."""
+    from lnorm_amd import gen
+    for (R, C, n, c) in ((30, 20, 1, 2 ** 26 + 1), (14, 16, 3, 2 ** 31 + 5)):
+        A = gen.gen_matrix(R, C, 100, 77 + n)
+        base = pkg.calc_lnorm(A, n, 0)
+        big = pkg.calc_lnorm64(A.astype(np.int64) * c, n, 0)
+        assert big[0] == c * base[0] and big[0] > 2 ** 39
+        assert big[1] == base[1] and big[2]["best_rank"] == base[2]["best_rank"]
+
+
+def test_int64_inside_int32_bound_uses_int32_kernels(pkg):
+    from lnorm_amd import gen
+    A = gen.gen_matrix(24, 40, 100, 5)
+    a = pkg.calc_lnorm(A, 1, 0)
+    b = pkg.calc_lnorm64(A.astype(np.int64), 1, 0)
+    assert a[:2] == b[:2] and b[2]["kernel_name"] == a[2]["kernel_name"]
+    assert b[2]["kernel_name"].startswith("blocked")
+
+
+def test_int64_errors(pkg):
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm64(np.full((3, 2), 2 ** 62, np.int64), 1, 0)
+    assert e.value.code == -2
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm64(np.full((12, 3), 2 ** 40, np.int64), 1, 0, kernel="blocked")
+    assert e.value.code == -5
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm64(np.full((12, 3), 2 ** 40, np.int64), 2, 1)
+    assert e.value.code == -1
