@@ -47,3 +47,32 @@ def test_cli_cuda_argv_form(pkg, golden, tmp_path):
         assert r.stdout.splitlines() == want, case["name"]
         files = {f: (d / f).read_text() for f in os.listdir(d) if f.startswith("strategy_")}
         assert files == case["strategy"], case["name"]
+
+
+def test_cli_marginal_workload_preprocessing_invariance(pkg, tmp_path):
+    """The 36 x 36 marginal workload (entries in [-100, 100], seed 102) with 4
+    duplicated rows, 3 duplicated columns and 2 further rows zeroed (40 x 39
+    file): L_HIP prints the same LM with preprocessing ('y m', about 2^33
+    sign vectors after the merges) and without it ('n m', 2^36).  Merging
+    duplicates is exact for this objective (convex in each row's sign), so
+    the two runs must agree; they share no enumeration."""
+    import numpy as np
+    from lnorm_amd import gen
+    M = gen.gen_matrix(36, 36, 100, 102)
+    rng = np.random.default_rng(102)
+    dup_r = rng.choice(np.arange(1, 36), 4, replace=False)
+    dup_c = rng.choice(np.arange(1, 36), 3, replace=False)
+    V = np.vstack([M, M[dup_r]])
+    V = np.hstack([V, V[:, dup_c]])
+    zero = [r for r in range(1, 36) if r not in set(dup_r.tolist())][:2]
+    V[zero, :] = 0
+    assert V.shape == (40, 39)
+    gen.write_matrix(tmp_path / "v.txt", V)
+    norms = {}
+    for flag in ("y", "n"):
+        print("cli marginal workload", flag, flush=True)
+        r = _run(pkg, ["--omp", "16", "v.txt", "1", flag, "m"], tmp_path)
+        assert r.returncode == 0, r.stderr
+        norms[flag] = int(r.stdout.strip().splitlines()[-1].split()[-1])
+        assert r.stdout.strip().splitlines()[-1].startswith("LM is: ")
+    assert norms["y"] == norms["n"], norms
