@@ -134,40 +134,105 @@ def mfma_share(res, k: int, ny: int) -> dict:
             "note": "table build only; the walk over the %d high rows is VALU (v_sad_u16) + SALU" % (k - 1 - kl)}
 
 
-def cpu_baseline(gen, rows: int = 64, r: int = 127, seed: int = 104, budget_cols: int = 35):
+def cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_reference(gen, M, d: int, units: int, extra=(), runs: int = 3) -> dict:
     """The reference L_OpenMP (built from its sources into oracle/_ref) on the
-    host cores over a bounded sample: the workload's own entry distribution
-    with `budget_cols` columns, i.e. 2^(budget_cols-1) sign vectors of `rows`
-    partial sums each (the reference transposes it to budget_cols x rows).
-    Falls back to the oracle's restatement when the reference binary is
-    absent."""
+    host cores: `runs` runs of the file M with argv tail `d [extra]`, the
+    median wall time (the time_order_1.sh method: whole process).  Falls back
+    to the oracle's restatement when the reference binary is absent."""
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     ref = os.path.join(ROOT, "oracle", "_ref", "L_OpenMP")
-    M = gen.gen_matrix(rows, budget_cols, r, seed)
-    units = 2 ** (budget_cols - 1)
-    sample = "%d x %d file, entries in [-%d, %d] (k = %d after the reference's transpose, %d partial sums): " \
-             "2^%d sign vectors" % (rows, budget_cols, r, r, budget_cols, rows, budget_cols - 1)
+    times, norm = [], None
     if os.path.exists(ref):
         with tempfile.TemporaryDirectory() as td:
-            path = os.path.join(td, "m.txt")
-            gen.write_matrix(path, M)
-            t = time.perf_counter()
-            p = subprocess.run([ref, str(threads), "m.txt", "1"], cwd=td, capture_output=True, text=True,
-                               timeout=600)
-            dt = time.perf_counter() - t
-        if p.returncode != 0:
-            raise RuntimeError("reference L_OpenMP failed: %s" % p.stderr)
-        norm = int(p.stdout.strip().splitlines()[-1].split()[-1])
+            gen.write_matrix(os.path.join(td, "m.txt"), M)
+            for _ in range(runs):
+                t = time.perf_counter()
+                p = subprocess.run([ref, str(threads), "m.txt", str(d), *extra], cwd=td, capture_output=True,
+                                   text=True, timeout=600)
+                times.append(time.perf_counter() - t)
+                if p.returncode != 0:
+                    raise RuntimeError("reference L_OpenMP failed: %s" % p.stderr)
+                norm = int(p.stdout.strip().splitlines()[-1].split()[-1])
         kind = "reference"
     else:
         sys.path.insert(0, ROOT)
         import oracle
-        t = time.perf_counter()
-        norm = oracle.calc_lnorm(M.T.copy(), 1, 0, threads)[0]
-        dt = time.perf_counter() - t
+        from lnorm_amd import host
+        pr, _, _ = host.load(gen.matrix_text(M).encode(), d, "n", 0)
+        A = pr.matrix_laid_out()
+        host.free(pr)
+        for _ in range(runs):
+            t = time.perf_counter()
+            norm = oracle.calc_lnorm(A, d, 0, threads)[0]
+            times.append(time.perf_counter() - t)
         kind = "port"
-    return {"value": units / dt, "unit": "sign-vectors/s", "cores": threads, "kind": kind,
-            "sample": sample + "; wall %.2f s; L1 = %d" % (dt, norm), "seconds": dt}
+    dt = sorted(times)[len(times) // 2]
+    return {"value": units / dt, "cores": threads, "kind": kind, "seconds": dt, "runs": len(times),
+            "cpu": cpu_model(), "norm": norm}
+
+
+def cpu_baseline(gen, rows: int = 64, r: int = 127, seed: int = 104, budget_cols: int = 34):
+    """The headline workload's CPU baseline: the same entry distribution with
+    `budget_cols` columns, 2^(budget_cols-1) sign vectors of `rows` partial
+    sums (the reference transposes the file to budget_cols x rows)."""
+    M = gen.gen_matrix(rows, budget_cols, r, seed)
+    c = cpu_reference(gen, M, 1, 2 ** (budget_cols - 1))
+    sample = ("%d x %d file, entries in [-%d, %d] (k = %d after the reference's transpose, %d partial sums): "
+              "2^%d sign vectors; median of %d runs %.2f s; L1 = %d; %s"
+              % (rows, budget_cols, r, r, budget_cols, rows, budget_cols - 1, c["runs"], c["seconds"], c["norm"],
+                 c["cpu"]))
+    return {"value": c["value"], "unit": "sign-vectors/s", "cores": c["cores"], "kind": c["kind"],
+            "sample": sample, "seconds": c["seconds"]}
+
+
+def l3_line(pkg, gen, steps: int, warmup: int, cpu: bool) -> dict:
+    """L_3 of a 24 x 32 file (entries in [-50, 50], seed 103): 3^23 = 9.4e10
+    labelings on the packed 16-bit d-ary walk, beside the 32-bit d-ary walk and
+    the reference L_OpenMP on a 20-row sample of the same file (3^19)."""
+    B = gen.gen_matrix(24, 32, 50, 103)
+    total = 3 ** 23
+    kw = {"autotune": 1 if AUTOTUNE else 0}
+    if AUTOTUNE:
+        pkg.autotune(B, 3, 0)
+    for _ in range(warmup):
+        pkg.calc_lnorm(B, 3, 0, **kw)
+    t0 = time.perf_counter()
+    kms, out = [], None
+    for _ in range(max(1, steps)):
+        norm, strat, res = pkg.calc_lnorm(B, 3, 0, **kw)
+        kms.append(res["ms_kernel"])
+        if out is None:
+            out = (norm, strat, res["best_rank"])
+        elif out != (norm, strat, res["best_rank"]):
+            raise RuntimeError("result changed between steps")
+    wall = (time.perf_counter() - t0) / max(1, steps)
+    km = sum(kms) / len(kms)
+    n2, s2, r2 = pkg.calc_lnorm(B, 3, 0, kernel="plain")
+    if (n2, s2, r2["best_rank"]) != out:
+        raise RuntimeError("the packed and 32-bit d-ary walks disagree")
+    ops = 4 * 16 + 3                      # 4 VALU per two columns (NW = 16 words) + the label sums and max
+    line = {"value": total / wall, "unit": "labelings/s", "ms_per_step": 1e3 * wall, "kernel_ms": km,
+            "kernel": res["kernel_name"], "valu_frac": total * ops / (km / 1e3) / VALU_PEAK,
+            "plain32": {"value": total / (r2["ms_total"] / 1e3), "kernel": r2["kernel_name"]},
+            "norm": out[0]}
+    if cpu:
+        c = cpu_reference(gen, B[:20], 3, 3 ** 19)
+        line["cpu_baseline"] = {"value": c["value"], "cores": c["cores"], "kind": c["kind"],
+                                "sample": "first 20 rows of the file: 3^19 labelings; median of %d runs %.2f s; "
+                                          "L3 = %d" % (c["runs"], c["seconds"], c["norm"])}
+        line["speedup_vs_cpu"] = line["value"] / c["value"]
+    return line
 
 
 def batch_valu_ops(d: int, cols: int) -> int:
@@ -386,13 +451,14 @@ def main():
         line["int16_128x40"] = {"blocked": s3["value"], "plain": s4["value"], "speedup": s3["value"] / s4["value"],
                                 "blocked_valu_frac": s3["valu_frac"], "plain_valu_frac": s4["valu_frac"],
                                 "kernel": s3["kernel_name"], "norm": s3["norm"]}
+        line["L3_24x32"] = l3_line(pkg, gen, a.steps, a.warmup, not a.no_cpu)
         line["batch_4096"] = batch_line(pkg, gen, max(1, a.steps // 2))
     if not a.no_cpu:
         cb = cpu_baseline(gen)
         line["cpu_baseline"] = {k2: cb[k2] for k2 in ("value", "unit", "cores", "kind", "sample")}
         line["speedup_vs_cpu"] = sb["value"] / cb["value"]
         if "n40_m40" in line:
-            c40 = cpu_baseline(gen, rows=40, r=100, seed=106, budget_cols=35)
+            c40 = cpu_baseline(gen, rows=40, r=100, seed=106, budget_cols=34)
             line["n40_m40"]["cpu_baseline"] = {k2: c40[k2] for k2 in ("value", "cores", "kind", "sample")}
             line["n40_m40"]["speedup_vs_cpu"] = line["n40_m40"]["blocked"] / c40["value"]
     print(json.dumps(line), flush=True)
