@@ -326,6 +326,23 @@ struct Mover16 {
   }
 };
 
+// The d-1 moves of row 0 inside one block of d steps: up 0->1->..->d-1, or
+// down d-1->..->0, each followed by an evaluation of step ++tt.
+template <int K, int D, int NW, bool UP>
+struct Digit0Run {
+  template <class Eval>
+  __device__ __forceinline__ static void run(uint32_t (&t)[D][NW], uint32_t (&S)[D], const cuint32_t *row0,
+                                             uint32_t B, uint32_t &tt, Eval &eval) {
+    if constexpr (K < D) {
+      constexpr int F = UP ? K - 1 : D - K;
+      constexpr int T = UP ? K : D - K - 1;
+      apply_move16<F, T, D, NW>(t, S, row0, B);
+      eval(++tt);
+      Digit0Run<K + 1, D, NW, UP>::run(t, S, row0, B, tt, eval);
+    }
+  }
+};
+
 // Ap: R x NW packed rows, word w = (A[i][2w+1] & 0xFFFF) << 16 | (A[i][2w] & 0xFFFF).
 template <int D, int NW, int MODE>
 __device__ __forceinline__ void walk_p16(
@@ -388,25 +405,40 @@ __device__ __forceinline__ void walk_p16(
     }
   };
 
+  // Steps come in blocks of d: step b*d changes a digit j >= 1 (the generic
+  // move, dispatched on its labels), and steps b*d+1 .. b*d+d-1 move row 0
+  // through all d labels in the direction of digit 0's parity, which is fixed
+  // inside the block.  Those d-1 moves are unrolled with compile-time labels
+  // (Digit0Run), so the digit counter and the dispatch run once per d steps.
+  // The counter below covers digits 1..q-1 (block index b).
   uint64_t biaspat = 0;
-  for (int i = 0; i < q; ++i) biaspat |= (uint64_t)(16 - D) << (4 * i);
+  for (int i = 1; i < q; ++i) biaspat |= (uint64_t)(16 - D) << (4 * (i - 1));
   uint64_t cnt = biaspat;
   uint32_t pm = par_mask;
-  uint32_t nsteps = 1;
-  for (int i = 0; i < q; ++i) nsteps *= (uint32_t)D;
+  uint32_t nblocks = 1;
+  for (int i = 1; i < q; ++i) nblocks *= (uint32_t)D;
 
   eval(0);
-  for (uint32_t tt = 1; tt < nsteps; ++tt) {
-    const int j = __builtin_ctzll(~cnt) >> 2;
-    const uint64_t low = (1ull << (4 * j)) - 1ull;
-    cnt = (cnt + 1ull) | (biaspat & low);
-    const int v = (int)((cnt >> (4 * j)) & 15ull) - (16 - D);
-    const int p = (pm >> j) & 1;
-    const int from = p ? D - v : v - 1;
-    pm ^= (1u << j) - 1u;
-    Mover16<0, D, NW>::run(t, S, Ap + (size_t)j * NW, from, p ? 0 : 1, B);
-    eval(tt);
+  uint32_t tt = 0;
+  if (q > 0) {
+    if (pm & 1u) Digit0Run<1, D, NW, false>::run(t, S, Ap, B, tt, eval);
+    else Digit0Run<1, D, NW, true>::run(t, S, Ap, B, tt, eval);
+    for (uint32_t b = 1; b < nblocks; ++b) {
+      const int jj = __builtin_ctzll(~cnt) >> 2;      // digit jj + 1 of the step counter changes
+      const uint64_t low = (1ull << (4 * jj)) - 1ull;
+      cnt = (cnt + 1ull) | (biaspat & low);
+      const int v = (int)((cnt >> (4 * jj)) & 15ull) - (16 - D);
+      const int j = jj + 1;
+      const int p = (pm >> j) & 1;
+      const int from = p ? D - v : v - 1;
+      pm ^= (1u << j) - 1u;
+      Mover16<0, D, NW>::run(t, S, Ap + (size_t)j * NW, from, p ? 0 : 1, B);
+      eval(++tt);
+      if (pm & 1u) Digit0Run<1, D, NW, false>::run(t, S, Ap, B, tt, eval);
+      else Digit0Run<1, D, NW, true>::run(t, S, Ap, B, tt, eval);
+    }
   }
+  const uint32_t nsteps = q > 0 ? nblocks * (uint32_t)D : 1u;
 
   if constexpr (MODE == MODE_MAX) {
     if (active) unit_best[local] = best;
