@@ -185,6 +185,21 @@ struct Mover64 {
   }
 };
 
+template <int K, int D, int NY, bool UP>
+struct Digit0Run64 {
+  template <class Eval>
+  __device__ __forceinline__ static void run(uint64_t (&t)[D][NY], int64_t (&S)[D], const cint64_t *row0,
+                                             uint32_t &tt, Eval &eval) {
+    if constexpr (K < D) {
+      constexpr int F = UP ? K - 1 : D - K;
+      constexpr int T = UP ? K : D - K - 1;
+      apply_move64<F, T, D, NY>(t, S, row0);
+      eval(++tt);
+      Digit0Run64<K + 1, D, NY, UP>::run(t, S, row0, tt, eval);
+    }
+  }
+};
+
 template <int D, int NY, int MODE>
 __global__ __launch_bounds__(256) void k_plain64_dary(const int64_t *__restrict__ A_, int R, int q, uint64_t seg0,
                                                       uint64_t nseg, int64_t target, int64_t *__restrict__ unit_best,
@@ -247,26 +262,36 @@ __global__ __launch_bounds__(256) void k_plain64_dary(const int64_t *__restrict_
     }
   };
 
-  // nibble counter: digit value v stored as v + (16 - d)
+  // blocks of d steps: a dispatched move of digit j >= 1, then the d-1
+  // moves of row 0 unrolled (the counter covers digits 1..q-1)
   uint64_t biaspat = 0;
-  for (int i = 0; i < q; ++i) biaspat |= (uint64_t)(16 - D) << (4 * i);
+  for (int i = 1; i < q; ++i) biaspat |= (uint64_t)(16 - D) << (4 * (i - 1));
   uint64_t cnt = biaspat;
   uint32_t pm = par_mask;
-  uint32_t nsteps = 1;
-  for (int i = 0; i < q; ++i) nsteps *= (uint32_t)D;
+  uint32_t nblocks = 1;
+  for (int i = 1; i < q; ++i) nblocks *= (uint32_t)D;
 
   eval(0);
-  for (uint32_t tt = 1; tt < nsteps; ++tt) {
-    const int j = __builtin_ctzll(~cnt) >> 2;
-    const uint64_t low = (1ull << (4 * j)) - 1ull;
-    cnt = (cnt + 1ull) | (biaspat & low);
-    const int v = (int)((cnt >> (4 * j)) & 15ull) - (16 - D);
-    const int p = (pm >> j) & 1;
-    const int from = p ? D - v : v - 1;
-    pm ^= (1u << j) - 1u;
-    Mover64<0, D, NY>::run(t, S, A + (size_t)j * NY, from, p ? 0 : 1);
-    eval(tt);
+  uint32_t tt = 0;
+  if (q > 0) {
+    if (pm & 1u) Digit0Run64<1, D, NY, false>::run(t, S, A, tt, eval);
+    else Digit0Run64<1, D, NY, true>::run(t, S, A, tt, eval);
+    for (uint32_t b = 1; b < nblocks; ++b) {
+      const int jj = __builtin_ctzll(~cnt) >> 2;
+      const uint64_t low = (1ull << (4 * jj)) - 1ull;
+      cnt = (cnt + 1ull) | (biaspat & low);
+      const int v = (int)((cnt >> (4 * jj)) & 15ull) - (16 - D);
+      const int j = jj + 1;
+      const int p = (pm >> j) & 1;
+      const int from = p ? D - v : v - 1;
+      pm ^= (1u << j) - 1u;
+      Mover64<0, D, NY>::run(t, S, A + (size_t)j * NY, from, p ? 0 : 1);
+      eval(++tt);
+      if (pm & 1u) Digit0Run64<1, D, NY, false>::run(t, S, A, tt, eval);
+      else Digit0Run64<1, D, NY, true>::run(t, S, A, tt, eval);
+    }
   }
+  const uint32_t nsteps = q > 0 ? nblocks * (uint32_t)D : 1u;
   finish64<MODE>(active, local, best, seg * (uint64_t)nsteps + first, first != 0xFFFFFFFFu, unit_best, gbest,
                  gfirst);
 }

@@ -174,6 +174,22 @@ struct Mover {
   }
 };
 
+// The d-1 moves of row 0 inside one block of d steps (32-bit label vectors).
+template <int K, int D, int NY, bool UP>
+struct Digit0Run32 {
+  template <class Eval>
+  __device__ __forceinline__ static void run(uint32_t (&t)[D][NY], uint32_t (&S)[D], const cint32_t *row0,
+                                             uint32_t B, uint32_t &tt, Eval &eval) {
+    if constexpr (K < D) {
+      constexpr int F = UP ? K - 1 : D - K;
+      constexpr int T = UP ? K : D - K - 1;
+      apply_move<F, T, D, NY>(t, S, row0, B);
+      eval(++tt);
+      Digit0Run32<K + 1, D, NY, UP>::run(t, S, row0, B, tt, eval);
+    }
+  }
+};
+
 template <int D, int NY, int MODE>
 __device__ __forceinline__ void walk_dary(
     const cint32_t *A, int R, int q, uint64_t seg0, uint64_t nseg, int32_t target,
@@ -239,26 +255,38 @@ __device__ __forceinline__ void walk_dary(
     }
   };
 
-  // nibble counter: digit value v stored as v + (16 - d)
+  // Blocks of d steps, as in walk_p16: step b*d changes a digit j >= 1 (the
+  // dispatched move); the d-1 moves of row 0 in between are unrolled.  The
+  // nibble counter (digit value v stored as v + (16 - d)) covers digits
+  // 1..q-1.
   uint64_t biaspat = 0;
-  for (int i = 0; i < q; ++i) biaspat |= (uint64_t)(16 - D) << (4 * i);
+  for (int i = 1; i < q; ++i) biaspat |= (uint64_t)(16 - D) << (4 * (i - 1));
   uint64_t cnt = biaspat;
   uint32_t pm = par_mask;
-  uint32_t nsteps = 1;
-  for (int i = 0; i < q; ++i) nsteps *= (uint32_t)D;
+  uint32_t nblocks = 1;
+  for (int i = 1; i < q; ++i) nblocks *= (uint32_t)D;
 
   eval(0);
-  for (uint32_t tt = 1; tt < nsteps; ++tt) {
-    const int j = __builtin_ctzll(~cnt) >> 2;
-    const uint64_t low = (1ull << (4 * j)) - 1ull;
-    cnt = (cnt + 1ull) | (biaspat & low);
-    const int v = (int)((cnt >> (4 * j)) & 15ull) - (16 - D);
-    const int p = (pm >> j) & 1;
-    const int from = p ? D - v : v - 1;
-    pm ^= (1u << j) - 1u;
-    Mover<0, D, NY>::run(t, S, A + (size_t)j * NY, from, p ? 0 : 1, B);
-    eval(tt);
+  uint32_t tt = 0;
+  if (q > 0) {
+    if (pm & 1u) Digit0Run32<1, D, NY, false>::run(t, S, A, B, tt, eval);
+    else Digit0Run32<1, D, NY, true>::run(t, S, A, B, tt, eval);
+    for (uint32_t b = 1; b < nblocks; ++b) {
+      const int jj = __builtin_ctzll(~cnt) >> 2;
+      const uint64_t low = (1ull << (4 * jj)) - 1ull;
+      cnt = (cnt + 1ull) | (biaspat & low);
+      const int v = (int)((cnt >> (4 * jj)) & 15ull) - (16 - D);
+      const int j = jj + 1;
+      const int p = (pm >> j) & 1;
+      const int from = p ? D - v : v - 1;
+      pm ^= (1u << j) - 1u;
+      Mover<0, D, NY>::run(t, S, A + (size_t)j * NY, from, p ? 0 : 1, B);
+      eval(++tt);
+      if (pm & 1u) Digit0Run32<1, D, NY, false>::run(t, S, A, B, tt, eval);
+      else Digit0Run32<1, D, NY, true>::run(t, S, A, B, tt, eval);
+    }
   }
+  const uint32_t nsteps = q > 0 ? nblocks * (uint32_t)D : 1u;
 
   if constexpr (MODE == MODE_MAX) {
     if (active) unit_best[local] = best;
