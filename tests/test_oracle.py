@@ -251,3 +251,63 @@ def test_int64_walk_matches_bruteforce(n, marg):
         norm, strat, rank = oracle.calc_lnorm64(A, n, marg, 4)
         assert (norm, rank) == oracle.brute64(A, n, marg)
         assert oracle.objective64(A, n, marg, strat) == norm
+
+
+def _block_moves(D, q, par_mask):
+    """Restatement of the GPU d-ary walks' step order (walk_p16 / walk_dary /
+    k_plain64_dary): blocks of d steps, a counted move of digit j >= 1 at
+    step b*d, then the d-1 moves of row 0 unrolled in the direction of digit
+    0's parity.  Yields (step, row, from, to)."""
+    def ctz_nib(cnt):
+        inv = ~cnt & ((1 << 64) - 1)
+        return ((inv & -inv).bit_length() - 1) >> 2
+    biaspat = 0
+    for i in range(1, q):
+        biaspat |= (16 - D) << (4 * (i - 1))
+    cnt, pm, out, tt = biaspat, par_mask, [], 0
+    nblocks = D ** (q - 1) if q > 0 else 1
+
+    def run0(up):
+        nonlocal tt
+        for k in range(1, D):
+            f, t = (k - 1, k) if up else (D - k, D - k - 1)
+            tt += 1
+            out.append((tt, 0, f, t))
+    if q > 0:
+        run0(not (pm & 1))
+        for _ in range(1, nblocks):
+            jj = ctz_nib(cnt)
+            cnt = ((cnt + 1) | (biaspat & ((1 << (4 * jj)) - 1))) & ((1 << 64) - 1)
+            v = ((cnt >> (4 * jj)) & 15) - (16 - D)
+            j = jj + 1
+            p = (pm >> j) & 1
+            f = D - v if p else v - 1
+            pm ^= (1 << j) - 1
+            tt += 1
+            out.append((tt, j, f, f - 1 if p else f + 1))
+            run0(not (pm & 1))
+    return out
+
+
+@pytest.mark.parametrize("D", range(2, 11))
+def test_dary_block_order(D):
+    """The kernels' block-unrolled step order equals the reflected d-ary Gray
+    code of the reference (lo_digit, L_OpenMP.c:140,204): for aligned
+    segments of d^q ranks starting at seg*d^q, step t moves exactly the row
+    whose label changes between ranks N0+t-1 and N0+t, between those labels.
+    The initial parity mask is the kernels' (odd d: every digit below q
+    follows the segment's parity; even d: only digit q-1)."""
+    for q in range(0, 5 if D <= 4 else 3):
+        for seg in (0, 1, 2, 3, 6):
+            ps = seg & 1
+            pmask = (((1 << q) - 1) if ps else 0) if D % 2 else ((1 << (q - 1)) if (q > 0 and ps) else 0)
+            moves = _block_moves(D, q, pmask)
+            N0 = seg * D ** q
+            assert len(moves) == D ** q - 1
+            rows = q + 3
+            prev = [oracle.digit(N0, i, D) for i in range(rows)]
+            for (tt, j, f, t) in moves:
+                cur = [oracle.digit(N0 + tt, i, D) for i in range(rows)]
+                diff = [i for i in range(rows) if cur[i] != prev[i]]
+                assert diff == [j] and (prev[j], cur[j]) == (f, t), (D, q, seg, tt)
+                prev = cur
