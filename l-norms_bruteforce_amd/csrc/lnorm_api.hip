@@ -783,12 +783,18 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
       std::lock_guard<std::mutex> tl(g_tune_mu);
       g_tune[key] = tv;
     }
+    const lnorm_hip_opts asked = o;
     if (!o.patterns_per_lane && tv.P) o.patterns_per_lane = tv.P;
     if (!o.wave_log2 && tv.s) o.wave_log2 = tv.s;
     if (!o.block && tv.block) o.block = tv.block;
     if (ch.kernel == LNORM_KERNEL_BLOCKED) {
       std::string why;
-      if (!blocked_plan(p, o, ch.bp, why)) return fail(LNORM_E_UNSUPPORTED, "tuned blocked shape: %s", why.c_str());
+      if (!blocked_plan(p, o, ch.bp, why)) {
+        // the cached shape was tuned on a problem of the same width with more
+        // rows (the key has no row count): this one keeps the caller's shape
+        o = asked;
+        if (int r = choose_kernel(p, o, ch)) return r;
+      }
     }
   }
   const int kernel = ch.kernel;

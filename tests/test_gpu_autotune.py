@@ -63,3 +63,20 @@ def test_autotune_batch(pkg):
     ref = pkg.calc_batch(probs)
     assert pkg.calc_batch(probs, autotune=1) == ref
     assert pkg.calc_batch(probs, autotune=1, batch_graph=1) == ref
+
+
+def test_autotune_cache_entry_from_a_taller_problem(pkg):
+    """The cache is keyed by kernel shape, not by row count: a shape tuned on
+    a tall matrix (P up to 8 needs 6 + 3 low rows and more high rows) must not
+    make a shorter problem of the same width fail; it runs with the shape it
+    would have had without tuning."""
+    from lnorm_amd import gen
+    tall = gen.gen_matrix(36, 36, 100, 21)
+    best, _ = pkg.autotune(tall, 1, 1)
+    assert best["kernel"] == pkg.KERNEL_BLOCKED
+    for rows in (8, 11, 14, 20):
+        A = tall[-rows:].copy()
+        ref = pkg.calc_lnorm(A, 1, 1)
+        got = pkg.calc_lnorm(A, 1, 1, autotune=1)
+        assert got[:2] == ref[:2] and got[2]["best_rank"] == ref[2]["best_rank"], rows
+        assert got[:2] == oracle.calc_lnorm(A, 1, 1, 8)[:2]
