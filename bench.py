@@ -53,14 +53,15 @@ def load_pkg():
     return mod
 
 
-def valu_ops_per_unit(kernel: str, ny: int, marginal: int) -> int:
+def valu_ops_per_unit(kernel: str, ny: int, marginal: int, P: int = 2) -> float:
     """Minimal VALU lane-operations per sign vector of each design:
     plain:   one add of the row delta and one |.|-accumulate (v_sad_u32) per
              row, plus the running maximum;
     blocked: one v_sad_u16 per two |.| columns (the high-row update runs on
-             the scalar unit), plus the marginal add and the maximum."""
+             the scalar unit), plus the marginal add, and the running maximum
+             shared by two patterns (one v_max3_i32; P = 1: one v_max)."""
     if kernel == "blocked":
-        return (ny - marginal + 1) // 2 + 1 + marginal
+        return (ny - marginal + 1) // 2 + marginal + (1.0 if P == 1 else 0.5)
     if kernel == "blocked32":        # one v_sad_u32 per |.| column
         return ny - marginal + 1 + marginal
     return 2 * ny + 1
@@ -101,7 +102,8 @@ def run_steps(pkg, A, kernel, steps, warmup, P=0):
 def summarize(r, total, ny, kernel, marginal=0):
     steps = len(r["step_s"])
     kms = sum(r["kernel_ms"]) / steps
-    ops = valu_ops_per_unit(kernel, ny, marginal) * total
+    P = int(r["res"].get("patterns_per_lane") or 2)
+    ops = valu_ops_per_unit(kernel, ny, marginal, P) * total
     achieved = ops / (kms / 1e3)
     return {
         "value": total * steps / r["wall_s"],
@@ -109,7 +111,7 @@ def summarize(r, total, ny, kernel, marginal=0):
         "kernel_ms": kms,
         "kernel_sign_vectors_per_s": total / (kms / 1e3),
         "row_updates_per_s": total * ny / (kms / 1e3),
-        "valu_ops_per_sign_vector": valu_ops_per_unit(kernel, ny, marginal),
+        "valu_ops_per_sign_vector": valu_ops_per_unit(kernel, ny, marginal, P),
         "valu_TOPs": achieved / 1e12,
         "valu_frac": achieved / VALU_PEAK,
         "kernel_name": r["res"]["kernel_name"],

@@ -58,17 +58,23 @@ __global__ __launch_bounds__(256) void k_blocked16(
 #pragma unroll
   for (int c = 0; c < NW; ++c) S[c] = (uint32_t)__builtin_amdgcn_readlane((int)v, c);
 
-  int32_t best[P];
-#pragma unroll
-  for (int r = 0; r < P; ++r) best[r] = INT32_MIN;
-
+  // one running maximum over the lane's P patterns: two patterns' sums and
+  // the maximum meet in one v_max3_i32, half an instruction per sign vector
+  int32_t best = INT32_MIN;
   auto eval = [&]() {
+    int32_t a[P];
 #pragma unroll
     for (int r = 0; r < P; ++r) {
       uint32_t acc = MARG ? tab[r][NP] + S[NP] : 0u;
 #pragma unroll
       for (int c = 0; c < NP; ++c) acc = sad16(tab[r][c], S[c], acc);
-      best[r] = (int32_t)acc > best[r] ? (int32_t)acc : best[r];
+      a[r] = (int32_t)acc;
+    }
+#pragma unroll
+    for (int r = 0; r < P; r += 2) {
+      const int32_t y = (r + 1 < P) ? a[r + 1] : a[r];
+      const int32_t xy = a[r] > y ? a[r] : y;
+      best = xy > best ? xy : best;
     }
   };
 
@@ -87,10 +93,7 @@ __global__ __launch_bounds__(256) void k_blocked16(
   }
   eval();                        // state of rank nsteps-1
 
-  int32_t b = best[0];
-#pragma unroll
-  for (int r = 1; r < P; ++r) b = best[r] > b ? best[r] : b;
-  const int32_t m = wave_max_i32(b);
+  const int32_t m = wave_max_i32(best);
   if (lane == 0) {
     unit_best[wl] = m;
     atomicMax(gbest, m);
