@@ -190,7 +190,7 @@ struct Geometry {
 };
 
 // The lane- or wave-per-segment walks.
-enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2 };
+enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2, WALK_WIDE = 3 };
 
 // Segments for a range of base^e ranks: aim at about 2^target_log lanes.
 // The step counter of the d-ary and packed walks holds 15 base-d digits.
@@ -199,10 +199,11 @@ Geometry geometry_for(int base, int e, int target_log, int forced, Walk walk) {
   const double lb = std::log2((double)base);
   int qmax;
   if (walk == WALK_GENERAL) qmax = (int)(62.0 / lb);
-  else if (walk == WALK_PLAIN && base == 2) qmax = 30;
+  else if ((walk == WALK_PLAIN || walk == WALK_WIDE) && base == 2) qmax = 30;
   else qmax = std::min(15, (int)(31.0 / lb));
   int q = forced > 0 ? forced : std::max(0, e - (int)std::floor(target_log / lb));
   q = std::min(q, std::min(qmax, e));
+  if (walk == WALK_WIDE) q = std::max(q, std::min(3, e));   // steps combine in groups of 8
   g.q = q;
   g.seglen = ipow((uint64_t)base, q);
   g.nseg = ipow((uint64_t)base, e - q);
@@ -240,11 +241,11 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
       }
     return;
   }
-  const int W = walk == WALK_PLAIN ? ny_pad : p.C;
+  const int W = (walk == WALK_PLAIN || walk == WALK_WIDE) ? ny_pad : p.C;
   pp.A.assign((size_t)p.R * W, 0);
   for (int i = 0; i < p.R; ++i)
     for (int l = 0; l < p.C; ++l) pp.A[(size_t)i * W + l] = p.A[(size_t)i * p.C + l];
-  if (walk == WALK_PLAIN && p.n <= 2 && p.R > 1) {
+  if ((walk == WALK_PLAIN || walk == WALK_WIDE) && p.n <= 2 && p.R > 1) {
     // D[1][j] = +f A[j], D[0][j] = -f A[j]; f = 2 for L1/LM (y += 2A), 1 for L2
     const uint32_t f = p.n == 1 ? 2u : 1u;
     pp.D.assign((size_t)2 * (p.R - 1) * W, 0);
@@ -262,6 +263,7 @@ hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const Pl
   switch (pp.walk) {
     case WALK_PLAIN: return launch_plain(p.kind, mode, a, st, block);
     case WALK_PLAIN16: return launch_plain16(mode, a, st, block);
+    case WALK_WIDE: return launch_wide(p.kind, mode, a, st);
     default: return launch_general(p.kind, mode, a, p.C, st);
   }
 }
@@ -601,7 +603,7 @@ int select_device(const lnorm_hip_opts &o, int &dev) {
 struct Choice {
   int kernel = LNORM_KERNEL_AUTO;
   BlockedPlan bp;
-  int ny_plain = 0, w16 = 0;
+  int ny_plain = 0, w16 = 0, ny_wide = 0;
 };
 
 int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
@@ -612,17 +614,24 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
   c.ny_plain = plain_ny_pad(p.kind, p.n, p.C);
   const bool fits16 = p.n >= 2 && max_col_abs_sum(p, 0) <= 32767;
   c.w16 = fits16 ? plain16_words(p.n, p.C) : 0;
+  // the W-wave walk combines steps in groups of 8: at least 3 enumerated rows
+  c.ny_wide = (p.R - 1 >= 3) ? wide_ny_pad(p.kind, p.C) : 0;
   if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
     return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
   if (kernel == LNORM_KERNEL_PLAIN && !c.ny_plain)
     return fail(LNORM_E_UNSUPPORTED, "no plain instantiation for %d columns at n = %d", p.C, p.n);
+  if (kernel == LNORM_KERNEL_WIDE && !c.ny_wide)
+    return fail(LNORM_E_UNSUPPORTED, "W-wave walk needs n <= 2, at least 3 enumerated rows and an instantiated "
+                "width (%d columns at n = %d)", p.C, p.n);
   if (kernel == LNORM_KERNEL_PLAIN16 && !c.w16)
     return fail(LNORM_E_UNSUPPORTED, "packed 16-bit walk needs n >= 2, column sums <= 32767 and an instantiated "
                 "width (%d columns at n = %d)", p.C, p.n);
   if (kernel == LNORM_KERNEL_AUTO)
     kernel = can_block ? LNORM_KERNEL_BLOCKED
-                       : (c.w16 ? LNORM_KERNEL_PLAIN16 : (c.ny_plain ? LNORM_KERNEL_PLAIN : LNORM_KERNEL_GENERAL));
-  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_PLAIN16)
+                       : (c.w16 ? LNORM_KERNEL_PLAIN16
+                                : (c.ny_plain ? LNORM_KERNEL_PLAIN
+                                              : (c.ny_wide ? LNORM_KERNEL_WIDE : LNORM_KERNEL_GENERAL)));
+  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_WIDE)
     return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
   c.kernel = kernel;
   return 0;
@@ -630,6 +639,8 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
 
 void prepare_for(const Problem &p, const Choice &c, Prepared &pp) {
   if (c.kernel == LNORM_KERNEL_PLAIN16) prepare_walk(p, WALK_PLAIN16, 2 * c.w16, pp);
+  else if (c.kernel == LNORM_KERNEL_WIDE) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
+  else if (c.kernel == LNORM_KERNEL_BLOCKED && !c.ny_plain && c.ny_wide) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
   else if (c.kernel == LNORM_KERNEL_GENERAL || !c.ny_plain) prepare_walk(p, WALK_GENERAL, 0, pp);
   else prepare_walk(p, WALK_PLAIN, c.ny_plain, pp);
 }
@@ -827,6 +838,8 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "general<kind=%d,d=%d,q=%d>", (int)p.kind, p.n, g.q);
     else if (pp.walk == WALK_PLAIN16)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain16<d=%d,words=%d,q=%d>", p.n, pp.ny_pad / 2, g.q);
+    else if (pp.walk == WALK_WIDE)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide<kind=%d,ny=%d,q=%d>", (int)p.kind, pp.ny_pad, g.q);
     else
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain<kind=%d,d=%d,ny=%d,q=%d>", (int)p.kind, p.n,
                pp.ny_pad, g.q);

@@ -245,3 +245,35 @@ def test_blocked_equals_plain_large(pkg):
         npl, sp, rp = pkg.calc_lnorm(A, 1, marg, kernel="plain")
         assert (nb, sb, rb["best_rank"]) == (npl, sp, rp["best_rank"])
         assert oracle.objective(A, 1, marg, sb) == nb
+
+
+@pytest.mark.parametrize("n,marg,R,cols", [
+    (1, 0, 14, [129, 200, 256, 300, 384, 500, 700, 1000]),
+    (1, 1, 13, [130, 400, 1024]),
+    (2, 0, 12, [200, 300, 512]),
+    (1, 0, 4, [200]),
+])
+def test_wide_walk_vs_oracle(pkg, n, marg, R, cols):
+    """The W-wave walk (columns split over the waves of a workgroup, LDS
+    combine every 8 steps): bit-exact norm, strategy and rank for widths the
+    register-resident walks do not hold."""
+    rng = np.random.default_rng(700 + 10 * n + marg + R)
+    for C in cols:
+        A = _rand(rng, R, C, 40)
+        print("wide", n, marg, A.shape, flush=True)
+        res = _agree(pkg, A, n, marg)
+        assert res["kernel_name"].startswith("wide<"), res["kernel_name"]
+    # narrower problems on the forced W-wave walk (AUTO keeps them elsewhere)
+    A = _rand(rng, R, 65, 40)
+    assert _agree(pkg, A, n, marg, kernel="wide")["kernel_name"].startswith("wide<")
+
+
+def test_wide_equals_general_larger(pkg):
+    from lnorm_amd import gen
+    for (R, C, n) in ((24, 600, 1), (20, 300, 2)):
+        A = gen.gen_matrix(R, C, 30, 17 + n)
+        print("wide vs general", R, C, n, flush=True)
+        w = pkg.calc_lnorm(A, n, 0, kernel="wide")
+        g = pkg.calc_lnorm(A, n, 0, kernel="general")
+        assert w[:2] == g[:2] and w[2]["best_rank"] == g[2]["best_rank"]
+        assert oracle.objective(A, n, 0, w[1]) == w[0]
