@@ -20,6 +20,15 @@ __device__ __forceinline__ uint32_t sad16(uint32_t a, uint32_t b, uint32_t c) {
 
 __device__ __forceinline__ int wave_lane() { return (int)(threadIdx.x & 63); }
 
+// A zero the compiler cannot see through.  |y| + 0 written as sad32(y, B, 0)
+// is folded into min/max/sub (three instructions); with this accumulator it
+// stays one v_sad_u32.  Not volatile: computed once, outside the loops.
+__device__ __forceinline__ uint32_t opaque_zero() {
+  uint32_t z;
+  asm("v_mov_b32 %0, 0" : "=v"(z));
+  return z;
+}
+
 // Global wave index of the calling thread.  It is uniform within the wave;
 // readfirstlane tells the compiler so, which keeps everything derived from it
 // in SGPRs.

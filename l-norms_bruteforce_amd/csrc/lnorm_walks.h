@@ -73,6 +73,7 @@ __device__ __forceinline__ void walk_bin(
 
   int32_t best = INT32_MIN;
   uint32_t first = 0xFFFFFFFFu;
+  const uint32_t zero = opaque_zero();
   auto eval = [&](uint32_t t) {
     uint32_t acc;
     if constexpr (KIND == KIND_LM) {
@@ -80,7 +81,7 @@ __device__ __forceinline__ void walk_bin(
 #pragma unroll
       for (int l = 1; l < NY; ++l) acc = sad32(y[l], B, acc);
     } else {
-      acc = 0;
+      acc = zero;
 #pragma unroll
       for (int l = 0; l < NY; ++l) {
         acc = sad32(y[l], B, acc);

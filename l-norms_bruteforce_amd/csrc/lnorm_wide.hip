@@ -64,6 +64,7 @@ __global__ __launch_bounds__(64 * W) void k_wide_bin(
 
   int32_t best = INT32_MIN;
   uint32_t first = 0xFFFFFFFFu;
+  const uint32_t zero = opaque_zero();
   const uint32_t nsteps = 1u << q;       // the host guarantees q >= 3 (nsteps % K == 0)
   const uint32_t nu = par << q;
   for (uint32_t t0 = 0; t0 < nsteps; t0 += K) {
@@ -85,10 +86,10 @@ __global__ __launch_bounds__(64 * W) void k_wide_bin(
           }
         }
       }
-      uint32_t acc = 0;
+      uint32_t acc = zero;
       if constexpr (KIND == KIND_LM) {
         // column 0 enters without |.| (L_OpenMP.c:40-41); it is wave 0's first
-        acc = mw ? (y[0] ^ kBias32) : sad32(y[0], B, 0u);
+        acc = mw ? (y[0] ^ kBias32) : sad32(y[0], B, zero);
 #pragma unroll
         for (int l = 1; l < NY; ++l) acc = sad32(y[l], B, acc);
       } else {
