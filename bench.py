@@ -198,6 +198,25 @@ def cpu_baseline(gen, rows: int = 64, r: int = 127, seed: int = 104, budget_cols
             "sample": sample, "seconds": c["seconds"]}
 
 
+def single_line(pkg, A, n: int, total: int, ops, int64: bool = False) -> dict:
+    """One call on AUTO (after a warm-up call): labelings/s, the kernel AUTO
+    picked and, with an op count per unit, the VALU fraction by that model.
+    Used for the W-wave walk (L1 of a 36 x 256 matrix, 2^35 sign vectors,
+    2 ops per row-update) and the 64-bit walk (L1 of a 34 x 40 matrix scaled
+    by 2^28 + 3, beyond int32: 2^33 sign vectors)."""
+    calc = pkg.calc_lnorm64 if int64 else pkg.calc_lnorm
+    calc(A, n, 0)
+    t = time.perf_counter()
+    norm, strat, res = calc(A, n, 0)
+    wall = time.perf_counter() - t
+    kms = res["ms_kernel"]
+    out = {"value": total / wall, "unit": "sign-vectors/s", "kernel_ms": kms, "kernel": res["kernel_name"],
+           "kernel_sign_vectors_per_s": total / (kms / 1e3), "norm": norm}
+    if ops:
+        out["valu_frac"] = total * ops / (kms / 1e3) / VALU_PEAK
+    return out
+
+
 def cli_line(pkg, gen) -> dict:
     """The drop-in CLI end to end on the headline file (64 x 40, seed 104):
     `L_HIP --omp 16 m.txt 1`, a fresh process each time: reading,
@@ -483,6 +502,9 @@ def main():
                                 "blocked_valu_frac": s3["valu_frac"], "plain_valu_frac": s4["valu_frac"],
                                 "kernel": s3["kernel_name"], "norm": s3["norm"]}
         line["cli_end_to_end"] = cli_line(pkg, gen)
+        line["wide_36x256"] = single_line(pkg, gen.gen_matrix(36, 256, 100, 107), 1, 2 ** 35, 2 * 256 + 1)
+        line["int64_34x40"] = single_line(pkg, gen.gen_matrix(34, 40, 100, 108).astype("int64") * (2 ** 28 + 3),
+                                          1, 2 ** 33, None, int64=True)
         line["L3_24x32"] = l3_line(pkg, gen, a.steps, a.warmup, not a.no_cpu)
         line["batch_4096"] = batch_line(pkg, gen, max(1, a.steps // 2))
     if not a.no_cpu:
