@@ -190,7 +190,7 @@ struct Geometry {
 };
 
 // The lane- or wave-per-segment walks.
-enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2, WALK_WIDE = 3 };
+enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2, WALK_WIDE = 3, WALK_WIDE16 = 4 };
 
 // Segments for a range of base^e ranks: aim at about 2^target_log lanes.
 // The step counter of the d-ary and packed walks holds 15 base-d digits.
@@ -204,6 +204,7 @@ Geometry geometry_for(int base, int e, int target_log, int forced, Walk walk) {
   int q = forced > 0 ? forced : std::max(0, e - (int)std::floor(target_log / lb));
   q = std::min(q, std::min(qmax, e));
   if (walk == WALK_WIDE) q = std::max(q, std::min(3, e));   // steps combine in groups of 8
+  if (walk == WALK_WIDE16) q = std::max(q, std::min(2, e));
   g.q = q;
   g.seglen = ipow((uint64_t)base, q);
   g.nseg = ipow((uint64_t)base, e - q);
@@ -230,7 +231,7 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
   pp.walk = walk;
   pp.ny_pad = walk == WALK_GENERAL ? 0 : ny_pad;
   pp.D.clear();
-  if (walk == WALK_PLAIN16) {
+  if (walk == WALK_PLAIN16 || walk == WALK_WIDE16) {
     // two columns per word, low half = even column
     const int NW = ny_pad / 2;
     pp.A.assign((size_t)p.R * NW, 0);
@@ -264,6 +265,7 @@ hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const Pl
     case WALK_PLAIN: return launch_plain(p.kind, mode, a, st, block);
     case WALK_PLAIN16: return launch_plain16(mode, a, st, block);
     case WALK_WIDE: return launch_wide(p.kind, mode, a, st);
+    case WALK_WIDE16: return launch_wide16(mode, a, st);
     default: return launch_general(p.kind, mode, a, p.C, st);
   }
 }
@@ -603,7 +605,7 @@ int select_device(const lnorm_hip_opts &o, int &dev) {
 struct Choice {
   int kernel = LNORM_KERNEL_AUTO;
   BlockedPlan bp;
-  int ny_plain = 0, w16 = 0, ny_wide = 0;
+  int ny_plain = 0, w16 = 0, ny_wide = 0, w16_wide = 0;
 };
 
 int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
@@ -616,13 +618,14 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
   c.w16 = fits16 ? plain16_words(p.n, p.C) : 0;
   // the W-wave walk combines steps in groups of 8: at least 3 enumerated rows
   c.ny_wide = (p.R - 1 >= 3) ? wide_ny_pad(p.kind, p.C) : 0;
+  c.w16_wide = (fits16 && p.n >= 3) ? wide16_words(p.n, p.C) : 0;
   if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
     return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
   if (kernel == LNORM_KERNEL_PLAIN && !c.ny_plain)
     return fail(LNORM_E_UNSUPPORTED, "no plain instantiation for %d columns at n = %d", p.C, p.n);
-  if (kernel == LNORM_KERNEL_WIDE && !c.ny_wide)
-    return fail(LNORM_E_UNSUPPORTED, "W-wave walk needs n <= 2, at least 3 enumerated rows and an instantiated "
-                "width (%d columns at n = %d)", p.C, p.n);
+  if (kernel == LNORM_KERNEL_WIDE && !c.ny_wide && !c.w16_wide)
+    return fail(LNORM_E_UNSUPPORTED, "W-wave walk needs an instantiated width (%d columns at n = %d) and, for "
+                "n <= 2, at least 3 enumerated rows, for n >= 3, column sums <= 32767", p.C, p.n);
   if (kernel == LNORM_KERNEL_PLAIN16 && !c.w16)
     return fail(LNORM_E_UNSUPPORTED, "packed 16-bit walk needs n >= 2, column sums <= 32767 and an instantiated "
                 "width (%d columns at n = %d)", p.C, p.n);
@@ -630,7 +633,8 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
     kernel = can_block ? LNORM_KERNEL_BLOCKED
                        : (c.w16 ? LNORM_KERNEL_PLAIN16
                                 : (c.ny_plain ? LNORM_KERNEL_PLAIN
-                                              : (c.ny_wide ? LNORM_KERNEL_WIDE : LNORM_KERNEL_GENERAL)));
+                                              : ((c.ny_wide || c.w16_wide) ? LNORM_KERNEL_WIDE
+                                                                            : LNORM_KERNEL_GENERAL)));
   if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_WIDE)
     return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
   c.kernel = kernel;
@@ -639,6 +643,7 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
 
 void prepare_for(const Problem &p, const Choice &c, Prepared &pp) {
   if (c.kernel == LNORM_KERNEL_PLAIN16) prepare_walk(p, WALK_PLAIN16, 2 * c.w16, pp);
+  else if (c.kernel == LNORM_KERNEL_WIDE && c.w16_wide) prepare_walk(p, WALK_WIDE16, 2 * c.w16_wide, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
   else if (c.kernel == LNORM_KERNEL_BLOCKED && !c.ny_plain && c.ny_wide) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
   else if (c.kernel == LNORM_KERNEL_GENERAL || !c.ny_plain) prepare_walk(p, WALK_GENERAL, 0, pp);
@@ -840,6 +845,8 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain16<d=%d,words=%d,q=%d>", p.n, pp.ny_pad / 2, g.q);
     else if (pp.walk == WALK_WIDE)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide<kind=%d,ny=%d,q=%d>", (int)p.kind, pp.ny_pad, g.q);
+    else if (pp.walk == WALK_WIDE16)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide16<d=%d,words=%d,q=%d>", p.n, pp.ny_pad / 2, g.q);
     else
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain<kind=%d,d=%d,ny=%d,q=%d>", (int)p.kind, p.n,
                pp.ny_pad, g.q);

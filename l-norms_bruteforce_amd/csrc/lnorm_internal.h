@@ -128,6 +128,8 @@ hipError_t launch_plain(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st
 hipError_t launch_plain16(Mode mode, const PlainArgs &a, hipStream_t st, int block);
 int wide_ny_pad(Kind kind, int ny);   // 0: no W-wave instantiation for this width
 hipError_t launch_wide(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st);
+int wide16_words(int d, int ny);      // packed W-wave width in words (0: none)
+hipError_t launch_wide16(Mode mode, const PlainArgs &a, hipStream_t st);
 int plain16_words(int d, int ny);               // packed width instantiated for ny columns, 0 if none
 bool plain_supported(Kind kind, int d, int ny_pad);
 int plain_ny_pad(Kind kind, int d, int ny);     // smallest instantiated width >= ny, 0 if none

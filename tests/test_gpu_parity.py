@@ -268,6 +268,64 @@ def test_wide_walk_vs_oracle(pkg, n, marg, R, cols):
     assert _agree(pkg, A, n, marg, kernel="wide")["kernel_name"].startswith("wide<")
 
 
+# (d, rows, column counts): the first count is the narrowest that one wave's
+# packed walk does not hold; the others reach the 2-, 4- and 8-wave shapes
+# (d * words-per-wave state registers: 40 words at d = 3 ... 12 at d >= 9)
+WIDE16 = [
+    (3, 12, [97, 160, 161, 301, 640]),
+    (4, 10, [65, 112, 224, 447]),
+    (5, 8, [65, 96, 180, 384]),
+    (6, 8, [49, 80, 160, 320]),
+    (7, 7, [33, 64, 128, 255]),
+    (8, 7, [33, 56, 112, 224]),
+    (9, 6, [33, 48, 96, 192]),
+    (10, 6, [33, 47, 96, 191]),
+]
+
+
+@pytest.mark.parametrize("d,R,cols", WIDE16)
+def test_wide16_walk_vs_oracle(pkg, d, R, cols):
+    """The packed d-ary walk over W waves: bit-exact norm, strategy and rank
+    beyond the widths one wave holds, at the default segment length and at
+    forced ones (d^q steps per segment, combined in groups of 8 with a
+    shorter last group)."""
+    rng = np.random.default_rng(800 + d)
+    for C in cols:
+        A = _rand(rng, R, C, 60)
+        print("wide16", d, A.shape, flush=True)
+        res = _agree(pkg, A, d, 0)
+        assert res["kernel_name"].startswith("wide16<"), res["kernel_name"]
+    A = _rand(rng, R, cols[1], 60)
+    for q in (2, 3, 5):   # q = 5 at d >= 9 (5 enumerated rows): one segment
+        print("wide16 seg_log", d, A.shape, q, flush=True)
+        res = _agree(pkg, A, d, 0, seg_log=q, kernel="wide")
+        assert res["kernel_name"].startswith("wide16<") and f"q={q}>" in res["kernel_name"], res["kernel_name"]
+
+
+def test_wide16_equals_general_larger(pkg):
+    from lnorm_amd import gen
+    for (R, C, d) in ((16, 300, 3), (11, 150, 5), (8, 100, 9)):
+        A = gen.gen_matrix(R, C, 50, 30 + d)
+        print("wide16 vs general", R, C, d, flush=True)
+        w = pkg.calc_lnorm(A, d, 0)
+        g = pkg.calc_lnorm(A, d, 0, kernel="general")
+        assert w[2]["kernel_name"].startswith("wide16<"), w[2]["kernel_name"]
+        assert w[:2] == g[:2] and w[2]["best_rank"] == g[2]["best_rank"]
+        assert oracle.objective(A, d, 0, w[1]) == w[0]
+
+
+def test_wide16_needs_16bit_sums(pkg):
+    """Column sums above 32767 do not fit the packed halves: AUTO takes the
+    general kernel and a forced W-wave walk is refused."""
+    rng = np.random.default_rng(5)
+    A = _rand(rng, 6, 120, 30000)
+    res = _agree(pkg, A, 3, 0)
+    assert res["kernel_name"].startswith("general<"), res["kernel_name"]
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm(A, 3, 0, kernel="wide")
+    assert e.value.code == -5
+
+
 def test_wide_equals_general_larger(pkg):
     from lnorm_amd import gen
     for (R, C, n) in ((24, 600, 1), (20, 300, 2)):
