@@ -39,7 +39,8 @@ extern "C" {
 #define LNORM_KERNEL_BLOCKED 2  /* L1/LM: MFMA-built low-row table + scalar high walk (16- or 32-bit words) */
 #define LNORM_KERNEL_GENERAL 3  /* any width: wave per segment, columns over lanes */
 #define LNORM_KERNEL_PLAIN16 4  /* n >= 2: packed 16-bit Gray walk (column sums <= 32767) */
-#define LNORM_KERNEL_WIDE 5     /* n <= 2, 129..1024 columns (L2: ..512): W waves split the columns */
+#define LNORM_KERNEL_WIDE 5     /* W waves split the columns: n <= 2 at 129..1024 columns (L2: ..512);
+                                   n = 3..10 packed, column sums <= 32767, up to 640 (d = 3) .. 192 (d >= 9) */
 
 typedef struct lnorm_hip_opts {
     int32_t kernel;             /* LNORM_KERNEL_* */
