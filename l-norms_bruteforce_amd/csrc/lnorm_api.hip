@@ -190,11 +190,12 @@ struct Geometry {
 };
 
 // The lane- or wave-per-segment walks.
-enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2, WALK_WIDE = 3, WALK_WIDE16 = 4 };
+enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2, WALK_WIDE = 3, WALK_WIDE16 = 4, WALK_TAB16 = 5 };
 
 // Segments for a range of base^e ranks: aim at about 2^target_log lanes.
 // The step counter of the d-ary and packed walks holds 15 base-d digits.
-Geometry geometry_for(int base, int e, int target_log, int forced, Walk walk) {
+// qmin: the walk's smallest segment (the tabulated walk's table rows).
+Geometry geometry_for(int base, int e, int target_log, int forced, Walk walk, int qmin = 0) {
   Geometry g;
   const double lb = std::log2((double)base);
   int qmax;
@@ -205,6 +206,7 @@ Geometry geometry_for(int base, int e, int target_log, int forced, Walk walk) {
   q = std::min(q, std::min(qmax, e));
   if (walk == WALK_WIDE) q = std::max(q, std::min(3, e));   // steps combine in groups of 8
   if (walk == WALK_WIDE16) q = std::max(q, std::min(2, e));
+  q = std::max(q, std::min(qmin, e));
   g.q = q;
   g.seglen = ipow((uint64_t)base, q);
   g.nseg = ipow((uint64_t)base, e - q);
@@ -214,6 +216,7 @@ Geometry geometry_for(int base, int e, int target_log, int forced, Walk walk) {
 struct Prepared {
   Walk walk = WALK_PLAIN;
   int ny_pad = 0;      // kernel width in columns (packed walk: 2 x words)
+  int qmin = 0;        // smallest segment exponent (tabulated walk: its table rows)
   std::vector<int32_t> A, D;
 };
 
@@ -230,8 +233,9 @@ int64_t max_col_abs_sum(const Problem &p, int first_col) {
 void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
   pp.walk = walk;
   pp.ny_pad = walk == WALK_GENERAL ? 0 : ny_pad;
+  pp.qmin = 0;
   pp.D.clear();
-  if (walk == WALK_PLAIN16 || walk == WALK_WIDE16) {
+  if (walk == WALK_PLAIN16 || walk == WALK_WIDE16 || walk == WALK_TAB16) {
     // two columns per word, low half = even column
     const int NW = ny_pad / 2;
     pp.A.assign((size_t)p.R * NW, 0);
@@ -240,6 +244,21 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
         const uint32_t v = (uint32_t)p.A[(size_t)i * p.C + l] & 0xFFFFu;
         pp.A[(size_t)i * NW + l / 2] |= (int32_t)(v << (16 * (l & 1)));
       }
+    if (walk == WALK_TAB16) {
+      // subset table of the M fastest rows: word w of subset S holds
+      // -sum_{i in S} A[i][col] + 2^15 for its two columns
+      const int M = blk16_rows(p.n, NW);
+      pp.qmin = M;
+      pp.D.assign((size_t)NW << M, 0);
+      for (int S = 0; S < (1 << M); ++S)
+        for (int l = 0; l < 2 * NW; ++l) {
+          int32_t sum = 0;
+          for (int i = 0; i < M; ++i)
+            if (((S >> i) & 1) && l < p.C) sum += p.A[(size_t)i * p.C + l];
+          const uint32_t v = (uint32_t)(kBias16 - sum) & 0xFFFFu;
+          pp.D[(size_t)S * NW + l / 2] |= (int32_t)(v << (16 * (l & 1)));
+        }
+    }
     return;
   }
   const int W = (walk == WALK_PLAIN || walk == WALK_WIDE) ? ny_pad : p.C;
@@ -266,6 +285,7 @@ hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const Pl
     case WALK_PLAIN16: return launch_plain16(mode, a, st, block);
     case WALK_WIDE: return launch_wide(p.kind, mode, a, st);
     case WALK_WIDE16: return launch_wide16(mode, a, st);
+    case WALK_TAB16: return launch_blk16(mode, a, st, block);
     default: return launch_general(p.kind, mode, a, p.C, st);
   }
 }
@@ -334,7 +354,7 @@ int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometr
 int run_walk_first(DevCtx &cx, const Problem &p, const Prepared &pp, uint64_t u, int e, int32_t target,
                    hipStream_t st, int block, uint64_t &rank) {
   if (int r = ensure_result_bufs(cx)) return r;
-  const Geometry g = geometry_for(p.base, e, 16, 0, pp.walk);
+  const Geometry g = geometry_for(p.base, e, 16, 0, pp.walk, pp.qmin);
   unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
   HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
   PlainArgs a{};
@@ -605,7 +625,7 @@ int select_device(const lnorm_hip_opts &o, int &dev) {
 struct Choice {
   int kernel = LNORM_KERNEL_AUTO;
   BlockedPlan bp;
-  int ny_plain = 0, w16 = 0, ny_wide = 0, w16_wide = 0;
+  int ny_plain = 0, w16 = 0, ny_wide = 0, w16_wide = 0, wtab = 0;
 };
 
 int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
@@ -619,6 +639,8 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
   // the W-wave walk combines steps in groups of 8: at least 3 enumerated rows
   c.ny_wide = (p.R - 1 >= 3) ? wide_ny_pad(p.kind, p.C) : 0;
   c.w16_wide = (fits16 && p.n >= 3) ? wide16_words(p.n, p.C) : 0;
+  c.wtab = (fits16 && p.n >= 3) ? blk16_words(p.n, p.C) : 0;
+  if (c.wtab && p.R - 1 < blk16_rows(p.n, c.wtab)) c.wtab = 0;   // fewer enumerated rows than table rows
   if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
     return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
   if (kernel == LNORM_KERNEL_PLAIN && !c.ny_plain)
@@ -626,16 +648,20 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
   if (kernel == LNORM_KERNEL_WIDE && !c.ny_wide && !c.w16_wide)
     return fail(LNORM_E_UNSUPPORTED, "W-wave walk needs an instantiated width (%d columns at n = %d) and, for "
                 "n <= 2, at least 3 enumerated rows, for n >= 3, column sums <= 32767", p.C, p.n);
+  if (kernel == LNORM_KERNEL_TABLE16 && !c.wtab)
+    return fail(LNORM_E_UNSUPPORTED, "tabulated packed walk needs n = 3..10, column sums <= 32767, an "
+                "instantiated width and more enumerated rows than table rows (%d x %d at n = %d)", p.R, p.C, p.n);
   if (kernel == LNORM_KERNEL_PLAIN16 && !c.w16)
     return fail(LNORM_E_UNSUPPORTED, "packed 16-bit walk needs n >= 2, column sums <= 32767 and an instantiated "
                 "width (%d columns at n = %d)", p.C, p.n);
   if (kernel == LNORM_KERNEL_AUTO)
     kernel = can_block ? LNORM_KERNEL_BLOCKED
+                       : c.wtab ? LNORM_KERNEL_TABLE16
                        : (c.w16 ? LNORM_KERNEL_PLAIN16
                                 : (c.ny_plain ? LNORM_KERNEL_PLAIN
                                               : ((c.ny_wide || c.w16_wide) ? LNORM_KERNEL_WIDE
                                                                             : LNORM_KERNEL_GENERAL)));
-  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_WIDE)
+  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_TABLE16)
     return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
   c.kernel = kernel;
   return 0;
@@ -643,6 +669,7 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
 
 void prepare_for(const Problem &p, const Choice &c, Prepared &pp) {
   if (c.kernel == LNORM_KERNEL_PLAIN16) prepare_walk(p, WALK_PLAIN16, 2 * c.w16, pp);
+  else if (c.kernel == LNORM_KERNEL_TABLE16) prepare_walk(p, WALK_TAB16, 2 * c.wtab, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE && c.w16_wide) prepare_walk(p, WALK_WIDE16, 2 * c.w16_wide, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
   else if (c.kernel == LNORM_KERNEL_BLOCKED && !c.ny_plain && c.ny_wide) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
@@ -692,6 +719,8 @@ TuneKey tune_key(int dev, const Problem &p, const Choice &c) {
     k.wide = c.bp.wide ? 1 : 0;
   } else if (c.kernel == LNORM_KERNEL_PLAIN16) {
     k.width = c.w16;
+  } else if (c.kernel == LNORM_KERNEL_TABLE16) {
+    k.width = c.wtab;
   } else {
     k.width = c.ny_plain;
   }
@@ -734,15 +763,16 @@ int tune_single(DevCtx &cx, const Problem &p, const Choice &c, const lnorm_hip_o
         note(e);
       }
     }
-  } else if (c.kernel == LNORM_KERNEL_PLAIN || c.kernel == LNORM_KERNEL_PLAIN16) {
+  } else if (c.kernel == LNORM_KERNEL_PLAIN || c.kernel == LNORM_KERNEL_PLAIN16 ||
+             c.kernel == LNORM_KERNEL_TABLE16) {
     Prepared pp;
     prepare_for(p, c, pp);
     if (int r = upload_walk(cx, p, pp, st)) return r;
     const double lb = std::log2((double)p.base);
-    const Geometry full = geometry_for(p.base, p.R - 1, 20, o.seg_log, pp.walk);
+    const Geometry full = geometry_for(p.base, p.R - 1, 20, o.seg_log, pp.walk, pp.qmin);
     // segments of at most ~2^14 ranks, 2^20 of them: the probe's shape
     const int qp = std::min(full.q, (int)std::floor(14.0 / lb));
-    const Geometry g = geometry_for(p.base, p.R - 1, 63, std::max(qp, 0), pp.walk);
+    const Geometry g = geometry_for(p.base, p.R - 1, 63, std::max(qp, 0), pp.walk, pp.qmin);
     for (int block : {64, 128, 256}) {
       Phase1Out out;
       if (qp <= 0) break;
@@ -782,7 +812,7 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
   Choice ch;
   if (int r = choose_kernel(p, o, ch)) return r;
   if (o.autotune && (ch.kernel == LNORM_KERNEL_BLOCKED || ch.kernel == LNORM_KERNEL_PLAIN ||
-                     ch.kernel == LNORM_KERNEL_PLAIN16)) {
+                     ch.kernel == LNORM_KERNEL_PLAIN16 || ch.kernel == LNORM_KERNEL_TABLE16)) {
     // launch shapes from the cache (tuned on the first miss); explicit
     // options of the caller win
     const TuneKey key = tune_key(dev, p, ch);
@@ -836,7 +866,7 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "blocked16<np=%d,P=%d,marg=%d,s=%d>", bp.np, bp.P,
                p.kind == KIND_LM, bp.s);
   } else {
-    const Geometry g = geometry_for(p.base, p.R - 1, 20, o.seg_log, pp.walk);
+    const Geometry g = geometry_for(p.base, p.R - 1, 20, o.seg_log, pp.walk, pp.qmin);
     if (int r = run_walk_max(cx, p, pp, g, st, o.block, p1)) return r;
     unit_e = g.q;
     if (general)
@@ -847,6 +877,9 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide<kind=%d,ny=%d,q=%d>", (int)p.kind, pp.ny_pad, g.q);
     else if (pp.walk == WALK_WIDE16)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide16<d=%d,words=%d,q=%d>", p.n, pp.ny_pad / 2, g.q);
+    else if (pp.walk == WALK_TAB16)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16<d=%d,words=%d,M=%d,q=%d>", p.n, pp.ny_pad / 2,
+               pp.qmin, g.q);
     else
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain<kind=%d,d=%d,ny=%d,q=%d>", (int)p.kind, p.n,
                pp.ny_pad, g.q);
@@ -1368,7 +1401,7 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
   if (o.want_strategy && R > 1 && !strategy) return fail(LNORM_E_ARG, "strategy output is NULL");
   const int ny_plain = plain64_ny_pad(p.kind, p.n, C);
   int kernel = o.kernel;
-  if (kernel == LNORM_KERNEL_BLOCKED || kernel == LNORM_KERNEL_PLAIN16)
+  if (kernel == LNORM_KERNEL_BLOCKED || kernel == LNORM_KERNEL_PLAIN16 || kernel == LNORM_KERNEL_TABLE16)
     return fail(LNORM_E_UNSUPPORTED, "the blocked and packed 16-bit kernels have no 64-bit counterpart");
   if (kernel == LNORM_KERNEL_PLAIN && !ny_plain)
     return fail(LNORM_E_UNSUPPORTED, "no 64-bit plain instantiation for %d columns at n = %d", C, n);

@@ -129,6 +129,9 @@ hipError_t launch_plain16(Mode mode, const PlainArgs &a, hipStream_t st, int blo
 int wide_ny_pad(Kind kind, int ny);   // 0: no W-wave instantiation for this width
 hipError_t launch_wide(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st);
 int wide16_words(int d, int ny);      // packed W-wave width in words (0: none)
+int blk16_words(int d, int ny);       // tabulated packed walk: width in words (0: none)
+int blk16_rows(int d, int words);     // its tabulated rows M
+hipError_t launch_blk16(Mode mode, const PlainArgs &a, hipStream_t st, int block);
 hipError_t launch_wide16(Mode mode, const PlainArgs &a, hipStream_t st);
 int plain16_words(int d, int ny);               // packed width instantiated for ny columns, 0 if none
 bool plain_supported(Kind kind, int d, int ny_pad);

@@ -8,6 +8,8 @@
 #pragma once
 #include "lnorm_device.h"
 
+#include <utility>
+
 namespace lnorm {
 
 // Matrix and delta rows are read through the constant address space: every
@@ -486,6 +488,259 @@ __device__ __forceinline__ void walk_p16(
   }
 }
 
-// largest packed width instantiated per d (d * NW <= 160 words of state)
+// ---------------------------------------------------------------------------
+// Blocked packed d-ary walk (d = 3..10): the M fastest rows are tabulated.
+//
+// Rank N = h * d^M + l.  The high index h walks rows M..R-2 by the reflected
+// d-ary code (digit j of h is row M + j); the low index l runs through all
+// d^M labelings of rows 0..M-1.  Digit i < M of N is P[(N / d^i) mod 2d]
+// (L_OpenMP.c:127, :211-214), and N / d^i = h * d^(M-i) + l / d^i, where
+// h * d^(M-i) = d * (h & 1) mod 2d for odd d, and for even d when i = M-1
+// (else 0 mod 2d).  So the low labels of rank N are those of rank l,
+// complemented (c -> d-1-c) when h is odd: every low label for odd d, the
+// top one (row M-1) for even d.
+//
+// For label c let u_c be its packed sum over the rows outside the table
+// (rows M..R-2 by their labels, and row R-1, which has label 0).  For a
+// subset S of the low rows let
+//   F_c(S) = sum_l |u_c,l + A_S,l| = sum_w sad16(u_c[w], neg_S[w]),
+// with neg_S = -A_S + 2^15 per 16-bit half (a host-built table, read as
+// scalar operands; S = {} gives the plain |u_c|).  A labeling puts the low
+// rows into disjoint subsets S_c, and its objective is sum_c F_c(S_c).
+// A high step moves one row between two labels and changes the tables of
+// those two labels only: 2 * 2^M * NW v_sad_u16 for the d^M labelings of
+// the block.  Each labeling then costs one sum of table entries at compile-
+// time indices and half a v_max3:
+//   d = 3: F_0 + F_1 + F_2, one v_add3;
+//   d >= 4: sum_c |u_c| + sum over the labels with S_c != {} of
+//     G_c(S_c) = F_c(S_c) - F_c({}), at most M terms.
+// Exact under the packed walk's bound (every column's absolute sum <= 32767),
+// which bounds every |u_c,l + A_S,l|.
+// ---------------------------------------------------------------------------
+constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
+
+// Rows tabulated: d^M labelings per high step against 2^(M+1) table sums
+// per state word; d * NW state plus d * 2^M table registers stay below 220.
+constexpr int blk16_low_rows(int d, int nw) {
+  return d == 3 ? (nw <= 32 ? 5 : 4) : d == 4 ? 4 : (d == 5 || d == 6) ? 3 : (d >= 7 && d <= 10) ? 2 : 0;
+}
+
+template <int D, int M>
+struct LowCode {
+  static constexpr int NPAT = ipow_c(D, M);
+  // label of low row i at low index l, complemented for odd h
+  static constexpr int label(int l, int i, bool odd) {
+    int x = l;
+    for (int k = 0; k < i; ++k) x /= D;
+    const int r = x % (2 * D);
+    const int c = r < D ? r : 2 * D - 1 - r;
+    return (odd && ((D & 1) || i == M - 1)) ? D - 1 - c : c;
+  }
+  static constexpr int mask(int l, int c, bool odd) {
+    int m = 0;
+    for (int i = 0; i < M; ++i)
+      if (label(l, i, odd) == c) m |= 1 << i;
+    return m;
+  }
+};
+
+template <int D, int M, bool ODD, int L, int... Cs>
+__device__ __forceinline__ int32_t blk16_obj(const uint32_t (&T)[D][1 << M], uint32_t totU,
+                                             std::integer_sequence<int, Cs...>) {
+  using LC = LowCode<D, M>;
+  if constexpr (D == 3)
+    return (int32_t)(0u + ... + T[Cs][LC::mask(L, Cs, ODD)]);
+  else
+    return (int32_t)(totU + ... + (LC::mask(L, Cs, ODD) ? T[Cs][LC::mask(L, Cs, ODD)] : 0u));
+}
+
+template <int D, int M, bool ODD, int MODE, int L>
+__device__ __forceinline__ void blk16_eval_pair(const uint32_t (&T)[D][1 << M], uint32_t totU, int32_t &best,
+                                                uint32_t &first, uint32_t base, int32_t target) {
+  constexpr int NPAT = LowCode<D, M>::NPAT;
+  const auto cs = std::make_integer_sequence<int, D>{};
+  const int32_t a = blk16_obj<D, M, ODD, L>(T, totU, cs);
+  const int32_t b = blk16_obj<D, M, ODD, (L + 1 < NPAT ? L + 1 : L)>(T, totU, cs);
+  if constexpr (MODE == MODE_MAX) {
+    const int32_t m = a > b ? a : b;
+    best = m > best ? m : best;
+  } else {
+    const uint32_t ca = (a == target) ? base + L : 0xFFFFFFFFu;
+    const uint32_t cb = (b == target) ? base + (L + 1 < NPAT ? L + 1 : L) : 0xFFFFFFFFu;
+    const uint32_t c = ca < cb ? ca : cb;
+    first = c < first ? c : first;
+  }
+}
+
+template <int D, int M, bool ODD, int MODE, int... Ps>
+__device__ __forceinline__ void blk16_eval(const uint32_t (&T)[D][1 << M], uint32_t totU, int32_t &best,
+                                           uint32_t &first, uint32_t base, int32_t target,
+                                           std::integer_sequence<int, Ps...>) {
+  (blk16_eval_pair<D, M, ODD, MODE, 2 * Ps>(T, totU, best, first, base, target), ...);
+}
+
+// Table of label C from its sums u[C]: T[C][0] = |u_C|, and T[C][S] = F_C(S)
+// (d = 3) or F_C(S) - |u_C| (d >= 4, the accumulator starts at -|u_C|).
+template <int C, int D, int NW, int M>
+__device__ __forceinline__ void blk16_table(const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
+                                            const cuint32_t *neg) {
+  uint32_t U = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) U = sad16(u[C][w], neg[w], U);
+  T[C][0] = U;
+  const uint32_t base = D == 3 ? 0u : 0u - U;
+#pragma unroll
+  for (int S = 1; S < (1 << M); ++S) {
+    uint32_t acc = base;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) acc = sad16(u[C][w], neg[(size_t)S * NW + w], acc);
+    T[C][S] = acc;
+  }
+}
+
+template <int D, int NW, int M, int... Cs>
+__device__ __forceinline__ void blk16_tables_all(const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
+                                                 const cuint32_t *neg, std::integer_sequence<int, Cs...>) {
+  (blk16_table<Cs, D, NW, M>(u, T, neg), ...);
+}
+
+template <int C, int D, int NW, int M>
+struct Blk16Table {
+  __device__ __forceinline__ static void run(int c, const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
+                                             const cuint32_t *neg) {
+    if (c == C) blk16_table<C, D, NW, M>(u, T, neg);
+    else if constexpr (C + 1 < D) Blk16Table<C + 1, D, NW, M>::run(c, u, T, neg);
+  }
+};
+
+// Moves one packed row from label `from` to from + 1 (up) or from - 1.
+template <int C, int D, int NW>
+struct Move16 {
+  __device__ __forceinline__ static void run(uint32_t (&u)[D][NW], const cuint32_t *row, int from, int up) {
+    if (from == C) {
+      constexpr int TU = C + 1 < D ? C + 1 : C, TD = C > 0 ? C - 1 : C;
+      if (up) {
+        if constexpr (TU != C) {
+#pragma unroll
+          for (int w = 0; w < NW; ++w) {
+            u[C][w] = pk_sub(u[C][w], row[w]);
+            u[TU][w] = pk_add(u[TU][w], row[w]);
+          }
+        }
+      } else {
+        if constexpr (TD != C) {
+#pragma unroll
+          for (int w = 0; w < NW; ++w) {
+            u[C][w] = pk_sub(u[C][w], row[w]);
+            u[TD][w] = pk_add(u[TD][w], row[w]);
+          }
+        }
+      }
+    } else if constexpr (C + 1 < D) {
+      Move16<C + 1, D, NW>::run(u, row, from, up);
+    }
+  }
+};
+
+// Ap: R x NW packed rows (as walk_p16); neg: 2^M x NW packed words -A_S +
+// 2^15.  q >= M: a segment is d^(q-M) high steps of d^M labelings each.
+template <int D, int NW, int M, int MODE>
+__device__ __forceinline__ void walk_blk16(
+    const cuint32_t *Ap, const cuint32_t *neg, int R, int q, uint64_t seg0, uint64_t nseg, int32_t target,
+    int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest, unsigned long long *__restrict__ gfirst,
+    uint64_t w) {
+  constexpr int NPAT = LowCode<D, M>::NPAT;
+  const int lane = wave_lane();
+  if (((w >> 1) << 7) >= nseg) return;
+  const uint64_t local = lane_segment(w, lane);
+  const bool active = local < nseg;
+  const uint64_t seg = seg0 + local;
+  const uint32_t pseg = (uint32_t)__builtin_amdgcn_readfirstlane((int)((seg0 ^ w) & 1));
+  const int qh = q - M;   // high digits inside the segment
+
+  uint32_t par_mask;
+  if (D & 1) par_mask = pseg ? ((1u << qh) - 1u) : 0u;
+  else par_mask = (qh > 0 && pseg) ? (1u << (qh - 1)) : 0u;
+  // parity of h at the segment's first high step
+  const uint32_t hpar0 = ((D & 1) || qh == 0) ? pseg : 0u;
+
+  uint32_t u[D][NW];
+#pragma unroll
+  for (int c = 0; c < D; ++c)
+#pragma unroll
+    for (int l = 0; l < NW; ++l) u[c][l] = pk_add(kBias16x2, c == 0 ? Ap[(size_t)(R - 1) * NW + l] : 0u);
+  uint64_t x = seg;
+  for (int i = 0; i < R - 1 - M; ++i) {
+    int lab;
+    if (i < qh) {
+      lab = ((par_mask >> i) & 1) ? D - 1 : 0;
+    } else {
+      const uint32_t r = (uint32_t)(x % (uint64_t)(2 * D));
+      lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
+      x /= (uint64_t)D;
+    }
+    const cuint32_t *row = Ap + (size_t)(M + i) * NW;
+#pragma unroll
+    for (int c = 0; c < D; ++c)
+#pragma unroll
+      for (int l = 0; l < NW; ++l) u[c][l] = pk_add(u[c][l], (lab == c) ? row[l] : 0u);
+  }
+  uint32_t T[D][1 << M];
+  blk16_tables_all<D, NW, M>(u, T, neg, std::make_integer_sequence<int, D>{});
+  uint32_t totU = 0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) totU += T[c][0];
+
+  uint64_t biaspat = 0;
+  for (int i = 0; i < qh; ++i) biaspat |= (uint64_t)(16 - D) << (4 * i);
+  uint64_t cnt = biaspat;
+  uint32_t pm = par_mask;
+  uint32_t nblk = 1;
+  for (int i = 0; i < qh; ++i) nblk *= (uint32_t)D;
+
+  int32_t best = INT32_MIN;
+  uint32_t first = 0xFFFFFFFFu;
+  const auto ps = std::make_integer_sequence<int, (NPAT + 1) / 2>{};
+  for (uint32_t k = 0; k < nblk; ++k) {
+    if (k > 0) {
+      const int j = __builtin_ctzll(~cnt) >> 2;      // high digit j (row M + j) changes
+      const uint64_t low = (1ull << (4 * j)) - 1ull;
+      cnt = (cnt + 1ull) | (biaspat & low);
+      const int v = (int)((cnt >> (4 * j)) & 15ull) - (16 - D);
+      const int p = (pm >> j) & 1;
+      const int from = p ? D - v : v - 1;
+      const int up = p ? 0 : 1;
+      pm ^= (1u << j) - 1u;
+      Move16<0, D, NW>::run(u, Ap + (size_t)(M + j) * NW, from, up);
+      // One copy of the table code for both labels.  The table's scalar
+      // loads are loop-invariant; hoisted out of the loop, 2^M * NW values
+      // would not fit in SGPRs and be spilled to VGPR lanes (one v_readlane
+      // per use), so the pointer is re-laundered for every label.
+#pragma unroll 1
+      for (int e = 0; e < 2; ++e) {
+        const cuint32_t *negk = neg;
+        asm volatile("" : "+s"(negk));
+        Blk16Table<0, D, NW, M>::run(e ? (up ? from + 1 : from - 1) : from, u, T, negk);
+      }
+      totU = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) totU += T[c][0];
+    }
+    const uint32_t base = k * (uint32_t)NPAT;
+    if ((hpar0 ^ k) & 1u) blk16_eval<D, M, true, MODE>(T, totU, best, first, base, target, ps);
+    else blk16_eval<D, M, false, MODE>(T, totU, best, first, base, target, ps);
+  }
+  const uint32_t nsteps = nblk * (uint32_t)NPAT;
+
+  if constexpr (MODE == MODE_MAX) {
+    if (active) unit_best[local] = best;
+    const int32_t m = wave_max_i32(active ? best : INT32_MIN);
+    if (m > __builtin_amdgcn_readfirstlane(*(volatile int32_t *)gbest)) atomicMax(gbest, m);
+  } else {
+    const uint64_t r = (active && first != 0xFFFFFFFFu) ? (seg * (uint64_t)nsteps + first) : ~0ull;
+    const uint64_t m = wave_min_u64(r);
+    if (m != ~0ull) atomicMin(gfirst, (unsigned long long)m);
+  }
+}
 
 }  // namespace lnorm

@@ -76,3 +76,21 @@ def test_cli_marginal_workload_preprocessing_invariance(pkg, tmp_path):
         norms[flag] = int(r.stdout.strip().splitlines()[-1].split()[-1])
         assert r.stdout.strip().splitlines()[-1].startswith("LM is: ")
     assert norms["y"] == norms["n"], norms
+
+
+def test_timing_sweep_on_cli(pkg, tmp_path):
+    """The timing harness over the real L_HIP: every run of every (threads,
+    blocks) pair prints the oracle's norm, and the statistics table has one
+    row per pair."""
+    import oracle
+    from lnorm_amd import gen, timing
+    A = gen.gen_matrix(20, 20, 10, 101)
+    gen.write_matrix(str(tmp_path / "m.txt"), A)
+    want = oracle.calc_lnorm(A, 1, 0, 8)[0]   # L1 is the same for A and its transpose
+    print("timing sweep", flush=True)
+    grids = (((256, 64), (4, 16), ()), ((128,), (16,), ("n",)))   # grid files 1024 and 2048
+    res = timing.sweep(str(tmp_path / "m.txt"), 1, 2, grids, str(tmp_path / "o"), pkg.CLI, log=print)
+    assert {k: r.shape for k, r in res.items()} == {"grid_1024_order_1.txt": (2, 8), "grid_2048_order_1.txt": (1, 8)}
+    for (t, b) in ((256, 4), (64, 16), (128, 16)):
+        meas = (tmp_path / "o" / f"meas_thread_{t}_block_{b}_order_1.dat").read_text()
+        assert meas.count(f"L1 is: {want}\n") == 2, meas[-400:]

@@ -9,6 +9,10 @@ from oracle import host_oracle as ho
 
 pytestmark = pytest.mark.gpu
 
+# rows the tabulated walk keeps in its subset tables, per d (up to 64 columns;
+# 4 at d = 3 beyond 64 columns)
+TABLE_ROWS = {3: 5, 4: 4, 5: 3, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
+
 
 def _laid_out(case):
     argv = ["L", str(case["threads"]), "m.txt"] + case["args"]
@@ -43,11 +47,12 @@ def test_golden_vectors_all_kernels(pkg, golden):
         if p.R == 0:
             continue
         want_line = case["stdout"].strip().splitlines()[-1]
-        for kernel in ("auto", "plain", "general", "blocked", "plain16"):
+        for kernel in ("auto", "plain", "general", "blocked", "plain16", "table16"):
             try:
                 norm, strat, res = pkg.calc_lnorm(A, p.n, p.marginal, kernel=kernel)
             except pkg.LnormError as e:
-                assert e.code == -5 and kernel in ("plain", "blocked", "plain16"), (case["name"], kernel, str(e))
+                assert e.code == -5 and kernel in ("plain", "blocked", "plain16", "table16"), (case["name"], kernel,
+                                                                                              str(e))
                 continue
             line, fname, lines = ho.result_lines(p, norm, strat)
             assert line == want_line, (case["name"], kernel, res["kernel_name"])
@@ -73,6 +78,9 @@ def test_random_vs_oracle_plain_and_general(pkg, n, marg):
         if n >= 2 and C <= 2 * words and int(np.abs(A).sum(axis=0).max()) <= 32767:
             res = _agree(pkg, A, n, marg, kernel="plain16")
             assert res["kernel_name"].startswith("plain16")
+            if n >= 3 and R - 1 >= TABLE_ROWS[n]:
+                res = _agree(pkg, A, n, marg, kernel="table16")
+                assert res["kernel_name"].startswith("table16")
 
 
 @pytest.mark.parametrize("marg", [0, 1])
@@ -180,7 +188,7 @@ def test_segment_size_independence(pkg):
     B = _rand(rng, 9, 7, 3)
     want3 = oracle.calc_lnorm(B, 3, 0, 8)
     for s in (1, 2, 5, 8):
-        for k in ("plain", "plain16"):
+        for k in ("plain", "plain16", "table16"):
             norm, strat, res = pkg.calc_lnorm(B, 3, 0, kernel=k, seg_log=s)
             assert (norm, strat, res["best_rank"]) == want3
     want2 = oracle.calc_lnorm(A, 2, 0, 8)
@@ -233,6 +241,7 @@ def test_workloads_vs_oracle(pkg):
         print("cfg3 L2/L3", k, flush=True)
         _agree(pkg, B, 2, 0, kernel=k)
         _agree(pkg, B, 3, 0, kernel=k)
+    assert _agree(pkg, B, 3, 0)["kernel_name"].startswith("table16<")
 
 
 def test_blocked_equals_plain_large(pkg):
@@ -300,6 +309,66 @@ def test_wide16_walk_vs_oracle(pkg, d, R, cols):
         print("wide16 seg_log", d, A.shape, q, flush=True)
         res = _agree(pkg, A, d, 0, seg_log=q, kernel="wide")
         assert res["kernel_name"].startswith("wide16<") and f"q={q}>" in res["kernel_name"], res["kernel_name"]
+
+
+# (d, rows, column counts) for the tabulated walk: every instantiated width
+# (8..48 words) and odd column counts; rows leave several high digits
+TABLE16 = [
+    (3, 12, [1, 5, 16, 17, 33, 48, 64, 65, 80, 96]),
+    (4, 10, [3, 16, 31, 48, 64]),
+    (5, 8, [7, 16, 33, 48, 64]),
+    (6, 7, [9, 16, 32, 48]),
+    (7, 6, [5, 16, 32]),
+    (8, 6, [5, 16, 32]),
+    (9, 5, [5, 16, 32]),
+    (10, 5, [5, 16, 32]),
+]
+
+
+@pytest.mark.parametrize("d,R,cols", TABLE16)
+def test_table16_walk_vs_oracle(pkg, d, R, cols):
+    """The tabulated packed walk (the M fastest rows as per-label subset
+    tables): bit-exact norm, strategy and rank at every width, at forced
+    segment lengths q = M .. M+2 (one to d^2 high steps per segment), and
+    with exactly M enumerated rows (a single block)."""
+    rng = np.random.default_rng(900 + d)
+    M = TABLE_ROWS[d]
+    for C in cols:
+        A = _rand(rng, R, C, 60)
+        print("table16", d, A.shape, flush=True)
+        res = _agree(pkg, A, d, 0)
+        assert res["kernel_name"].startswith("table16<"), res["kernel_name"]
+    A = _rand(rng, R, cols[1], 60)
+    for q in (M, M + 1, M + 2):
+        if q > R - 1:
+            continue
+        print("table16 seg_log", d, A.shape, q, flush=True)
+        res = _agree(pkg, A, d, 0, seg_log=q, kernel="table16")
+        assert f",q={q}>" in res["kernel_name"], res["kernel_name"]
+    A = _rand(rng, M + 1, 9, 60)
+    res = _agree(pkg, A, d, 0, kernel="table16")
+    assert res["kernel_name"].startswith("table16<"), res["kernel_name"]
+    # fewer enumerated rows than table rows: AUTO takes the packed walk
+    A = _rand(rng, M, 9, 60)
+    assert not _agree(pkg, A, d, 0)["kernel_name"].startswith("table16<")
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm(A, d, 0, kernel="table16")
+    assert e.value.code == -5
+
+
+def test_table16_equals_plain16_larger(pkg):
+    """At sizes the oracle does not reach: the tabulated walk and the packed
+    walk agree on norm, strategy and rank (3^23 labelings for the 24 x 32 L3
+    workload), and the objective of the strategy is the norm."""
+    from lnorm_amd import gen
+    for (R, C, d) in ((24, 32, 3), (16, 64, 3), (13, 40, 4), (11, 24, 6), (9, 16, 10)):
+        A = gen.gen_matrix(R, C, 50, 40 + d)
+        print("table16 vs plain16", R, C, d, flush=True)
+        t = pkg.calc_lnorm(A, d, 0)
+        p = pkg.calc_lnorm(A, d, 0, kernel="plain16")
+        assert t[2]["kernel_name"].startswith("table16<"), t[2]["kernel_name"]
+        assert t[:2] == p[:2] and t[2]["best_rank"] == p[2]["best_rank"], (R, C, d)
+        assert oracle.objective(A, d, 0, t[1]) == t[0]
 
 
 def test_wide16_equals_general_larger(pkg):

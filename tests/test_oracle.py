@@ -311,3 +311,93 @@ def test_dary_block_order(D):
                 diff = [i for i in range(rows) if cur[i] != prev[i]]
                 assert diff == [j] and (prev[j], cur[j]) == (f, t), (D, q, seg, tt)
                 prev = cur
+
+
+def _low_label(D, M, l, i, odd):
+    """walk_blk16's low labels: rank l's digit i, complemented for odd h
+    (every low digit for odd d, the top one for even d)."""
+    x = l // D ** i
+    r = x % (2 * D)
+    c = r if r < D else 2 * D - 1 - r
+    return D - 1 - c if (odd and (D % 2 or i == M - 1)) else c
+
+
+def _tabulated_walk(D, M, q, seg, R):
+    """Restatement of walk_blk16's enumeration order for one segment of
+    d^q ranks (q >= M): yields (offset, labels of rows 0..R-2)."""
+    qh = q - M
+    ps = seg & 1
+    pm = (((1 << qh) - 1) if ps else 0) if D % 2 else ((1 << (qh - 1)) if (qh > 0 and ps) else 0)
+    hpar0 = ps if (D % 2 or qh == 0) else 0
+    high = []
+    x = seg
+    for i in range(R - 1 - M):
+        if i < qh:
+            high.append(D - 1 if (pm >> i) & 1 else 0)
+        else:
+            r = x % (2 * D)
+            high.append(r if r < D else 2 * D - 1 - r)
+            x //= D
+    biaspat = 0
+    for i in range(qh):
+        biaspat |= (16 - D) << (4 * i)
+    cnt = biaspat
+    for k in range(D ** qh):
+        if k > 0:
+            inv = ~cnt & ((1 << 64) - 1)
+            j = ((inv & -inv).bit_length() - 1) >> 2
+            cnt = ((cnt + 1) | (biaspat & ((1 << (4 * j)) - 1))) & ((1 << 64) - 1)
+            v = ((cnt >> (4 * j)) & 15) - (16 - D)
+            p = (pm >> j) & 1
+            f = D - v if p else v - 1
+            pm ^= (1 << j) - 1
+            assert high[j] == f
+            high[j] = f - 1 if p else f + 1
+        odd = (hpar0 ^ k) & 1
+        for l in range(D ** M):
+            yield k * D ** M + l, [_low_label(D, M, l, i, odd) for i in range(M)] + list(high)
+
+
+@pytest.mark.parametrize("D,M", [(3, 5), (3, 4), (4, 4), (5, 3), (6, 3), (7, 2), (8, 2), (9, 2), (10, 2)])
+def test_tabulated_walk_order(D, M):
+    """The tabulated walk's split of the Gray rank N = h * d^M + l: the low
+    labels are those of rank l, complemented when h is odd (all of them for
+    odd d, the top one for even d), and the high rows follow the reflected
+    code of h.  Every rank of several aligned segments is compared with the
+    oracle's digits (L_OpenMP.c:140, 204)."""
+    R = M + 4
+    for q in sorted({M, M + 1, M + 2} if D ** (M + 2) <= 20000 else {M, M + 1}):
+        for seg in (0, 1, 2, 3, 5):
+            if (seg + 1) * D ** q > D ** (R - 1):
+                continue
+            N0 = seg * D ** q
+            for off, labels in _tabulated_walk(D, M, q, seg, R):
+                want = [oracle.digit(N0 + off, i, D) for i in range(R - 1)]
+                assert labels == want, (D, M, q, seg, off)
+
+
+@pytest.mark.parametrize("D,M", [(3, 5), (4, 4), (5, 3), (7, 2), (10, 2)])
+def test_tabulated_objective(D, M):
+    """sum_c F_c(S_c) with F_c(S) = sum_l |u_c,l + sum_{i in S} A_i,l| (u_c
+    over the rows outside the table, the last row in label 0) equals the
+    objective; the d >= 4 form TotU + sum over non-empty S_c of
+    F_c(S_c) - F_c({}) too.  The subset table holds -A_S + 2^15 per 16-bit
+    half, so |u + A_S| is one unsigned 16-bit absolute difference."""
+    rng = np.random.default_rng(40 + D)
+    R, C = M + 4, 7
+    A = rng.integers(-90, 91, size=(R, C)).astype(np.int64)
+    for _ in range(30):
+        labels = list(rng.integers(0, D, size=R - 1)) + [0]
+        u = np.zeros((D, C), np.int64)
+        for i in range(M, R):
+            u[labels[i]] += A[i]
+        masks = [sum(1 << i for i in range(M) if labels[i] == c) for c in range(D)]
+
+        def F(c, S):
+            AS = sum((A[i] for i in range(M) if (S >> i) & 1), np.zeros(C, np.int64))
+            neg = (0x8000 - AS) & 0xFFFF
+            return int(np.abs((u[c] + 0x8000) - neg).sum())
+        want = oracle.objective(A.astype(np.int32), D, 0, [int(v) for v in labels[:-1]])
+        assert sum(F(c, masks[c]) for c in range(D)) == want
+        totU = sum(F(c, 0) for c in range(D))
+        assert totU + sum(F(c, masks[c]) - F(c, 0) for c in range(D) if masks[c]) == want
