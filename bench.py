@@ -238,8 +238,9 @@ def cli_line(pkg, gen) -> dict:
 
 def l3_line(pkg, gen, steps: int, warmup: int, cpu: bool) -> dict:
     """L_3 of a 24 x 32 file (entries in [-50, 50], seed 103): 3^23 = 9.4e10
-    labelings on the packed 16-bit d-ary walk, beside the 32-bit d-ary walk and
-    the reference L_OpenMP on a 20-row sample of the same file (3^19)."""
+    labelings on the tabulated packed walk, beside the packed walk without
+    tables, the 32-bit d-ary walk and the reference L_OpenMP on a 20-row
+    sample of the same file (3^19)."""
     B = gen.gen_matrix(24, 32, 50, 103)
     total = 3 ** 23
     kw = {"autotune": 1 if AUTOTUNE else 0}
@@ -259,16 +260,16 @@ def l3_line(pkg, gen, steps: int, warmup: int, cpu: bool) -> dict:
     wall = (time.perf_counter() - t0) / max(1, steps)
     km = sum(kms) / len(kms)
     n2, s2, r2 = pkg.calc_lnorm(B, 3, 0, kernel="plain")
-    if (n2, s2, r2["best_rank"]) != out:
-        raise RuntimeError("the packed and 32-bit d-ary walks disagree")
-    # op model: a move costs a v_pk_sub, a v_pk_add and two v_sad_u16 per two
-    # columns (NW = 16 words), plus the label sums and the maximum.  The
-    # compiler needs fewer inside the unrolled row-0 runs (a label that gains
-    # and then loses the row keeps its old register), so the hardware issue
-    # rate (rocprofv3 SQ_INSTS_VALU) is below this model fraction.
-    ops = 4 * 16 + 3
+    n3, s3, r3 = pkg.calc_lnorm(B, 3, 0, kernel="plain16")
+    if (n2, s2, r2["best_rank"]) != out or (n3, s3, r3["best_rank"]) != out:
+        raise RuntimeError("the tabulated, packed and 32-bit d-ary walks disagree")
+    # op model of the tabulated walk at 16 words (M = 5): 1.5 per labeling
+    # for the sums and the maximum, plus (2 + 64) * 16 per block of 243
+    ops = table16_ops(3, 16)
     line = {"value": total / wall, "unit": "labelings/s", "ms_per_step": 1e3 * wall, "kernel_ms": km,
-            "kernel": res["kernel_name"], "valu_frac_model": total * ops / (km / 1e3) / VALU_PEAK,
+            "kernel": res["kernel_name"], "valu_ops_per_labeling_model": ops,
+            "valu_frac_model": total * ops / (km / 1e3) / VALU_PEAK,
+            "plain16": {"value": total / (r3["ms_kernel"] / 1e3), "kernel": r3["kernel_name"]},
             "plain32": {"value": total / (r2["ms_total"] / 1e3), "kernel": r2["kernel_name"]},
             "norm": out[0]}
     if cpu:
@@ -280,17 +281,41 @@ def l3_line(pkg, gen, steps: int, warmup: int, cpu: bool) -> dict:
     return line
 
 
-def batch_valu_ops(d: int, cols: int) -> int:
+# rows the tabulated walk keeps in its subset tables (its width limit per d
+# in 16-bit words; d = 3 keeps 4 rows beyond 32 words)
+TABLE_ROWS = {3: 5, 4: 4, 5: 3, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
+TABLE_MAX_WORDS = {3: 48, 4: 32, 5: 32, 6: 24, 7: 16, 8: 16, 9: 16, 10: 16}
+
+
+def table16_rows(d: int, words: int) -> int:
+    return 4 if (d == 3 and words > 32) else TABLE_ROWS[d]
+
+
+def table16_ops(d: int, words: int) -> float:
+    """VALU lane-operations per labeling of the tabulated walk: per block of
+    d^M labelings, the high move (a v_pk_sub and a v_pk_add per word) and the
+    two changed labels' subset tables (2^M v_sad_u16 per word each), plus per
+    labeling one sum of table entries and half a v_max3 (d = 3: one v_add3,
+    1.5 in all; d >= 4 the compiler shares partial sums between labelings,
+    and 1.5 is an upper bound of what it issues)."""
+    M = table16_rows(d, words)
+    return 1.5 + (2 + 2 ** (M + 1)) * words / d ** M
+
+
+def batch_valu_ops(d: int, rows: int, cols: int) -> float:
     """VALU lane-operations per labeling of the batched walks by the op model,
     on the item's real width: L1/LM one add and one |.|-accumulate per column
-    plus the maximum; d >= 2 (packed 16-bit walk) a v_pk_sub, a v_pk_add and
-    two v_sad_u16 per two columns (the two label vectors that exchange a
-    row), d - 1 adds of the label sums and the maximum.  The unrolled row-0
-    runs of the d-ary walk need fewer, so for d >= 2 this is an upper bound of
-    the issued work."""
+    plus the maximum; d >= 3 with at least M + 1 rows the tabulated walk
+    (table16_ops); other d >= 2 items the packed 16-bit walk: a v_pk_sub, a
+    v_pk_add and two v_sad_u16 per two columns (the two label vectors that
+    exchange a row), d - 1 adds of the label sums and the maximum (an upper
+    bound: the unrolled row-0 runs need fewer)."""
     if d == 1:
         return 2 * cols + 1
-    return 4 * ((cols + 1) // 2) + d
+    words = (cols + 1) // 2
+    if d >= 3 and words <= TABLE_MAX_WORDS[d] and rows - 1 >= table16_rows(d, words):
+        return table16_ops(d, words)
+    return 4 * words + d
 
 
 def batch_line(pkg, gen, steps: int) -> dict:
@@ -311,7 +336,7 @@ def batch_line(pkg, gen, steps: int) -> dict:
         probs.append((p.matrix_laid_out(), p.n, p.marginal))
         per_d[p.n] = per_d.get(p.n, 0) + p.total
         row_updates += p.total * p.C
-        ops += p.total * batch_valu_ops(p.n, p.C)
+        ops += p.total * batch_valu_ops(p.n, p.R, p.C)
         host.free(p)
     eager, st0 = pkg.calc_batch(probs, stats=True)
     if any(s != 0 for _, _, s in eager):

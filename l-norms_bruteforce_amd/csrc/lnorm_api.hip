@@ -916,14 +916,15 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
 
 // ---- batches ------------------------------------------------------------------
 // Items that fit a batched walk (L1/LM with at most 128 columns; d >= 2 with
-// column sums <= 2^15 - 1, packed) run in one persistent launch per
-// instantiation; the rest go through calc_impl one by one.
+// column sums <= 2^15 - 1, packed, and tabulated for d >= 3 where the width
+// and row count allow) run in one persistent launch per instantiation; the
+// rest go through calc_impl one by one.
 struct BItem {
   int idx;             // position in the caller's array
   Problem p;
   Prepared pp;
   Geometry g1, g2;     // phase-1 segments; phase-2 sub-segments of one unit
-  double work;         // labelings x columns: longest first within a group
+  double work;         // labelings x walk_cost: longest first within a group
   size_t a_off = 0, d_off = 0;
   uint64_t ub_off = 0;
 };
@@ -949,13 +950,26 @@ constexpr uint64_t kBatchMaxTotal = 1ull << 32;
 // a short tail at the end of each group).
 constexpr int kBatchCapLog = 14;
 constexpr int kBatchStreams = 4;
-Geometry batch_geometry(int base, int e, int forced, Walk walk, int cap_log) {
-  if (forced > 0) return geometry_for(base, e, 63, forced, walk);
+Geometry batch_geometry(int base, int e, int forced, Walk walk, int cap_log, int qmin = 0) {
+  if (forced > 0) return geometry_for(base, e, 63, forced, walk, qmin);
   const double lb = std::log2((double)base);
-  Geometry g = geometry_for(base, e, 7, 0, walk);
+  Geometry g = geometry_for(base, e, 7, 0, walk, qmin);
   const int qcap = std::max(0, (int)std::floor(cap_log / lb));
-  if (g.q > qcap) g = geometry_for(base, e, 63, qcap, walk);
+  if (g.q > qcap) g = geometry_for(base, e, 63, qcap, walk, qmin);
   return g;
+}
+
+// Relative VALU work per labeling of a batch walk, for balancing groups over
+// streams: the binary walk 2 per column, the packed walk 4 per word plus d,
+// the tabulated walk 1.5 plus its table sums 2 + 2^(M+1) per word over d^M.
+double walk_cost(Walk walk, int d, int width) {
+  if (walk == WALK_PLAIN) return 2.0 * width;
+  const int nw = width / 2;
+  if (walk == WALK_TAB16) {
+    const int M = blk16_rows(d, nw);
+    return 1.5 + (2.0 + (double)(2 << M)) * nw / std::pow((double)d, (double)M);
+  }
+  return 4.0 * nw + d;
 }
 
 // Batch shape for the tuner's cache: FNV-1a over the sorted item sizes.
@@ -1065,10 +1079,18 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
     Walk walk;
     int width;
     if (b.p.n >= 2) {
-      const int w16 = max_col_abs_sum(b.p, 0) <= 32767 ? plain16_words(b.p.n, b.p.C) : 0;
-      if (!w16) { fallback.push_back(i); continue; }
-      walk = WALK_PLAIN16;
-      width = 2 * w16;
+      const bool fits16 = max_col_abs_sum(b.p, 0) <= 32767;
+      const int w16 = fits16 ? plain16_words(b.p.n, b.p.C) : 0;
+      int wtab = (fits16 && b.p.n >= 3) ? blk16_words(b.p.n, b.p.C) : 0;
+      if (wtab && b.p.R - 1 < blk16_rows(b.p.n, wtab)) wtab = 0;
+      if (wtab) {
+        walk = WALK_TAB16;
+        width = 2 * wtab;
+      } else {
+        if (!w16) { fallback.push_back(i); continue; }
+        walk = WALK_PLAIN16;
+        width = 2 * w16;
+      }
     } else {
       width = plain_ny_pad(b.p.kind, 1, b.p.C);
       if (!width) { fallback.push_back(i); continue; }
@@ -1076,9 +1098,9 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
     }
     if (b.p.total > kBatchMaxTotal) { fallback.push_back(i); continue; }
     prepare_walk(b.p, walk, width, b.pp);
-    b.g1 = batch_geometry(b.p.base, b.p.R - 1, o.seg_log, walk, cap_log);
-    b.g2 = geometry_for(b.p.base, b.g1.q, 8, 0, walk);
-    b.work = (double)b.p.total * b.p.C;
+    b.g1 = batch_geometry(b.p.base, b.p.R - 1, o.seg_log, walk, cap_log, b.pp.qmin);
+    b.g2 = geometry_for(b.p.base, b.g1.q, 8, 0, walk, b.pp.qmin);
+    b.work = (double)b.p.total * walk_cost(walk, b.p.n, width);
     bis.push_back(std::move(b));
   }
 
@@ -1206,7 +1228,8 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
         const BGroup &g = groups[gi];
         BatchLaunch bl{tasks + g.first, (int)g.members.size(), ph == 0 ? g.ntasks1 : g.ntasks2,
                        cnt + (size_t)(2 * gi + ph) * kCounterShards * kCounterStride,
-                       g.kind, g.d, g.width, g.walk == WALK_PLAIN16};
+                       g.kind, g.d, g.width, g.walk == WALK_PLAIN16 || g.walk == WALK_TAB16,
+                       g.walk == WALK_TAB16};
         HIPCHK(launch_batch_walk(mode, bl, lane_stream(glane[gi])));
       }
       for (int l = 1; l < kLanes; ++l) {

@@ -96,6 +96,31 @@ __global__ __launch_bounds__(256) void k_batch_p16(const BatchTask *__restrict__
   }
 }
 
+template <int D, int NW, int MODE>
+__global__ __launch_bounds__(256) void k_batch_t16(const BatchTask *__restrict__ tasks, int nitems,
+                                                   uint64_t ntasks, unsigned int *__restrict__ counter) {
+  int shard = (int)(blockIdx.x & (kCounterShards - 1)), left = kCounterShards;
+  uint32_t t;
+  while (next_task(counter, ntasks, shard, left, t)) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tasks[mid].task0 <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchTask &it = tasks[lo];
+    uint64_t seg0 = 0;
+    int32_t target = 0;
+    if (MODE == MODE_FIRST) {
+      seg0 = *it.unit * it.seg_base;
+      target = *it.gbest;
+    }
+    walk_blk16<D, NW, blk16_low_rows(D, NW), MODE>(as_const((const uint32_t *)it.A), as_const((const uint32_t *)it.D),
+                                                    it.R, it.q, seg0, it.nseg, target, it.unit_best, it.gbest,
+                                                    it.gfirst, t - it.task0);
+  }
+}
+
 // One workgroup per item: the smallest unit index whose best equals the
 // item's maximum.
 __global__ __launch_bounds__(256) void k_batch_first_unit(const BatchTask *__restrict__ tasks,
@@ -158,6 +183,28 @@ static hipError_t lb_p16(Mode mode, const BatchLaunch &b, hipStream_t st) {
   return hipGetLastError();
 }
 
+template <int D, int NW>
+static hipError_t lb_t16(Mode mode, const BatchLaunch &b, hipStream_t st) {
+  const dim3 grid(batch_grid(b.ntasks)), blk(256);
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_batch_t16<D, NW, MODE_MAX>), grid, blk, 0, st, b.tasks, b.nitems, b.ntasks, b.counter);
+  else
+    hipLaunchKernelGGL((k_batch_t16<D, NW, MODE_FIRST>), grid, blk, 0, st, b.tasks, b.nitems, b.ntasks, b.counter);
+  return hipGetLastError();
+}
+
+template <int D>
+static hipError_t lb_t16_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
+  if (blk16_words(D, b.width) * 2 != b.width) return hipErrorInvalidValue;
+  switch (b.width / 2) {
+#define X(n) case n: if constexpr (n <= dary16_max_words(D)) return lb_t16<D, n>(mode, b, st); break;
+    X(8) X(16) X(24) X(32) X(40) X(48)
+#undef X
+    default: break;
+  }
+  return hipErrorInvalidValue;
+}
+
 template <int KIND>
 static hipError_t lb_bin_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
   switch (b.width) {
@@ -185,6 +232,19 @@ hipError_t launch_batch_walk(Mode mode, const BatchLaunch &b, hipStream_t st) {
     if (b.kind == KIND_L1) return lb_bin_w<KIND_L1>(mode, b, st);
     if (b.kind == KIND_LM) return lb_bin_w<KIND_LM>(mode, b, st);
     return hipErrorInvalidValue;
+  }
+  if (b.table) {
+    switch (b.d) {
+      case 3: return lb_t16_w<3>(mode, b, st);
+      case 4: return lb_t16_w<4>(mode, b, st);
+      case 5: return lb_t16_w<5>(mode, b, st);
+      case 6: return lb_t16_w<6>(mode, b, st);
+      case 7: return lb_t16_w<7>(mode, b, st);
+      case 8: return lb_t16_w<8>(mode, b, st);
+      case 9: return lb_t16_w<9>(mode, b, st);
+      case 10: return lb_t16_w<10>(mode, b, st);
+      default: return hipErrorInvalidValue;
+    }
   }
   switch (b.d) {
     case 2: return lb_p16_w<2>(mode, b, st);

@@ -91,7 +91,7 @@ constexpr int dary16_max_words(int d) {
 // batch kernels).  Tasks [task0, task0 + 2*ceil(nseg/128)) are its waves.
 struct BatchTask {
   const int32_t *A;            // binary walk: R x NY; packed walk: R x NW words
-  const int32_t *D;            // binary walk: [2][R-1][NY] deltas
+  const int32_t *D;            // binary walk: [2][R-1][NY] deltas; tabulated walk: 2^M x NW subset table
   int R, q;                    // rows; segment = base^q ranks
   uint64_t nseg;               // segments of this item in this phase
   uint64_t seg_base;           // MODE_FIRST: segments per unit (seg0 = *unit * seg_base)
@@ -118,6 +118,7 @@ struct BatchLaunch {
   int d;
   int width;                   // binary: NY; packed: 2 * NW
   bool packed;
+  bool table;                  // packed and tabulated (walk_blk16)
 };
 
 hipError_t launch_batch_walk(Mode mode, const BatchLaunch &b, hipStream_t st);

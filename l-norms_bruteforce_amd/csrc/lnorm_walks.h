@@ -554,29 +554,32 @@ __device__ __forceinline__ int32_t blk16_obj(const uint32_t (&T)[D][1 << M], uin
     return (int32_t)(totU + ... + (LC::mask(L, Cs, ODD) ? T[Cs][LC::mask(L, Cs, ODD)] : 0u));
 }
 
+// MODE_MAX: two labelings and the running maximum meet in one v_max3.
+// MODE_FIRST: fb keeps the smallest low index l in the block whose objective
+// equals the target (l is a literal).
 template <int D, int M, bool ODD, int MODE, int L>
 __device__ __forceinline__ void blk16_eval_pair(const uint32_t (&T)[D][1 << M], uint32_t totU, int32_t &best,
-                                                uint32_t &first, uint32_t base, int32_t target) {
+                                                uint32_t &fb, int32_t target) {
   constexpr int NPAT = LowCode<D, M>::NPAT;
+  constexpr int L2 = L + 1 < NPAT ? L + 1 : L;
   const auto cs = std::make_integer_sequence<int, D>{};
   const int32_t a = blk16_obj<D, M, ODD, L>(T, totU, cs);
-  const int32_t b = blk16_obj<D, M, ODD, (L + 1 < NPAT ? L + 1 : L)>(T, totU, cs);
+  const int32_t b = blk16_obj<D, M, ODD, L2>(T, totU, cs);
   if constexpr (MODE == MODE_MAX) {
     const int32_t m = a > b ? a : b;
     best = m > best ? m : best;
   } else {
-    const uint32_t ca = (a == target) ? base + L : 0xFFFFFFFFu;
-    const uint32_t cb = (b == target) ? base + (L + 1 < NPAT ? L + 1 : L) : 0xFFFFFFFFu;
+    const uint32_t ca = (a == target) ? (uint32_t)L : 0xFFFFFFFFu;
+    const uint32_t cb = (b == target) ? (uint32_t)L2 : 0xFFFFFFFFu;
     const uint32_t c = ca < cb ? ca : cb;
-    first = c < first ? c : first;
+    fb = c < fb ? c : fb;
   }
 }
 
 template <int D, int M, bool ODD, int MODE, int... Ps>
 __device__ __forceinline__ void blk16_eval(const uint32_t (&T)[D][1 << M], uint32_t totU, int32_t &best,
-                                           uint32_t &first, uint32_t base, int32_t target,
-                                           std::integer_sequence<int, Ps...>) {
-  (blk16_eval_pair<D, M, ODD, MODE, 2 * Ps>(T, totU, best, first, base, target), ...);
+                                           uint32_t &fb, int32_t target, std::integer_sequence<int, Ps...>) {
+  (blk16_eval_pair<D, M, ODD, MODE, 2 * Ps>(T, totU, best, fb, target), ...);
 }
 
 // Table of label C from its sums u[C]: T[C][0] = |u_C|, and T[C][S] = F_C(S)
@@ -726,9 +729,31 @@ __device__ __forceinline__ void walk_blk16(
 #pragma unroll
       for (int c = 0; c < D; ++c) totU += T[c][0];
     }
-    const uint32_t base = k * (uint32_t)NPAT;
-    if ((hpar0 ^ k) & 1u) blk16_eval<D, M, true, MODE>(T, totU, best, first, base, target, ps);
-    else blk16_eval<D, M, false, MODE>(T, totU, best, first, base, target, ps);
+    if constexpr (MODE == MODE_MAX) {
+      // A block holds all d^M low labelings whatever the parity of h; only
+      // their order differs, and the maximum does not depend on it.  (Both
+      // orders in one loop also let the compiler merge their identical sums
+      // into one set of d^M live values, which spilled.)
+      uint32_t unused = 0;
+      blk16_eval<D, M, false, MODE>(T, totU, best, unused, target, ps);
+    } else {
+      uint32_t fb = 0xFFFFFFFFu;
+      if ((hpar0 ^ k) & 1u) {
+        // The two orders sum the same table entries; an identity barrier on
+        // the table keeps the compiler from hoisting all d^M sums out of
+        // both branches as live values.
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+#pragma unroll
+          for (int S = 0; S < (1 << M); ++S) asm volatile("" : "+v"(T[c][S]));
+        asm volatile("" : "+v"(totU));
+        blk16_eval<D, M, true, MODE>(T, totU, best, fb, target, ps);
+      } else {
+        blk16_eval<D, M, false, MODE>(T, totU, best, fb, target, ps);
+      }
+      const uint32_t c = fb != 0xFFFFFFFFu ? k * (uint32_t)NPAT + fb : 0xFFFFFFFFu;
+      first = c < first ? c : first;
+    }
   }
   const uint32_t nsteps = nblk * (uint32_t)NPAT;
 
