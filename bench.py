@@ -283,19 +283,19 @@ def l3_line(pkg, gen, steps: int, warmup: int, cpu: bool) -> dict:
 
 # rows the tabulated walk keeps in its subset tables (its width limit per d
 # in 16-bit words; d = 3 keeps 4 rows beyond 32 words)
-TABLE_ROWS = {3: 5, 4: 4, 5: 3, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
-TABLE_MAX_WORDS = {3: 48, 4: 32, 5: 32, 6: 24, 7: 16, 8: 16, 9: 16, 10: 16}
+TABLE_ROWS = {2: 5, 3: 5, 4: 4, 5: 3, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
+TABLE_MAX_WORDS = {2: 64, 3: 48, 4: 32, 5: 32, 6: 24, 7: 16, 8: 16, 9: 16, 10: 16}
 
 
 def table16_rows(d: int, words: int) -> int:
-    return 4 if (d == 3 and words > 32) else TABLE_ROWS[d]
+    return 4 if (d <= 3 and words > 32) else TABLE_ROWS[d]
 
 
 def table16_ops(d: int, words: int) -> float:
     """VALU lane-operations per labeling of the tabulated walk: per block of
     d^M labelings, the high move (a v_pk_sub and a v_pk_add per word) and the
     two changed labels' subset tables (2^M v_sad_u16 per word each), plus per
-    labeling one sum of table entries and half a v_max3 (d = 3: one v_add3,
+    labeling one sum of table entries and half a v_max3 (d = 2, 3: one add,
     1.5 in all; d >= 4 the compiler shares partial sums between labelings,
     and 1.5 is an upper bound of what it issues)."""
     M = table16_rows(d, words)
@@ -305,7 +305,7 @@ def table16_ops(d: int, words: int) -> float:
 def batch_valu_ops(d: int, rows: int, cols: int) -> float:
     """VALU lane-operations per labeling of the batched walks by the op model,
     on the item's real width: L1/LM one add and one |.|-accumulate per column
-    plus the maximum; d >= 3 with at least M + 1 rows the tabulated walk
+    plus the maximum; d >= 2 with at least M + 1 rows the tabulated walk
     (table16_ops); other d >= 2 items the packed 16-bit walk: a v_pk_sub, a
     v_pk_add and two v_sad_u16 per two columns (the two label vectors that
     exchange a row), d - 1 adds of the label sums and the maximum (an upper
@@ -313,7 +313,7 @@ def batch_valu_ops(d: int, rows: int, cols: int) -> float:
     if d == 1:
         return 2 * cols + 1
     words = (cols + 1) // 2
-    if d >= 3 and words <= TABLE_MAX_WORDS[d] and rows - 1 >= table16_rows(d, words):
+    if words <= TABLE_MAX_WORDS[d] and rows - 1 >= table16_rows(d, words):
         return table16_ops(d, words)
     return 4 * words + d
 

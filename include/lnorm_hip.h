@@ -34,7 +34,7 @@ extern "C" {
 #define LNORM_E_UNSUPPORTED (-5) /* requested kernel cannot run this problem */
 
 /* kernel selection */
-#define LNORM_KERNEL_AUTO 0     /* blocked (n = 1), table16 (n >= 3), packed plain16 (n >= 2), else plain,
+#define LNORM_KERNEL_AUTO 0     /* blocked (n = 1), table16 (n >= 2), packed plain16 (n >= 2), else plain,
                                    wide, general */
 #define LNORM_KERNEL_PLAIN 1    /* register-resident Gray walk, lane per segment */
 #define LNORM_KERNEL_BLOCKED 2  /* L1/LM: MFMA-built low-row table + scalar high walk (16- or 32-bit words) */
@@ -42,8 +42,9 @@ extern "C" {
 #define LNORM_KERNEL_PLAIN16 4  /* n >= 2: packed 16-bit Gray walk (column sums <= 32767) */
 #define LNORM_KERNEL_WIDE 5     /* W waves split the columns: n <= 2 at 129..1024 columns (L2: ..512);
                                    n = 3..10 packed, column sums <= 32767, up to 640 (d = 3) .. 192 (d >= 9) */
-#define LNORM_KERNEL_TABLE16 6  /* n = 3..10: packed walk over the high rows; the 2..5 fastest rows are
-                                   tabulated per label (column sums <= 32767; up to 96 columns at d = 3) */
+#define LNORM_KERNEL_TABLE16 6  /* n = 2..10: packed walk over the high rows; the 2..5 fastest rows are
+                                   tabulated per label (column sums <= 32767; up to 128 columns at d = 2,
+                                   96 at d = 3) */
 
 typedef struct lnorm_hip_opts {
     int32_t kernel;             /* LNORM_KERNEL_* */

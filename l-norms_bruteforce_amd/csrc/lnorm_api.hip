@@ -639,7 +639,7 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
   // the W-wave walk combines steps in groups of 8: at least 3 enumerated rows
   c.ny_wide = (p.R - 1 >= 3) ? wide_ny_pad(p.kind, p.C) : 0;
   c.w16_wide = (fits16 && p.n >= 3) ? wide16_words(p.n, p.C) : 0;
-  c.wtab = (fits16 && p.n >= 3) ? blk16_words(p.n, p.C) : 0;
+  c.wtab = (fits16 && p.n >= 2) ? blk16_words(p.n, p.C) : 0;
   if (c.wtab && p.R - 1 < blk16_rows(p.n, c.wtab)) c.wtab = 0;   // fewer enumerated rows than table rows
   if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
     return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
@@ -649,7 +649,7 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
     return fail(LNORM_E_UNSUPPORTED, "W-wave walk needs an instantiated width (%d columns at n = %d) and, for "
                 "n <= 2, at least 3 enumerated rows, for n >= 3, column sums <= 32767", p.C, p.n);
   if (kernel == LNORM_KERNEL_TABLE16 && !c.wtab)
-    return fail(LNORM_E_UNSUPPORTED, "tabulated packed walk needs n = 3..10, column sums <= 32767, an "
+    return fail(LNORM_E_UNSUPPORTED, "tabulated packed walk needs n = 2..10, column sums <= 32767, an "
                 "instantiated width and more enumerated rows than table rows (%d x %d at n = %d)", p.R, p.C, p.n);
   if (kernel == LNORM_KERNEL_PLAIN16 && !c.w16)
     return fail(LNORM_E_UNSUPPORTED, "packed 16-bit walk needs n >= 2, column sums <= 32767 and an instantiated "
@@ -916,7 +916,7 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
 
 // ---- batches ------------------------------------------------------------------
 // Items that fit a batched walk (L1/LM with at most 128 columns; d >= 2 with
-// column sums <= 2^15 - 1, packed, and tabulated for d >= 3 where the width
+// column sums <= 2^15 - 1, packed, and tabulated where the width
 // and row count allow) run in one persistent launch per instantiation; the
 // rest go through calc_impl one by one.
 struct BItem {
@@ -1081,7 +1081,7 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
     if (b.p.n >= 2) {
       const bool fits16 = max_col_abs_sum(b.p, 0) <= 32767;
       const int w16 = fits16 ? plain16_words(b.p.n, b.p.C) : 0;
-      int wtab = (fits16 && b.p.n >= 3) ? blk16_words(b.p.n, b.p.C) : 0;
+      int wtab = fits16 ? blk16_words(b.p.n, b.p.C) : 0;
       if (wtab && b.p.R - 1 < blk16_rows(b.p.n, wtab)) wtab = 0;
       if (wtab) {
         walk = WALK_TAB16;

@@ -198,7 +198,7 @@ static hipError_t lb_t16_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
   if (blk16_words(D, b.width) * 2 != b.width) return hipErrorInvalidValue;
   switch (b.width / 2) {
 #define X(n) case n: if constexpr (n <= dary16_max_words(D)) return lb_t16<D, n>(mode, b, st); break;
-    X(8) X(16) X(24) X(32) X(40) X(48)
+    X(8) X(16) X(24) X(32) X(40) X(48) X(64)
 #undef X
     default: break;
   }
@@ -235,6 +235,7 @@ hipError_t launch_batch_walk(Mode mode, const BatchLaunch &b, hipStream_t st) {
   }
   if (b.table) {
     switch (b.d) {
+      case 2: return lb_t16_w<2>(mode, b, st);
       case 3: return lb_t16_w<3>(mode, b, st);
       case 4: return lb_t16_w<4>(mode, b, st);
       case 5: return lb_t16_w<5>(mode, b, st);

@@ -1,4 +1,4 @@
-// lnorm_dary.hip -- the blocked packed d-ary walk (d = 3..10), one lane per
+// lnorm_dary.hip -- the blocked packed d-ary walk (d = 2..10), one lane per
 // segment: the M fastest rows are tabulated per label (walk_blk16).
 #include "lnorm_walks.h"
 
@@ -13,12 +13,12 @@ __global__ __launch_bounds__(256) void k_blk16_dary(
                                                   gbest, gfirst, global_wave());
 }
 
-static const int kBlkWords[] = {8, 16, 24, 32, 40, 48};
+static const int kBlkWords[] = {8, 16, 24, 32, 40, 48, 64};
 
 int blk16_words(int d, int ny) {
   const int need = (ny + 1) / 2;
   const int mx = dary16_max_words(d);
-  if (d < 3 || d > 10) return 0;
+  if (d < 2 || d > 10) return 0;
   for (int w : kBlkWords) if (w >= need && w <= mx) return w;
   return 0;
 }
@@ -45,7 +45,7 @@ template <int D>
 static hipError_t launch_blk(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
   switch (a.ny_pad / 2) {
 #define X(n) case n: if constexpr (n <= dary16_max_words(D)) return launch_blk_nw<D, n>(mode, a, st, block); break;
-    X(8) X(16) X(24) X(32) X(40) X(48)
+    X(8) X(16) X(24) X(32) X(40) X(48) X(64)
 #undef X
     default: break;
   }
@@ -59,6 +59,7 @@ hipError_t launch_blk16(Mode mode, const PlainArgs &a, hipStream_t st, int block
   if (blk16_words(a.d, a.ny_pad) != words || words * 2 != a.ny_pad) return hipErrorInvalidValue;
   if (a.q < blk16_low_rows(a.d, words) || a.R - 1 < a.q) return hipErrorInvalidValue;
   switch (a.d) {
+    case 2: return launch_blk<2>(mode, a, st, block);
     case 3: return launch_blk<3>(mode, a, st, block);
     case 4: return launch_blk<4>(mode, a, st, block);
     case 5: return launch_blk<5>(mode, a, st, block);

@@ -489,7 +489,7 @@ __device__ __forceinline__ void walk_p16(
 }
 
 // ---------------------------------------------------------------------------
-// Blocked packed d-ary walk (d = 3..10): the M fastest rows are tabulated.
+// Blocked packed d-ary walk (d = 2..10): the M fastest rows are tabulated.
 //
 // Rank N = h * d^M + l.  The high index h walks rows M..R-2 by the reflected
 // d-ary code (digit j of h is row M + j); the low index l runs through all
@@ -511,7 +511,7 @@ __device__ __forceinline__ void walk_p16(
 // those two labels only: 2 * 2^M * NW v_sad_u16 for the d^M labelings of
 // the block.  Each labeling then costs one sum of table entries at compile-
 // time indices and half a v_max3:
-//   d = 3: F_0 + F_1 + F_2, one v_add3;
+//   d = 2, 3: F_0 + F_1 (+ F_2), one add;
 //   d >= 4: sum_c |u_c| + sum over the labels with S_c != {} of
 //     G_c(S_c) = F_c(S_c) - F_c({}), at most M terms.
 // Exact under the packed walk's bound (every column's absolute sum <= 32767),
@@ -521,8 +521,11 @@ constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
 
 // Rows tabulated: d^M labelings per high step against 2^(M+1) table sums
 // per state word; d * NW state plus d * 2^M table registers stay below 220.
+// (d = 2: both labels change at every high step, 2 sums per word and
+// labeling whatever M.)
 constexpr int blk16_low_rows(int d, int nw) {
-  return d == 3 ? (nw <= 32 ? 5 : 4) : d == 4 ? 4 : (d == 5 || d == 6) ? 3 : (d >= 7 && d <= 10) ? 2 : 0;
+  return (d == 2 || d == 3) ? (nw <= 32 ? 5 : 4) : d == 4 ? 4 : (d == 5 || d == 6) ? 3
+                                                     : (d >= 7 && d <= 10) ? 2 : 0;
 }
 
 template <int D, int M>
@@ -548,7 +551,7 @@ template <int D, int M, bool ODD, int L, int... Cs>
 __device__ __forceinline__ int32_t blk16_obj(const uint32_t (&T)[D][1 << M], uint32_t totU,
                                              std::integer_sequence<int, Cs...>) {
   using LC = LowCode<D, M>;
-  if constexpr (D == 3)
+  if constexpr (D <= 3)
     return (int32_t)(0u + ... + T[Cs][LC::mask(L, Cs, ODD)]);
   else
     return (int32_t)(totU + ... + (LC::mask(L, Cs, ODD) ? T[Cs][LC::mask(L, Cs, ODD)] : 0u));
@@ -583,7 +586,7 @@ __device__ __forceinline__ void blk16_eval(const uint32_t (&T)[D][1 << M], uint3
 }
 
 // Table of label C from its sums u[C]: T[C][0] = |u_C|, and T[C][S] = F_C(S)
-// (d = 3) or F_C(S) - |u_C| (d >= 4, the accumulator starts at -|u_C|).
+// (d <= 3) or F_C(S) - |u_C| (d >= 4, the accumulator starts at -|u_C|).
 template <int C, int D, int NW, int M>
 __device__ __forceinline__ void blk16_table(const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
                                             const cuint32_t *neg) {
@@ -591,7 +594,7 @@ __device__ __forceinline__ void blk16_table(const uint32_t (&u)[D][NW], uint32_t
 #pragma unroll
   for (int w = 0; w < NW; ++w) U = sad16(u[C][w], neg[w], U);
   T[C][0] = U;
-  const uint32_t base = D == 3 ? 0u : 0u - U;
+  const uint32_t base = D <= 3 ? 0u : 0u - U;
 #pragma unroll
   for (int S = 1; S < (1 << M); ++S) {
     uint32_t acc = base;

@@ -11,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 # rows the tabulated walk keeps in its subset tables, per d (up to 64 columns;
 # 4 at d = 3 beyond 64 columns)
-TABLE_ROWS = {3: 5, 4: 4, 5: 3, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
+TABLE_ROWS = {2: 5, 3: 5, 4: 4, 5: 3, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
 
 
 def _laid_out(case):
@@ -78,7 +78,7 @@ def test_random_vs_oracle_plain_and_general(pkg, n, marg):
         if n >= 2 and C <= 2 * words and int(np.abs(A).sum(axis=0).max()) <= 32767:
             res = _agree(pkg, A, n, marg, kernel="plain16")
             assert res["kernel_name"].startswith("plain16")
-            if n >= 3 and R - 1 >= TABLE_ROWS[n]:
+            if R - 1 >= TABLE_ROWS[n]:
                 res = _agree(pkg, A, n, marg, kernel="table16")
                 assert res["kernel_name"].startswith("table16")
 
@@ -314,6 +314,7 @@ def test_wide16_walk_vs_oracle(pkg, d, R, cols):
 # (d, rows, column counts) for the tabulated walk: every instantiated width
 # (8..48 words) and odd column counts; rows leave several high digits
 TABLE16 = [
+    (2, 15, [1, 5, 16, 33, 64, 65, 100, 128]),
     (3, 12, [1, 5, 16, 17, 33, 48, 64, 65, 80, 96]),
     (4, 10, [3, 16, 31, 48, 64]),
     (5, 8, [7, 16, 33, 48, 64]),
