@@ -235,6 +235,43 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
   pp.ny_pad = walk == WALK_GENERAL ? 0 : ny_pad;
   pp.qmin = 0;
   pp.D.clear();
+  if (walk == WALK_TAB16 && p.n == 1) {
+    // L1 / LM sign-pattern table (walk_tab16_bin): A = R x NW packed |.|
+    // columns, R x NW packed 2A, (LM) R column-0 entries; D = 2^M x NW words
+    // -A_S + 2^15, (LM) 2^M column-0 sums A_S,0
+    const int marg = p.kind == KIND_LM ? 1 : 0;
+    const int NW = ny_pad / 2, M = kTabBinRowsHost;
+    auto pack = [&](int32_t *dst, int i, int f) {
+      for (int l = marg; l < p.C; ++l) {
+        const uint32_t v = (uint32_t)(f * p.A[(size_t)i * p.C + l]) & 0xFFFFu;
+        dst[(l - marg) / 2] |= (int32_t)(v << (16 * ((l - marg) & 1)));
+      }
+    };
+    pp.A.assign((size_t)2 * p.R * NW + (marg ? p.R : 0), 0);
+    for (int i = 0; i < p.R; ++i) {
+      pack(pp.A.data() + (size_t)i * NW, i, 1);
+      pack(pp.A.data() + (size_t)(p.R + i) * NW, i, 2);
+      if (marg) pp.A[(size_t)2 * p.R * NW + i] = p.A[(size_t)i * p.C];
+    }
+    pp.D.assign(((size_t)NW << M) + (marg ? (1u << M) : 0), 0);
+    for (int S = 0; S < (1 << M); ++S) {
+      for (int l = 0; l < p.C; ++l) {
+        int32_t sum = 0;
+        for (int i = 0; i < M; ++i) sum += ((S >> i) & 1 ? 1 : -1) * p.A[(size_t)i * p.C + l];
+        if (marg && l == 0) {
+          pp.D[((size_t)NW << M) + S] = sum;
+        } else {
+          const int c = l - marg;
+          const uint32_t v = (uint32_t)(kBias16 - sum) & 0xFFFFu;
+          pp.D[(size_t)S * NW + c / 2] |= (int32_t)(v << (16 * (c & 1)));
+        }
+      }
+      for (int c = p.C - marg; c < 2 * NW; ++c)   // padding columns: |0 + 0|
+        pp.D[(size_t)S * NW + c / 2] |= (int32_t)(kBias16 << (16 * (c & 1)));
+    }
+    pp.qmin = M;
+    return;
+  }
   if (walk == WALK_PLAIN16 || walk == WALK_WIDE16 || walk == WALK_TAB16) {
     // two columns per word, low half = even column
     const int NW = ny_pad / 2;
@@ -285,7 +322,7 @@ hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const Pl
     case WALK_PLAIN16: return launch_plain16(mode, a, st, block);
     case WALK_WIDE: return launch_wide(p.kind, mode, a, st);
     case WALK_WIDE16: return launch_wide16(mode, a, st);
-    case WALK_TAB16: return launch_blk16(mode, a, st, block);
+    case WALK_TAB16: return p.n == 1 ? launch_tab16_bin(p.kind, mode, a, st, block) : launch_blk16(mode, a, st, block);
     default: return launch_general(p.kind, mode, a, p.C, st);
   }
 }
@@ -639,8 +676,14 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
   // the W-wave walk combines steps in groups of 8: at least 3 enumerated rows
   c.ny_wide = (p.R - 1 >= 3) ? wide_ny_pad(p.kind, p.C) : 0;
   c.w16_wide = (fits16 && p.n >= 3) ? wide16_words(p.n, p.C) : 0;
-  c.wtab = (fits16 && p.n >= 2) ? blk16_words(p.n, p.C) : 0;
-  if (c.wtab && p.R - 1 < blk16_rows(p.n, c.wtab)) c.wtab = 0;   // fewer enumerated rows than table rows
+  if (p.n == 1) {
+    const int marg = p.kind == KIND_LM ? 1 : 0;
+    c.wtab = max_col_abs_sum(p, marg) <= 32767 ? tab16_bin_words(p.C - marg) : 0;
+    if (p.R - 1 < kTabBinRowsHost) c.wtab = 0;
+  } else {
+    c.wtab = (fits16 && p.n >= 2) ? blk16_words(p.n, p.C) : 0;
+    if (c.wtab && p.R - 1 < blk16_rows(p.n, c.wtab)) c.wtab = 0;   // fewer enumerated rows than table rows
+  }
   if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
     return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
   if (kernel == LNORM_KERNEL_PLAIN && !c.ny_plain)
@@ -649,14 +692,17 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
     return fail(LNORM_E_UNSUPPORTED, "W-wave walk needs an instantiated width (%d columns at n = %d) and, for "
                 "n <= 2, at least 3 enumerated rows, for n >= 3, column sums <= 32767", p.C, p.n);
   if (kernel == LNORM_KERNEL_TABLE16 && !c.wtab)
-    return fail(LNORM_E_UNSUPPORTED, "tabulated packed walk needs n = 2..10, column sums <= 32767, an "
+    return fail(LNORM_E_UNSUPPORTED, "tabulated packed walk needs |.| column sums <= 32767, an "
                 "instantiated width and more enumerated rows than table rows (%d x %d at n = %d)", p.R, p.C, p.n);
   if (kernel == LNORM_KERNEL_PLAIN16 && !c.w16)
     return fail(LNORM_E_UNSUPPORTED, "packed 16-bit walk needs n >= 2, column sums <= 32767 and an instantiated "
                 "width (%d columns at n = %d)", p.C, p.n);
   if (kernel == LNORM_KERNEL_AUTO)
-    kernel = can_block ? LNORM_KERNEL_BLOCKED
+    // n = 1: the 16-bit blocked kernel, else the tabulated walk, else the
+    // 32-bit blocked kernel
+    kernel = (can_block && !c.bp.wide) ? LNORM_KERNEL_BLOCKED
                        : c.wtab ? LNORM_KERNEL_TABLE16
+                       : can_block ? LNORM_KERNEL_BLOCKED
                        : (c.w16 ? LNORM_KERNEL_PLAIN16
                                 : (c.ny_plain ? LNORM_KERNEL_PLAIN
                                               : ((c.ny_wide || c.w16_wide) ? LNORM_KERNEL_WIDE
@@ -877,6 +923,9 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide<kind=%d,ny=%d,q=%d>", (int)p.kind, pp.ny_pad, g.q);
     else if (pp.walk == WALK_WIDE16)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide16<d=%d,words=%d,q=%d>", p.n, pp.ny_pad / 2, g.q);
+    else if (pp.walk == WALK_TAB16 && p.n == 1)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16<%s,words=%d,M=%d,q=%d>",
+               p.kind == KIND_LM ? "LM" : "L1", pp.ny_pad / 2, pp.qmin, g.q);
     else if (pp.walk == WALK_TAB16)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16<d=%d,words=%d,M=%d,q=%d>", p.n, pp.ny_pad / 2,
                pp.qmin, g.q);
@@ -965,6 +1014,7 @@ Geometry batch_geometry(int base, int e, int forced, Walk walk, int cap_log, int
 double walk_cost(Walk walk, int d, int width) {
   if (walk == WALK_PLAIN) return 2.0 * width;
   const int nw = width / 2;
+  if (walk == WALK_TAB16 && d == 1) return nw + 1.5 + nw / 32.0;
   if (walk == WALK_TAB16) {
     const int M = blk16_rows(d, nw);
     return 1.5 + (2.0 + (double)(2 << M)) * nw / std::pow((double)d, (double)M);
@@ -1092,9 +1142,17 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
         width = 2 * w16;
       }
     } else {
-      width = plain_ny_pad(b.p.kind, 1, b.p.C);
-      if (!width) { fallback.push_back(i); continue; }
-      walk = WALK_PLAIN;
+      const int marg = b.p.kind == KIND_LM ? 1 : 0;
+      const int wtb = (max_col_abs_sum(b.p, marg) <= 32767 && b.p.R - 1 >= kTabBinRowsHost)
+                          ? tab16_bin_words(b.p.C - marg) : 0;
+      if (wtb) {
+        walk = WALK_TAB16;
+        width = 2 * wtb;
+      } else {
+        width = plain_ny_pad(b.p.kind, 1, b.p.C);
+        if (!width) { fallback.push_back(i); continue; }
+        walk = WALK_PLAIN;
+      }
     }
     if (b.p.total > kBatchMaxTotal) { fallback.push_back(i); continue; }
     prepare_walk(b.p, walk, width, b.pp);

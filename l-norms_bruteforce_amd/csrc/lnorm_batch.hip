@@ -121,6 +121,30 @@ __global__ __launch_bounds__(256) void k_batch_t16(const BatchTask *__restrict__
   }
 }
 
+template <int NW, bool MARG, int MODE>
+__global__ __launch_bounds__(256) void k_batch_tb16(const BatchTask *__restrict__ tasks, int nitems,
+                                                    uint64_t ntasks, unsigned int *__restrict__ counter) {
+  int shard = (int)(blockIdx.x & (kCounterShards - 1)), left = kCounterShards;
+  uint32_t t;
+  while (next_task(counter, ntasks, shard, left, t)) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tasks[mid].task0 <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchTask &it = tasks[lo];
+    uint64_t seg0 = 0;
+    int32_t target = 0;
+    if (MODE == MODE_FIRST) {
+      seg0 = *it.unit * it.seg_base;
+      target = *it.gbest;
+    }
+    walk_tab16_bin<NW, MARG, MODE>(as_const((const uint32_t *)it.A), as_const((const uint32_t *)it.D), it.R, it.q,
+                                   seg0, it.nseg, target, it.unit_best, it.gbest, it.gfirst, t - it.task0);
+  }
+}
+
 // One workgroup per item: the smallest unit index whose best equals the
 // item's maximum.
 __global__ __launch_bounds__(256) void k_batch_first_unit(const BatchTask *__restrict__ tasks,
@@ -205,6 +229,28 @@ static hipError_t lb_t16_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
   return hipErrorInvalidValue;
 }
 
+template <int NW, bool MARG>
+static hipError_t lb_tb16(Mode mode, const BatchLaunch &b, hipStream_t st) {
+  const dim3 grid(batch_grid(b.ntasks)), blk(256);
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_batch_tb16<NW, MARG, MODE_MAX>), grid, blk, 0, st, b.tasks, b.nitems, b.ntasks, b.counter);
+  else
+    hipLaunchKernelGGL((k_batch_tb16<NW, MARG, MODE_FIRST>), grid, blk, 0, st, b.tasks, b.nitems, b.ntasks,
+                       b.counter);
+  return hipGetLastError();
+}
+
+template <bool MARG>
+static hipError_t lb_tb16_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
+  switch (b.width / 2) {
+#define X(n) case n: return lb_tb16<n, MARG>(mode, b, st);
+    X(4) X(8) X(12) X(16) X(24) X(32) X(48) X(64)
+#undef X
+    default: break;
+  }
+  return hipErrorInvalidValue;
+}
+
 template <int KIND>
 static hipError_t lb_bin_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
   switch (b.width) {
@@ -233,6 +279,8 @@ hipError_t launch_batch_walk(Mode mode, const BatchLaunch &b, hipStream_t st) {
     if (b.kind == KIND_LM) return lb_bin_w<KIND_LM>(mode, b, st);
     return hipErrorInvalidValue;
   }
+  if (b.table && b.kind == KIND_L1) return lb_tb16_w<false>(mode, b, st);
+  if (b.table && b.kind == KIND_LM) return lb_tb16_w<true>(mode, b, st);
   if (b.table) {
     switch (b.d) {
       case 2: return lb_t16_w<2>(mode, b, st);

@@ -133,6 +133,9 @@ int wide16_words(int d, int ny);      // packed W-wave width in words (0: none)
 int blk16_words(int d, int ny);       // tabulated packed walk: width in words (0: none)
 int blk16_rows(int d, int words);     // its tabulated rows M
 hipError_t launch_blk16(Mode mode, const PlainArgs &a, hipStream_t st, int block);
+int tab16_bin_words(int nabs);        // tabulated L1/LM walk: width in words for nabs |.| columns (0: none)
+constexpr int kTabBinRowsHost = 5;    // its table rows (the sign patterns of rows 0..4)
+hipError_t launch_tab16_bin(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block);
 hipError_t launch_wide16(Mode mode, const PlainArgs &a, hipStream_t st);
 int plain16_words(int d, int ny);               // packed width instantiated for ny columns, 0 if none
 bool plain_supported(Kind kind, int d, int ny_pad);

@@ -771,4 +771,141 @@ __device__ __forceinline__ void walk_blk16(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Tabulated binary walk for L1 / LM (packed 16-bit).  Rank N = h * 2^M + l:
+// BRGC bit i of N (L_OpenMP.c:58-69, bit i of N ^ (N >> 1)) is bit i of
+// l ^ (l >> 1) for i < M - 1, that bit xor (h & 1) for i = M - 1, and bit
+// i - M of h ^ (h >> 1) for the high rows.  With the lane's high sum
+//   u = A_{R-1} + sum_{i >= M} s_i A_i   (s_i = 2 b_i - 1),
+// low sign pattern S (bit i = b_i) has the objective sum_l |u_l + A_S,l|,
+// A_S = sum_{i < M} (2 S_i - 1) A_i: one v_sad_u16 per two columns against
+// the scalar table word -A_S + 2^15.  LM adds column 0 without |.|
+// (L_OpenMP.c:30, :40-41): u_0 + A_S,0 in 32 bits starts the accumulator.
+// A high step adds +-2 A_r to u, one packed add per word for 2^M sign
+// vectors.  Exact when every |.| column's absolute sum is <= 32767.
+//
+// Ap: R x NW packed |.| columns (LM: columns 1..C-1), then R x NW packed
+// 2 A, then (LM) the R column-0 entries.  Tab: 2^M x NW words -A_S + 2^15,
+// then (LM) the 2^M column-0 sums A_S,0.
+// ---------------------------------------------------------------------------
+constexpr int kTabBinRows = kTabBinRowsHost;
+
+template <int NW, int M, bool MARG, int MODE, int S>
+__device__ __forceinline__ int32_t tb16_obj(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
+                                            const cint32_t *pos0) {
+  uint32_t acc = MARG ? (uint32_t)(u0 + pos0[S]) : 0u;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) acc = sad16(u[w], neg[(size_t)S * NW + w], acc);
+  return (int32_t)acc;
+}
+
+// MODE_MAX: patterns in table order, two per v_max3.
+template <int NW, int M, bool MARG, int... Ps>
+__device__ __forceinline__ void tb16_max(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
+                                         const cint32_t *pos0, int32_t &best, std::integer_sequence<int, Ps...>) {
+  auto pair = [&](auto P) {
+    constexpr int S = decltype(P)::value;
+    const int32_t a = tb16_obj<NW, M, MARG, MODE_MAX, 2 * S>(u, u0, neg, pos0);
+    const int32_t b = tb16_obj<NW, M, MARG, MODE_MAX, 2 * S + 1>(u, u0, neg, pos0);
+    const int32_t m = a > b ? a : b;
+    best = m > best ? m : best;
+  };
+  (pair(std::integral_constant<int, Ps>{}), ...);
+}
+
+// MODE_FIRST: patterns in the order of l; fb = the first l matching.
+template <int NW, int M, bool MARG, bool ODD, int... Ls>
+__device__ __forceinline__ void tb16_first(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
+                                           const cint32_t *pos0, int32_t target, uint32_t &fb,
+                                           std::integer_sequence<int, Ls...>) {
+  auto one = [&](auto L) {
+    constexpr int l = decltype(L)::value;
+    constexpr int S = (l ^ (l >> 1)) ^ (ODD ? (1 << (M - 1)) : 0);
+    const int32_t a = tb16_obj<NW, M, MARG, MODE_FIRST, S>(u, u0, neg, pos0);
+    const uint32_t c = (a == target) ? (uint32_t)l : 0xFFFFFFFFu;
+    fb = c < fb ? c : fb;
+  };
+  (one(std::integral_constant<int, Ls>{}), ...);
+}
+
+template <int NW, bool MARG, int MODE>
+__device__ __forceinline__ void walk_tab16_bin(
+    const cuint32_t *Ap, const cuint32_t *Tab, int R, int q, uint64_t seg0, uint64_t nseg, int32_t target,
+    int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest, unsigned long long *__restrict__ gfirst,
+    uint64_t w) {
+  constexpr int M = kTabBinRows;
+  const int lane = wave_lane();
+  if (((w >> 1) << 7) >= nseg) return;
+  const uint64_t local = lane_segment(w, lane);
+  const bool active = local < nseg;
+  const uint64_t seg = seg0 + local;
+  const uint32_t pseg = (uint32_t)__builtin_amdgcn_readfirstlane((int)((seg0 ^ w) & 1));
+  const int qh = q - M;
+  const cuint32_t *A2 = Ap + (size_t)R * NW;                       // packed 2 A
+  const cint32_t *A0 = (const cint32_t *)(Ap + (size_t)2 * R * NW);  // LM: column 0
+  const cint32_t *pos0 = (const cint32_t *)(Tab + ((size_t)NW << M));
+
+  // seed: the high rows' BRGC bits at h0 = seg * 2^qh
+  const uint64_t h0 = seg << qh;
+  const uint64_t g0 = h0 ^ (h0 >> 1);
+  uint32_t u[NW];
+  int32_t u0 = MARG ? A0[R - 1] : 0;
+#pragma unroll
+  for (int l = 0; l < NW; ++l) u[l] = pk_add(kBias16x2, Ap[(size_t)(R - 1) * NW + l]);
+  for (int j = 0; j < R - 1 - M; ++j) {
+    const bool b = (g0 >> j) & 1;
+    const cuint32_t *row = Ap + (size_t)(M + j) * NW;
+#pragma unroll
+    for (int l = 0; l < NW; ++l) u[l] = b ? pk_add(u[l], row[l]) : pk_sub(u[l], row[l]);
+    if (MARG) u0 += b ? A0[M + j] : -A0[M + j];
+  }
+
+  int32_t best = INT32_MIN;
+  uint32_t first = 0xFFFFFFFFu;
+  const uint32_t nblk = 1u << qh;
+  for (uint32_t k = 0; k < nblk; ++k) {
+    if (k > 0) {
+      // high digit j flips to 1 ^ (bit j + 1 of h); bit qh of h is the
+      // segment's parity
+      const int j = __builtin_ctz(k);
+      const uint32_t hb = (j + 1 < qh) ? ((k >> (j + 1)) & 1u) : pseg;
+      const cuint32_t *row2 = A2 + (size_t)(M + j) * NW;
+      if (hb == 0) {
+#pragma unroll
+        for (int l = 0; l < NW; ++l) u[l] = pk_add(u[l], row2[l]);
+        if (MARG) u0 += 2 * A0[M + j];
+      } else {
+#pragma unroll
+        for (int l = 0; l < NW; ++l) u[l] = pk_sub(u[l], row2[l]);
+        if (MARG) u0 -= 2 * A0[M + j];
+      }
+    }
+    // the table's 2^M * NW scalar loads stay inside the loop (hoisted, they
+    // would not fit in SGPRs)
+    const cuint32_t *neg = Tab;
+    asm volatile("" : "+s"(neg));
+    if constexpr (MODE == MODE_MAX) {
+      tb16_max<NW, M, MARG>(u, u0, neg, pos0, best, std::make_integer_sequence<int, (1 << M) / 2>{});
+    } else {
+      uint32_t fb = 0xFFFFFFFFu;
+      const uint32_t hp = qh > 0 ? (k & 1u) : pseg;
+      if (hp) tb16_first<NW, M, MARG, true>(u, u0, neg, pos0, target, fb, std::make_integer_sequence<int, 1 << M>{});
+      else tb16_first<NW, M, MARG, false>(u, u0, neg, pos0, target, fb, std::make_integer_sequence<int, 1 << M>{});
+      const uint32_t c = fb != 0xFFFFFFFFu ? (k << M) + fb : 0xFFFFFFFFu;
+      first = c < first ? c : first;
+    }
+  }
+  const uint32_t nsteps = nblk << M;
+
+  if constexpr (MODE == MODE_MAX) {
+    if (active) unit_best[local] = best;
+    const int32_t m = wave_max_i32(active ? best : INT32_MIN);
+    if (m > __builtin_amdgcn_readfirstlane(*(volatile int32_t *)gbest)) atomicMax(gbest, m);
+  } else {
+    const uint64_t r = (active && first != 0xFFFFFFFFu) ? (seg * (uint64_t)nsteps + first) : ~0ull;
+    const uint64_t m = wave_min_u64(r);
+    if (m != ~0ull) atomicMin(gfirst, (unsigned long long)m);
+  }
+}
+
 }  // namespace lnorm

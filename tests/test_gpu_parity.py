@@ -357,6 +357,46 @@ def test_table16_walk_vs_oracle(pkg, d, R, cols):
     assert e.value.code == -5
 
 
+@pytest.mark.parametrize("marg", [0, 1])
+def test_table16_binary_vs_oracle(pkg, marg):
+    """The tabulated L1 / LM walk (the 5 fastest rows as a sign-pattern
+    table): bit-exact norm, strategy and rank at every width (4..64 words of
+    |.| columns, odd counts, LM's column 0 outside the words), at forced
+    segment lengths, with exactly 5 enumerated rows, and AUTO's choice of it
+    beyond the 16-bit blocked kernel's 64 |.| columns."""
+    rng = np.random.default_rng(950 + marg)
+    for R in (6, 9, 14):
+        for C in (1 + marg, 2, 7, 8, 9, 16, 17, 31, 33, 64, 65, 100, 128 + marg):
+            A = _rand(rng, R, C, 60)
+            print("table16 L1", marg, A.shape, flush=True)
+            res = _agree(pkg, A, 1, marg, kernel="table16")
+            assert res["kernel_name"].startswith("table16<L"), res["kernel_name"]
+    A = _rand(rng, 14, 40, 60)
+    for q in (5, 6, 9, 13):
+        print("table16 L1 seg_log", marg, q, flush=True)
+        res = _agree(pkg, A, 1, marg, kernel="table16", seg_log=q)
+        assert f",q={q}>" in res["kernel_name"], res["kernel_name"]
+    A = _rand(rng, 14, 100, 60)
+    assert _agree(pkg, A, 1, marg)["kernel_name"].startswith("table16<L"), "AUTO beyond 64 |.| columns"
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm(_rand(rng, 5, 9, 60), 1, marg, kernel="table16")
+    assert e.value.code == -5
+
+
+def test_table16_binary_equals_blocked_large(pkg):
+    """2^33 sign vectors: the tabulated L1 / LM walk equals the blocked
+    kernel (norm, strategy, rank), and the objective of the strategy is the
+    norm."""
+    from lnorm_amd import gen
+    for (R, C, marg) in ((34, 40, 0), (34, 36, 1)):
+        A = gen.gen_matrix(R, C, 100, 60 + marg)
+        print("table16 vs blocked", R, C, marg, flush=True)
+        t = pkg.calc_lnorm(A, 1, marg, kernel="table16")
+        b = pkg.calc_lnorm(A, 1, marg, kernel="blocked")
+        assert t[:2] == b[:2] and t[2]["best_rank"] == b[2]["best_rank"], (R, C, marg)
+        assert oracle.objective(A, 1, marg, t[1]) == t[0]
+
+
 def test_table16_equals_plain16_larger(pkg):
     """At sizes the oracle does not reach: the tabulated walk and the packed
     walk agree on norm, strategy and rank (3^23 labelings for the 24 x 32 L3

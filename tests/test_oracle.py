@@ -402,3 +402,62 @@ def test_tabulated_objective(D, M):
         assert sum(F(c, masks[c]) for c in range(D)) == want
         totU = sum(F(c, 0) for c in range(D))
         assert totU + sum(F(c, masks[c]) - F(c, 0) for c in range(D) if masks[c]) == want
+
+
+def _tab_bin_walk(M, q, seg, R):
+    """Restatement of walk_tab16_bin's order for one segment of 2^q ranks:
+    yields (offset, BRGC bits of rows 0..R-2).  Low bits of pattern l:
+    l ^ (l >> 1), the top one xor (h & 1); a high step k flips row M + ctz(k)
+    to 1 ^ (bit ctz(k) + 1 of h), bit qh of h being the segment's parity."""
+    qh = q - M
+    ps = seg & 1
+    h0 = seg << qh
+    g0 = h0 ^ (h0 >> 1)
+    high = [(g0 >> j) & 1 for j in range(R - 1 - M)]
+    for k in range(1 << qh):
+        if k > 0:
+            j = (k & -k).bit_length() - 1
+            hb = (k >> (j + 1)) & 1 if j + 1 < qh else ps
+            high[j] = 1 ^ hb
+        hp = (k & 1) if qh > 0 else ps
+        for l in range(1 << M):
+            S = (l ^ (l >> 1)) ^ (hp << (M - 1))
+            yield (k << M) + l, [(S >> i) & 1 for i in range(M)] + list(high)
+
+
+def test_tabulated_binary_walk_order():
+    """The L1 / LM tabulated walk visits ranks in the reference's BRGC order
+    (L_OpenMP.c:58-69): every rank of aligned segments of 2^q ranks, q = 5..8,
+    against the oracle's digits."""
+    M, R = 5, 11
+    for q in (5, 6, 8):
+        for seg in (0, 1, 2, 3, 6, 7):
+            if (seg + 1) << q > 1 << (R - 1):
+                continue
+            N0 = seg << q
+            for off, bits in _tab_bin_walk(M, q, seg, R):
+                want = [oracle.digit(N0 + off, i, 2) for i in range(R - 1)]
+                assert bits == want, (q, seg, off)
+
+
+@pytest.mark.parametrize("marg", [0, 1])
+def test_tabulated_binary_objective(marg):
+    """sum_l |u_l + A_S,l| over the |.| columns (u = A_{R-1} + sum of the
+    high rows' signed rows, A_S the low rows' signed sum), plus u_0 + A_S,0
+    for LM, equals the objective; the table word -A_S + 2^15 makes each |.| one
+    unsigned 16-bit absolute difference."""
+    rng = np.random.default_rng(77 + marg)
+    M, R, C = 5, 9, 7
+    A = rng.integers(-90, 91, size=(R, C)).astype(np.int64)
+    for _ in range(40):
+        bits = list(rng.integers(0, 2, size=R - 1))
+        s = [2 * b - 1 for b in bits] + [1]
+        u = sum(s[i] * A[i] for i in range(M, R))
+        S = sum(bits[i] << i for i in range(M))
+        AS = sum((2 * ((S >> i) & 1) - 1) * A[i] for i in range(M))
+        cols = range(marg, C)
+        neg = [(0x8000 - int(AS[l])) & 0xFFFF for l in cols]
+        got = sum(abs((int(u[l]) + 0x8000) - n) for l, n in zip(cols, neg))
+        if marg:
+            got += int(u[0] + AS[0])
+        assert got == oracle.objective(A.astype(np.int32), 1, marg, [int(b) for b in bits])
