@@ -64,6 +64,9 @@ def valu_ops_per_unit(kernel: str, ny: int, marginal: int, P: int = 2) -> float:
         return (ny - marginal + 1) // 2 + marginal + (1.0 if P == 1 else 0.5)
     if kernel == "blocked32":        # one v_sad_u32 per |.| column
         return ny - marginal + 1 + marginal
+    if kernel == "table16":          # tabulated walk: one v_sad_u16 per two |.| columns, half a v_max3,
+        words = (ny - marginal + 1) // 2    # the marginal add, and the high step's packed add per 32
+        return words + 0.5 + marginal + words / 32.0
     return 2 * ny + 1
 
 
@@ -302,7 +305,7 @@ def table16_ops(d: int, words: int) -> float:
     return 1.5 + (2 + 2 ** (M + 1)) * words / d ** M
 
 
-def batch_valu_ops(d: int, rows: int, cols: int) -> float:
+def batch_valu_ops(d: int, rows: int, cols: int, marginal: int = 0) -> float:
     """VALU lane-operations per labeling of the batched walks by the op model,
     on the item's real width: L1/LM one add and one |.|-accumulate per column
     plus the maximum; d >= 2 with at least M + 1 rows the tabulated walk
@@ -311,6 +314,8 @@ def batch_valu_ops(d: int, rows: int, cols: int) -> float:
     exchange a row), d - 1 adds of the label sums and the maximum (an upper
     bound: the unrolled row-0 runs need fewer)."""
     if d == 1:
+        if rows - 1 >= 5:           # tabulated L1 / LM walk (16-bit sums: entries <= 100, <= 30 rows)
+            return valu_ops_per_unit("table16", cols, marginal)
         return 2 * cols + 1
     words = (cols + 1) // 2
     if words <= TABLE_MAX_WORDS[d] and rows - 1 >= table16_rows(d, words):
@@ -336,7 +341,7 @@ def batch_line(pkg, gen, steps: int) -> dict:
         probs.append((p.matrix_laid_out(), p.n, p.marginal))
         per_d[p.n] = per_d.get(p.n, 0) + p.total
         row_updates += p.total * p.C
-        ops += p.total * batch_valu_ops(p.n, p.R, p.C)
+        ops += p.total * batch_valu_ops(p.n, p.R, p.C, p.marginal)
         host.free(p)
     eager, st0 = pkg.calc_batch(probs, stats=True)
     if any(s != 0 for _, _, s in eager):
@@ -503,16 +508,27 @@ def main():
                                                     "valu_TOPs", "valu_frac", "kernel_name")}
         line["speedup_vs_plain"] = sb["value"] / sp["value"]
         line["kernel_speedup_vs_plain"] = sp["kernel_ms"] / sb["kernel_ms"]
+        tb = run_steps(pkg, A, "table16", a.steps, a.warmup)
+        if (tb["norm"], tb["best_rank"]) != (blk["norm"], blk["best_rank"]):
+            raise RuntimeError("tabulated and blocked kernels disagree")
+        st16 = summarize(tb, total, ny, "table16")
+        line["table16_same_matrix"] = {k2: st16[k2] for k2 in ("value", "ms_per_step", "kernel_ms", "valu_TOPs",
+                                                              "valu_frac", "kernel_name")}
         B = gen.gen_matrix(40, 40, 100, 106)
         nb = run_steps(pkg, B, "blocked", a.steps, a.warmup)
         npl = run_steps(pkg, B, "plain", max(1, a.steps // 2), a.warmup)
         if (nb["norm"], nb["best_rank"]) != (npl["norm"], npl["best_rank"]):
             raise RuntimeError("plain and blocked kernels disagree on 40x40")
+        ntb = run_steps(pkg, B, "table16", a.steps, a.warmup)
+        if (ntb["norm"], ntb["best_rank"]) != (npl["norm"], npl["best_rank"]):
+            raise RuntimeError("plain and tabulated kernels disagree on 40x40")
         s1 = summarize(nb, 2 ** 39, 40, "blocked")
         s2 = summarize(npl, 2 ** 39, 40, "plain")
-        line["n40_m40"] = {"blocked": s1["value"], "plain": s2["value"], "speedup": s1["value"] / s2["value"],
+        s5 = summarize(ntb, 2 ** 39, 40, "table16")
+        line["n40_m40"] = {"blocked": s1["value"], "plain": s2["value"], "table16": s5["value"],
+                           "speedup": s1["value"] / s2["value"],
                            "blocked_valu_frac": s1["valu_frac"], "plain_valu_frac": s2["valu_frac"],
-                           "norm": s1["norm"]}
+                           "table16_valu_frac": s5["valu_frac"], "norm": s1["norm"]}
         # 128 x 40 int16 (entries in [-30000, 30000], seed 105), laid out 40 x 128:
         # column sums beyond 2^15, so the blocked path uses 32-bit words
         # (hi/lo int8 MFMA planes for the table, one v_sad_u32 per column)

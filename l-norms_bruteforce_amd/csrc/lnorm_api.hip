@@ -698,9 +698,11 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
     return fail(LNORM_E_UNSUPPORTED, "packed 16-bit walk needs n >= 2, column sums <= 32767 and an instantiated "
                 "width (%d columns at n = %d)", p.C, p.n);
   if (kernel == LNORM_KERNEL_AUTO)
-    // n = 1: the 16-bit blocked kernel, else the tabulated walk, else the
+    // n = 1: the 16-bit blocked kernel from 2^33 sign vectors (measured 4-11%
+    // ahead of the tabulated walk at 2^35-2^39; below, its waves get too few
+    // high steps: 2^29 ran 27x slower), else the tabulated walk, else the
     // 32-bit blocked kernel
-    kernel = (can_block && !c.bp.wide) ? LNORM_KERNEL_BLOCKED
+    kernel = (can_block && !c.bp.wide && (p.total >= (1ull << 33) || !c.wtab)) ? LNORM_KERNEL_BLOCKED
                        : c.wtab ? LNORM_KERNEL_TABLE16
                        : can_block ? LNORM_KERNEL_BLOCKED
                        : (c.w16 ? LNORM_KERNEL_PLAIN16

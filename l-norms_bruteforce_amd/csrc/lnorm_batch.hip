@@ -244,7 +244,7 @@ template <bool MARG>
 static hipError_t lb_tb16_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
   switch (b.width / 2) {
 #define X(n) case n: return lb_tb16<n, MARG>(mode, b, st);
-    X(4) X(8) X(12) X(16) X(24) X(32) X(48) X(64)
+    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64)
 #undef X
     default: break;
   }
