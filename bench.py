@@ -399,8 +399,8 @@ def tune_sweep(pkg, gen) -> dict:
 
 
 def run_case(pkg, gen, case: str, steps: int, warmup: int) -> int:
-    """One workload alone: the headline matrix on the blocked or the plain
-    kernel, or the 4096-item batch."""
+    """One workload alone: the headline matrix on the blocked, plain or
+    tabulated kernel, the L3 workload, or the 4096-item batch."""
     if case == "batch":
         print(json.dumps({"case": case, **batch_line(pkg, gen, steps)}), flush=True)
         return 0
@@ -425,7 +425,7 @@ def main():
                     help="run the default launch shapes instead of the autotuner's (tuned before timing)")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the autotuner's candidates on the bench workloads and print them as JSON")
-    ap.add_argument("--case", choices=["blocked", "plain", "batch", "l3"],
+    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "l3"],
                     help="run only this workload (for profiler passes) and print its figures")
     ap.add_argument("--copy-calibration", type=int, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB (torch) and exit: the reference "

@@ -779,7 +779,7 @@ TuneKey tune_key(int dev, const Problem &p, const Choice &c) {
 // fields of o) and appends one entry per candidate to `log` when given.
 int tune_single(DevCtx &cx, const Problem &p, const Choice &c, const lnorm_hip_opts &o, hipStream_t st,
                 TuneVal &best, std::vector<lnorm_hip_tune_entry> *log) {
-  double best_rate = -1;
+  double best_rate = -1, default_rate = -1;
   auto note = [&](const lnorm_hip_tune_entry &e) {
     if (log) log->push_back(e);
     if (e.labelings_per_s > best_rate) {
@@ -809,7 +809,19 @@ int tune_single(DevCtx &cx, const Problem &p, const Choice &c, const lnorm_hip_o
         e.ms = out.ms_kernel;
         e.labelings_per_s = (double)(out.units * per_wave) / (out.ms_kernel / 1e3);
         note(e);
+        // the default shape (P and s chosen by blocked_plan) stays unless a
+        // candidate beats it by more than 2%: shapes within the probes' noise
+        // would otherwise change from run to run
+        lnorm_hip_opts od = o;
+        od.patterns_per_lane = 0;
+        od.wave_log2 = 0;
+        BlockedPlan bd;
+        if (blocked_plan(p, od, bd, why) && bd.P == P && bd.s == s) default_rate = e.labelings_per_s;
       }
+    }
+    if (default_rate > 0 && default_rate * 1.02 >= best_rate) {
+      best.P = 0;   // 0: blocked_plan's own choice
+      best.s = 0;
     }
   } else if (c.kernel == LNORM_KERNEL_PLAIN || c.kernel == LNORM_KERNEL_PLAIN16 ||
              c.kernel == LNORM_KERNEL_TABLE16) {
@@ -1665,6 +1677,10 @@ int lnorm_hip_autotune(const int32_t *A, int R, int C, int n, int marginal, cons
     best->kernel = ch.kernel;
     if (tv.P) best->patterns_per_lane = tv.P;
     if (tv.s) best->wave_log2 = tv.s;
+    if (ch.kernel == LNORM_KERNEL_BLOCKED && !tv.P) {   // the default shape kept: report it
+      best->patterns_per_lane = ch.bp.P;
+      best->wave_log2 = ch.bp.s;
+    }
     if (tv.block) best->block = tv.block;
   }
   for (int i = 0; i < (int)log.size() && i < max_entries; ++i) entries[i] = log[i];

@@ -33,8 +33,11 @@ def test_autotune_single(pkg, case):
     assert all(e["ms"] > 0 and _rate(e) > 0 for e in entries)
     top = max(entries, key=_rate)
     if case.startswith("blocked"):
+        # the fastest shape, or the default one when it is within 2% of it
         assert best["kernel"] == pkg.KERNEL_BLOCKED
-        assert (best["patterns_per_lane"], best["wave_log2"]) == (top["patterns_per_lane"], top["wave_log2"])
+        chosen = [e for e in entries if (e["patterns_per_lane"], e["wave_log2"]) ==
+                  (best["patterns_per_lane"], best["wave_log2"])]
+        assert len(chosen) == 1 and _rate(chosen[0]) * 1.02 >= _rate(top)
         assert len({(e["patterns_per_lane"], e["wave_log2"]) for e in entries}) == len(entries) > 4
     else:
         assert best["block"] == top["block"] and len(entries) == 3
