@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define LNORM_HIP_ABI_VERSION 3
+#define LNORM_HIP_ABI_VERSION 4
 
 /* error codes */
 #define LNORM_OK 0
