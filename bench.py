@@ -201,16 +201,16 @@ def cpu_baseline(gen, rows: int = 64, r: int = 127, seed: int = 104, budget_cols
             "sample": sample, "seconds": c["seconds"]}
 
 
-def single_line(pkg, A, n: int, total: int, ops, int64: bool = False) -> dict:
-    """One call on AUTO (after a warm-up call): labelings/s, the kernel AUTO
-    picked and, with an op count per unit, the VALU fraction by that model.
-    Used for the W-wave walk (L1 of a 36 x 256 matrix, 2^35 sign vectors,
-    2 ops per row-update) and the 64-bit walk (L1 of a 34 x 40 matrix scaled
-    by 2^28 + 3, beyond int32: 2^33 sign vectors)."""
+def single_line(pkg, A, n: int, total: int, ops, int64: bool = False, kernel: str = "auto") -> dict:
+    """One call (after a warm-up call): labelings/s, the kernel that ran and,
+    with an op count per unit, the VALU fraction by that model.  Used for the
+    wide L1 matrix (36 x 256, 2^35 sign vectors) on AUTO's tabulated walk and
+    on the W-wave walk (2 ops per row-update), and for the 64-bit walk (L1 of
+    a 34 x 40 matrix scaled by 2^28 + 3, beyond int32: 2^33 sign vectors)."""
     calc = pkg.calc_lnorm64 if int64 else pkg.calc_lnorm
-    calc(A, n, 0)
+    calc(A, n, 0, kernel=kernel)
     t = time.perf_counter()
-    norm, strat, res = calc(A, n, 0)
+    norm, strat, res = calc(A, n, 0, kernel=kernel)
     wall = time.perf_counter() - t
     kms = res["ms_kernel"]
     out = {"value": total / wall, "unit": "sign-vectors/s", "kernel_ms": kms, "kernel": res["kernel_name"],
@@ -543,7 +543,13 @@ def main():
                                 "blocked_valu_frac": s3["valu_frac"], "plain_valu_frac": s4["valu_frac"],
                                 "kernel": s3["kernel_name"], "norm": s3["norm"]}
         line["cli_end_to_end"] = cli_line(pkg, gen)
-        line["wide_36x256"] = single_line(pkg, gen.gen_matrix(36, 256, 100, 107), 1, 2 ** 35, 2 * 256 + 1)
+        W = gen.gen_matrix(36, 256, 100, 107)
+        wt = single_line(pkg, W, 1, 2 ** 35, valu_ops_per_unit("table16", 256, 0) + 128 / 16 - 128 / 32)
+        ww = single_line(pkg, W, 1, 2 ** 35, 2 * 256 + 1, kernel="wide")
+        if wt["norm"] != ww["norm"]:
+            raise RuntimeError("tabulated and W-wave walks disagree on 36x256")
+        line["wide_36x256"] = {**wt, "wwave": {k2: ww[k2] for k2 in ("value", "kernel", "kernel_ms", "valu_frac")},
+                               "speedup_vs_wwave": wt["value"] / ww["value"]}
         line["int64_34x40"] = single_line(pkg, gen.gen_matrix(34, 40, 100, 108).astype("int64") * (2 ** 28 + 3),
                                           1, 2 ** 33, None, int64=True)
         line["L3_24x32"] = l3_line(pkg, gen, a.steps, a.warmup, not a.no_cpu)

@@ -240,7 +240,7 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
     // columns, R x NW packed 2A, (LM) R column-0 entries; D = 2^M x NW words
     // -A_S + 2^15, (LM) 2^M column-0 sums A_S,0
     const int marg = p.kind == KIND_LM ? 1 : 0;
-    const int NW = ny_pad / 2, M = kTabBinRowsHost;
+    const int NW = ny_pad / 2, M = tab_bin_rows(NW);
     auto pack = [&](int32_t *dst, int i, int f) {
       for (int l = marg; l < p.C; ++l) {
         const uint32_t v = (uint32_t)(f * p.A[(size_t)i * p.C + l]) & 0xFFFFu;
@@ -679,7 +679,7 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
   if (p.n == 1) {
     const int marg = p.kind == KIND_LM ? 1 : 0;
     c.wtab = max_col_abs_sum(p, marg) <= 32767 ? tab16_bin_words(p.C - marg) : 0;
-    if (p.R - 1 < kTabBinRowsHost) c.wtab = 0;
+    if (c.wtab && p.R - 1 < tab_bin_rows(c.wtab)) c.wtab = 0;
   } else {
     c.wtab = (fits16 && p.n >= 2) ? blk16_words(p.n, p.C) : 0;
     if (c.wtab && p.R - 1 < blk16_rows(p.n, c.wtab)) c.wtab = 0;   // fewer enumerated rows than table rows
@@ -1157,8 +1157,8 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
       }
     } else {
       const int marg = b.p.kind == KIND_LM ? 1 : 0;
-      const int wtb = (max_col_abs_sum(b.p, marg) <= 32767 && b.p.R - 1 >= kTabBinRowsHost)
-                          ? tab16_bin_words(b.p.C - marg) : 0;
+      int wtb = max_col_abs_sum(b.p, marg) <= 32767 ? tab16_bin_words(b.p.C - marg) : 0;
+      if (wtb > 64 || (wtb && b.p.R - 1 < tab_bin_rows(wtb))) wtb = 0;   // batched widths: up to 64 words
       if (wtb) {
         walk = WALK_TAB16;
         width = 2 * wtb;

@@ -84,7 +84,7 @@ __global__ __launch_bounds__(256) void k_tab16_bin(
                                  global_wave());
 }
 
-static const int kTabBinWords[] = {4, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64};
+static const int kTabBinWords[] = {4, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96, 128, 160, 192};
 
 int tab16_bin_words(int nabs) {
   const int need = (nabs + 1) / 2;
@@ -112,7 +112,7 @@ template <bool MARG>
 static hipError_t launch_tb(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
   switch (a.ny_pad / 2) {
 #define X(n) case n: return launch_tb_nw<n, MARG>(mode, a, st, block);
-    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64)
+    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64) X(80) X(96) X(128) X(160) X(192)
 #undef X
     default: break;
   }
@@ -124,7 +124,7 @@ hipError_t launch_tab16_bin(Kind kind, Mode mode, const PlainArgs &a, hipStream_
   if (block <= 0 || block % 64 || block > 256) block = 256;
   const int words = a.ny_pad / 2;
   if (words * 2 != a.ny_pad || tab16_bin_words(2 * words) != words) return hipErrorInvalidValue;
-  if (a.q < kTabBinRows || a.R - 1 < a.q) return hipErrorInvalidValue;
+  if (a.q < tab_bin_rows(words) || a.R - 1 < a.q) return hipErrorInvalidValue;
   if (kind == KIND_L1) return launch_tb<false>(mode, a, st, block);
   if (kind == KIND_LM) return launch_tb<true>(mode, a, st, block);
   return hipErrorInvalidValue;

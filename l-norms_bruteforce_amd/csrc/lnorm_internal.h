@@ -134,7 +134,9 @@ int blk16_words(int d, int ny);       // tabulated packed walk: width in words (
 int blk16_rows(int d, int words);     // its tabulated rows M
 hipError_t launch_blk16(Mode mode, const PlainArgs &a, hipStream_t st, int block);
 int tab16_bin_words(int nabs);        // tabulated L1/LM walk: width in words for nabs |.| columns (0: none)
-constexpr int kTabBinRowsHost = 5;    // its table rows (the sign patterns of rows 0..4)
+// its table rows: the sign patterns of rows 0..M-1; 4 beyond 64 words keeps
+// the table (2^M x NW scalar words) and the unrolled block within bounds
+constexpr int tab_bin_rows(int nw) { return nw <= 64 ? 5 : 4; }
 hipError_t launch_tab16_bin(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block);
 hipError_t launch_wide16(Mode mode, const PlainArgs &a, hipStream_t st);
 int plain16_words(int d, int ny);               // packed width instantiated for ny columns, 0 if none

@@ -777,7 +777,8 @@ __device__ __forceinline__ void walk_blk16(
 // l ^ (l >> 1) for i < M - 1, that bit xor (h & 1) for i = M - 1, and bit
 // i - M of h ^ (h >> 1) for the high rows.  With the lane's high sum
 //   u = A_{R-1} + sum_{i >= M} s_i A_i   (s_i = 2 b_i - 1),
-// low sign pattern S (bit i = b_i) has the objective sum_l |u_l + A_S,l|,
+// low sign pattern S (bit i = b_i; M = 5 rows, 4 beyond 64 words) has the
+// objective sum_l |u_l + A_S,l|,
 // A_S = sum_{i < M} (2 S_i - 1) A_i: one v_sad_u16 per two columns against
 // the scalar table word -A_S + 2^15.  LM adds column 0 without |.|
 // (L_OpenMP.c:30, :40-41): u_0 + A_S,0 in 32 bits starts the accumulator.
@@ -788,7 +789,6 @@ __device__ __forceinline__ void walk_blk16(
 // 2 A, then (LM) the R column-0 entries.  Tab: 2^M x NW words -A_S + 2^15,
 // then (LM) the 2^M column-0 sums A_S,0.
 // ---------------------------------------------------------------------------
-constexpr int kTabBinRows = kTabBinRowsHost;
 
 template <int NW, int M, bool MARG, int MODE, int S>
 __device__ __forceinline__ int32_t tb16_obj(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
@@ -833,7 +833,7 @@ __device__ __forceinline__ void walk_tab16_bin(
     const cuint32_t *Ap, const cuint32_t *Tab, int R, int q, uint64_t seg0, uint64_t nseg, int32_t target,
     int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest, unsigned long long *__restrict__ gfirst,
     uint64_t w) {
-  constexpr int M = kTabBinRows;
+  constexpr int M = tab_bin_rows(NW);
   const int lane = wave_lane();
   if (((w >> 1) << 7) >= nseg) return;
   const uint64_t local = lane_segment(w, lane);

@@ -270,7 +270,8 @@ def test_wide_walk_vs_oracle(pkg, n, marg, R, cols):
     for C in cols:
         A = _rand(rng, R, C, 40)
         print("wide", n, marg, A.shape, flush=True)
-        res = _agree(pkg, A, n, marg)
+        # AUTO takes the tabulated walk up to 384 columns with 16-bit sums
+        res = _agree(pkg, A, n, marg, **({"kernel": "wide"} if (n == 1 and C - marg <= 384 and R > 5) else {}))
         assert res["kernel_name"].startswith("wide<"), res["kernel_name"]
     # narrower problems on the forced W-wave walk (AUTO keeps them elsewhere)
     A = _rand(rng, R, 65, 40)
@@ -381,6 +382,26 @@ def test_table16_binary_vs_oracle(pkg, marg):
     with pytest.raises(pkg.LnormError) as e:
         pkg.calc_lnorm(_rand(rng, 5, 9, 60), 1, marg, kernel="table16")
     assert e.value.code == -5
+
+
+@pytest.mark.parametrize("marg", [0, 1])
+def test_table16_binary_wide_vs_oracle(pkg, marg):
+    """The tabulated L1 / LM walk beyond 128 |.| columns (80..192 words, 4
+    table rows): bit-exact against the oracle, AUTO's choice there, and equal
+    to the W-wave walk on 2^29 sign vectors."""
+    rng = np.random.default_rng(970 + marg)
+    for R in (6, 12):
+        for C in (129, 130, 161, 200, 256, 257, 320, 384 + marg):
+            A = _rand(rng, R, C, 40)
+            print("table16 wide", marg, A.shape, flush=True)
+            res = _agree(pkg, A, 1, marg)
+            assert res["kernel_name"].startswith("table16<L"), res["kernel_name"]
+    from lnorm_amd import gen
+    A = gen.gen_matrix(30, 300, 60, 71 + marg)
+    t = pkg.calc_lnorm(A, 1, marg)
+    w = pkg.calc_lnorm(A, 1, marg, kernel="wide")
+    assert t[2]["kernel_name"].startswith("table16<L") and w[2]["kernel_name"].startswith("wide<")
+    assert t[:2] == w[:2] and t[2]["best_rank"] == w[2]["best_rank"]
 
 
 def test_table16_binary_equals_blocked_large(pkg):
