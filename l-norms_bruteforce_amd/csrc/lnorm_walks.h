@@ -610,12 +610,45 @@ __device__ __forceinline__ void blk16_tables_all(const uint32_t (&u)[D][NW], uin
   (blk16_table<Cs, D, NW, M>(u, T, neg), ...);
 }
 
+// Tables of the two labels a high step changes, C and C + 1, in one pass
+// over the subset table: each scalar word feeds two v_sad_u16 (one per
+// label), half the scalar loads of two single-label passes, and the two
+// accumulator chains are independent.
 template <int C, int D, int NW, int M>
-struct Blk16Table {
-  __device__ __forceinline__ static void run(int c, const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
+__device__ __forceinline__ void blk16_table_pair(const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
+                                                 const cuint32_t *neg) {
+  uint32_t U0 = 0, U1 = 0;
+#pragma unroll
+  for (int w = 0; w < NW; ++w) {
+    const uint32_t n = neg[w];
+    U0 = sad16(u[C][w], n, U0);
+    U1 = sad16(u[C + 1][w], n, U1);
+  }
+  T[C][0] = U0;
+  T[C + 1][0] = U1;
+  const uint32_t b0 = D <= 3 ? 0u : 0u - U0, b1 = D <= 3 ? 0u : 0u - U1;
+#pragma unroll
+  for (int S = 1; S < (1 << M); ++S) {
+    uint32_t a0 = b0, a1 = b1;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t n = neg[(size_t)S * NW + w];
+      a0 = sad16(u[C][w], n, a0);
+      a1 = sad16(u[C + 1][w], n, a1);
+    }
+    T[C][S] = a0;
+    T[C + 1][S] = a1;
+  }
+}
+
+template <int C, int D, int NW, int M>
+struct Blk16TablePair {
+  __device__ __forceinline__ static void run(int lo, const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
                                              const cuint32_t *neg) {
-    if (c == C) blk16_table<C, D, NW, M>(u, T, neg);
-    else if constexpr (C + 1 < D) Blk16Table<C + 1, D, NW, M>::run(c, u, T, neg);
+    if constexpr (C + 1 < D) {
+      if (lo == C) blk16_table_pair<C, D, NW, M>(u, T, neg);
+      else Blk16TablePair<C + 1, D, NW, M>::run(lo, u, T, neg);
+    }
   }
 };
 
@@ -718,16 +751,12 @@ __device__ __forceinline__ void walk_blk16(
       const int up = p ? 0 : 1;
       pm ^= (1u << j) - 1u;
       Move16<0, D, NW>::run(u, Ap + (size_t)(M + j) * NW, from, up);
-      // One copy of the table code for both labels.  The table's scalar
-      // loads are loop-invariant; hoisted out of the loop, 2^M * NW values
-      // would not fit in SGPRs and be spilled to VGPR lanes (one v_readlane
-      // per use), so the pointer is re-laundered for every label.
-#pragma unroll 1
-      for (int e = 0; e < 2; ++e) {
-        const cuint32_t *negk = neg;
-        asm volatile("" : "+s"(negk));
-        Blk16Table<0, D, NW, M>::run(e ? (up ? from + 1 : from - 1) : from, u, T, negk);
-      }
+      // The table's scalar loads are loop-invariant; hoisted out of the
+      // loop, 2^M * NW values would not fit in SGPRs and be spilled to VGPR
+      // lanes (one v_readlane per use), so the pointer is laundered here.
+      const cuint32_t *negk = neg;
+      asm volatile("" : "+s"(negk));
+      Blk16TablePair<0, D, NW, M>::run(up ? from : from - 1, u, T, negk);
       totU = 0;
 #pragma unroll
       for (int c = 0; c < D; ++c) totU += T[c][0];
