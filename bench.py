@@ -284,14 +284,17 @@ def l3_line(pkg, gen, steps: int, warmup: int, cpu: bool) -> dict:
     return line
 
 
-# rows the tabulated walk keeps in its subset tables (its width limit per d
-# in 16-bit words; d = 3 keeps 4 rows beyond 32 words)
-TABLE_ROWS = {2: 5, 3: 5, 4: 4, 5: 3, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
-TABLE_MAX_WORDS = {2: 64, 3: 48, 4: 32, 5: 32, 6: 24, 7: 16, 8: 16, 9: 16, 10: 16}
+# the tabulated walk's width limit per d (16-bit words; batched items: 32)
+# and the rows it keeps in its subset tables at a width
+TABLE_MAX_WORDS = {2: 32, 3: 32, 4: 32, 5: 32, 6: 32, 7: 32, 8: 24, 9: 24, 10: 16}
 
 
 def table16_rows(d: int, words: int) -> int:
-    return 4 if (d <= 3 and words > 32) else TABLE_ROWS[d]
+    if d <= 3:
+        return 5 if words <= 32 else 4
+    return {4: 4 if words <= 32 else 3 if words <= 40 else 2, 5: 3 if words <= 32 else 2,
+            6: 3 if words <= 24 else 2, 7: 2 if words <= 24 else 1, 8: 2, 9: 2 if words <= 16 else 1,
+            10: 2}[d]
 
 
 def table16_ops(d: int, words: int) -> float:

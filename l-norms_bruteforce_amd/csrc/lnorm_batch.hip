@@ -221,8 +221,8 @@ template <int D>
 static hipError_t lb_t16_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
   if (blk16_words(D, b.width) * 2 != b.width) return hipErrorInvalidValue;
   switch (b.width / 2) {
-#define X(n) case n: if constexpr (n <= dary16_max_words(D)) return lb_t16<D, n>(mode, b, st); break;
-    X(8) X(16) X(24) X(32) X(40) X(48) X(64)
+#define X(n) case n: if constexpr (n <= blk16_max_words(D)) return lb_t16<D, n>(mode, b, st); break;
+    X(8) X(16) X(24) X(32)
 #undef X
     default: break;
   }

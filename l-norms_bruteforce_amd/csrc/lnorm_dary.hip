@@ -14,11 +14,11 @@ __global__ __launch_bounds__(256) void k_blk16_dary(
                                                   gbest, gfirst, global_wave());
 }
 
-static const int kBlkWords[] = {8, 16, 24, 32, 40, 48, 64};
+static const int kBlkWords[] = {8, 16, 24, 32, 40, 48, 56, 64, 80, 96};
 
 int blk16_words(int d, int ny) {
   const int need = (ny + 1) / 2;
-  const int mx = dary16_max_words(d);
+  const int mx = blk16_max_words(d);
   if (d < 2 || d > 10) return 0;
   for (int w : kBlkWords) if (w >= need && w <= mx) return w;
   return 0;
@@ -45,8 +45,8 @@ static hipError_t launch_blk_nw(Mode mode, const PlainArgs &a, hipStream_t st, i
 template <int D>
 static hipError_t launch_blk(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
   switch (a.ny_pad / 2) {
-#define X(n) case n: if constexpr (n <= dary16_max_words(D)) return launch_blk_nw<D, n>(mode, a, st, block); break;
-    X(8) X(16) X(24) X(32) X(40) X(48) X(64)
+#define X(n) case n: if constexpr (n <= blk16_max_words(D)) return launch_blk_nw<D, n>(mode, a, st, block); break;
+    X(8) X(16) X(24) X(32) X(40) X(48) X(56) X(64) X(80) X(96)
 #undef X
     default: break;
   }

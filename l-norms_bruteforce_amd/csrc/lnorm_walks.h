@@ -519,13 +519,31 @@ __device__ __forceinline__ void walk_p16(
 // ---------------------------------------------------------------------------
 constexpr int ipow_c(int b, int e) { return e == 0 ? 1 : b * ipow_c(b, e - 1); }
 
-// Rows tabulated: d^M labelings per high step against 2^(M+1) table sums
-// per state word; d * NW state plus d * 2^M table registers stay below 220.
-// (d = 2: both labels change at every high step, 2 sums per word and
-// labeling whatever M.)
+// Widest instantiation per d (16-bit words), and the rows tabulated at a
+// width: d^M labelings per high step against 2^(M+1) table sums per state
+// word, with the d * NW state and the d * 2^M table held in registers.
+// Every (d, NW, M) below compiles without VGPR spills (checked in the
+// register report of the build); M shrinks as the state grows.  (d = 2:
+// both labels change at every high step, 2 sums per word and labeling
+// whatever M.)
+constexpr int blk16_max_words(int d) {
+  return d == 2 ? 96 : d == 3 ? 64 : d == 4 ? 48 : d == 5 ? 40 : (d == 6 || d == 7) ? 32
+                                              : (d == 8 || d == 9) ? 24 : d == 10 ? 16 : 0;
+}
 constexpr int blk16_low_rows(int d, int nw) {
-  return (d == 2 || d == 3) ? (nw <= 32 ? 5 : 4) : d == 4 ? 4 : (d == 5 || d == 6) ? 3
-                                                     : (d >= 7 && d <= 10) ? 2 : 0;
+  if (nw > blk16_max_words(d)) return 0;
+  switch (d) {
+    case 2: return nw <= 32 ? 5 : 4;
+    case 3: return nw <= 32 ? 5 : 4;
+    case 4: return nw <= 32 ? 4 : nw <= 40 ? 3 : 2;
+    case 5: return nw <= 32 ? 3 : 2;
+    case 6: return nw <= 24 ? 3 : 2;
+    case 7: return nw <= 24 ? 2 : 1;
+    case 8: return 2;
+    case 9: return nw <= 16 ? 2 : 1;
+    case 10: return 2;
+    default: return 0;
+  }
 }
 
 template <int D, int M>
@@ -718,11 +736,19 @@ __device__ __forceinline__ void walk_blk16(
       lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
       x /= (uint64_t)D;
     }
+    // the lane's label of row M + i as a mask per label: `(lab == c) ? row
+    // : 0` here (a lane-varying select per state word) made the compiler
+    // hold a second copy of the state, about twice the registers
     const cuint32_t *row = Ap + (size_t)(M + i) * NW;
+    uint32_t mask[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c)
+    for (int c = 0; c < D; ++c) mask[c] = (lab == c) ? 0xFFFFFFFFu : 0u;
 #pragma unroll
-      for (int l = 0; l < NW; ++l) u[c][l] = pk_add(u[c][l], (lab == c) ? row[l] : 0u);
+    for (int l = 0; l < NW; ++l) {
+      const uint32_t v = row[l];
+#pragma unroll
+      for (int c = 0; c < D; ++c) u[c][l] = pk_add(u[c][l], v & mask[c]);
+    }
   }
   uint32_t T[D][1 << M];
   blk16_tables_all<D, NW, M>(u, T, neg, std::make_integer_sequence<int, D>{});

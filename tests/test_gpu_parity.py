@@ -303,8 +303,11 @@ def test_wide16_walk_vs_oracle(pkg, d, R, cols):
     for C in cols:
         A = _rand(rng, R, C, 60)
         print("wide16", d, A.shape, flush=True)
-        res = _agree(pkg, A, d, 0)
+        res = _agree(pkg, A, d, 0, kernel="wide")
         assert res["kernel_name"].startswith("wide16<"), res["kernel_name"]
+    # beyond the tabulated walk's widths AUTO takes the W-wave walk
+    A = _rand(rng, R, cols[-1], 60)
+    assert _agree(pkg, A, d, 0)["kernel_name"].startswith("wide16<")
     A = _rand(rng, R, cols[1], 60)
     for q in (2, 3, 5):   # q = 5 at d >= 9 (5 enumerated rows): one segment
         print("wide16 seg_log", d, A.shape, q, flush=True)
@@ -315,14 +318,14 @@ def test_wide16_walk_vs_oracle(pkg, d, R, cols):
 # (d, rows, column counts) for the tabulated walk: every instantiated width
 # (8..48 words) and odd column counts; rows leave several high digits
 TABLE16 = [
-    (2, 15, [1, 5, 16, 33, 64, 65, 100, 128]),
-    (3, 12, [1, 5, 16, 17, 33, 48, 64, 65, 80, 96]),
-    (4, 10, [3, 16, 31, 48, 64]),
-    (5, 8, [7, 16, 33, 48, 64]),
-    (6, 7, [9, 16, 32, 48]),
-    (7, 6, [5, 16, 32]),
-    (8, 6, [5, 16, 32]),
-    (9, 5, [5, 16, 32]),
+    (2, 15, [1, 5, 16, 33, 64, 65, 100, 128, 129, 160, 192]),
+    (3, 12, [1, 5, 16, 17, 33, 48, 64, 65, 80, 96, 97, 112, 128]),
+    (4, 10, [3, 16, 31, 48, 64, 65, 80, 96]),
+    (5, 8, [7, 16, 33, 48, 64, 65, 80]),
+    (6, 7, [9, 16, 32, 48, 49, 64]),
+    (7, 6, [5, 16, 32, 33, 48, 64]),
+    (8, 6, [5, 16, 32, 33, 48]),
+    (9, 5, [5, 16, 32, 33, 48]),
     (10, 5, [5, 16, 32]),
 ]
 
