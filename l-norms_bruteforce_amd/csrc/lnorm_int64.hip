@@ -233,11 +233,18 @@ __global__ __launch_bounds__(256) void k_plain64_dary(const int64_t *__restrict_
       lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
       x /= (uint64_t)D;
     }
+    // per-label lane masks, as seed_add32 (a lane-varying select per state
+    // word doubles the registers)
     const cint64_t *row = A + (size_t)i * NY;
+    uint64_t mask[D];
 #pragma unroll
-    for (int c = 0; c < D; ++c)
+    for (int c = 0; c < D; ++c) mask[c] = (lab == c) ? ~0ull : 0ull;
 #pragma unroll
-      for (int l = 0; l < NY; ++l) t[c][l] += (lab == c) ? (uint64_t)row[l] : 0ull;
+    for (int l = 0; l < NY; ++l) {
+      const uint64_t v = (uint64_t)row[l];
+#pragma unroll
+      for (int c = 0; c < D; ++c) t[c][l] += v & mask[c];
+    }
   }
   int64_t S[D];
 #pragma unroll

@@ -193,6 +193,24 @@ struct Digit0Run32 {
   }
 };
 
+// Seeding a lane-per-segment d-ary walk: row `row` joins label `lab`, which
+// differs per lane.  Each state word adds the scalar row word ANDed with a
+// per-label lane mask.  Written as `(lab == c) ? row : 0` per state word (a
+// lane-varying select), the compiler held a second copy of the state, about
+// twice the registers, and spilled the wider shapes.
+template <int D, int NY>
+__device__ __forceinline__ void seed_add32(uint32_t (&t)[D][NY], const cint32_t *row, int lab) {
+  uint32_t mask[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) mask[c] = (lab == c) ? 0xFFFFFFFFu : 0u;
+#pragma unroll
+  for (int l = 0; l < NY; ++l) {
+    const uint32_t v = (uint32_t)row[l];
+#pragma unroll
+    for (int c = 0; c < D; ++c) t[c][l] += v & mask[c];
+  }
+}
+
 template <int D, int NY, int MODE>
 __device__ __forceinline__ void walk_dary(
     const cint32_t *A, int R, int q, uint64_t seg0, uint64_t nseg, int32_t target,
@@ -229,11 +247,7 @@ __device__ __forceinline__ void walk_dary(
       lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
       x /= (uint64_t)D;
     }
-    const cint32_t *row = A + (size_t)i * NY;
-#pragma unroll
-    for (int c = 0; c < D; ++c)
-#pragma unroll
-      for (int l = 0; l < NY; ++l) t[c][l] += (lab == c) ? (uint32_t)row[l] : 0u;
+    seed_add32<D, NY>(t, A + (size_t)i * NY, lab);
   }
   uint32_t S[D];
 #pragma unroll
@@ -324,6 +338,20 @@ __device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) {
   return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
 }
 
+// The packed counterpart of seed_add32.
+template <int D, int NW>
+__device__ __forceinline__ void seed_add16(uint32_t (&t)[D][NW], const cuint32_t *row, int lab) {
+  uint32_t mask[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) mask[c] = (lab == c) ? 0xFFFFFFFFu : 0u;
+#pragma unroll
+  for (int l = 0; l < NW; ++l) {
+    const uint32_t v = row[l];
+#pragma unroll
+    for (int c = 0; c < D; ++c) t[c][l] = pk_add(t[c][l], v & mask[c]);
+  }
+}
+
 template <int F, int T, int D, int NW>
 __device__ __forceinline__ void apply_move16(uint32_t (&t)[D][NW], uint32_t (&S)[D],
                                              const cuint32_t *row, uint32_t B) {
@@ -407,11 +435,7 @@ __device__ __forceinline__ void walk_p16(
       lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
       x /= (uint64_t)D;
     }
-    const cuint32_t *row = Ap + (size_t)i * NW;
-#pragma unroll
-    for (int c = 0; c < D; ++c)
-#pragma unroll
-      for (int l = 0; l < NW; ++l) t[c][l] = pk_add(t[c][l], (lab == c) ? row[l] : 0u);
+    seed_add16<D, NW>(t, Ap + (size_t)i * NW, lab);
   }
   uint32_t S[D];
 #pragma unroll
@@ -736,19 +760,7 @@ __device__ __forceinline__ void walk_blk16(
       lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
       x /= (uint64_t)D;
     }
-    // the lane's label of row M + i as a mask per label: `(lab == c) ? row
-    // : 0` here (a lane-varying select per state word) made the compiler
-    // hold a second copy of the state, about twice the registers
-    const cuint32_t *row = Ap + (size_t)(M + i) * NW;
-    uint32_t mask[D];
-#pragma unroll
-    for (int c = 0; c < D; ++c) mask[c] = (lab == c) ? 0xFFFFFFFFu : 0u;
-#pragma unroll
-    for (int l = 0; l < NW; ++l) {
-      const uint32_t v = row[l];
-#pragma unroll
-      for (int c = 0; c < D; ++c) u[c][l] = pk_add(u[c][l], v & mask[c]);
-    }
+    seed_add16<D, NW>(u, Ap + (size_t)(M + i) * NW, lab);
   }
   uint32_t T[D][1 << M];
   blk16_tables_all<D, NW, M>(u, T, neg, std::make_integer_sequence<int, D>{});

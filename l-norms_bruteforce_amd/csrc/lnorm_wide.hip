@@ -191,11 +191,7 @@ __global__ __launch_bounds__(64 * W) void k_wide_p16(
       lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
       x /= (uint64_t)D;
     }
-    const cuint32_t *row = Ap + (size_t)i * NWT + wbase;
-#pragma unroll
-    for (int c = 0; c < D; ++c)
-#pragma unroll
-      for (int l = 0; l < NW; ++l) t[c][l] = pk_add(t[c][l], (lab == c) ? row[l] : 0u);
+    seed_add16<D, NW>(t, Ap + (size_t)i * NWT + wbase, lab);
   }
   const uint32_t zero = opaque_zero();
   uint32_t S[D];
