@@ -43,9 +43,10 @@ extern "C" {
 #define LNORM_KERNEL_WIDE 5     /* W waves split the columns: n <= 2 at 129..1024 columns (L2: ..512);
                                    n = 3..10 packed, column sums <= 32767, up to 640 (d = 3) .. 192 (d >= 9) */
 #define LNORM_KERNEL_TABLE16 6  /* packed walk over the high rows; the 1..5 fastest rows are tabulated:
-                                   per-label subset tables for n = 2..10 (column sums <= 32767; up to 192
-                                   columns at d = 2 ... 32 at d = 10), sign-pattern tables for n = 1
-                                   (|.| column sums <= 32767, up to 384 columns) */
+                                   per-label subset tables for n = 2..10 (column sums <= 32767; one wave
+                                   up to 192 columns at d = 2 ... 32 at d = 10, W waves up to 512 at
+                                   d <= 6 ... 128 at d = 10), sign-pattern tables for n = 1 (|.| column
+                                   sums <= 32767, up to 384 columns) */
 
 typedef struct lnorm_hip_opts {
     int32_t kernel;             /* LNORM_KERNEL_* */

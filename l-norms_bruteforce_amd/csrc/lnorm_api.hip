@@ -190,7 +190,8 @@ struct Geometry {
 };
 
 // The lane- or wave-per-segment walks.
-enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2, WALK_WIDE = 3, WALK_WIDE16 = 4, WALK_TAB16 = 5 };
+enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2, WALK_WIDE = 3, WALK_WIDE16 = 4, WALK_TAB16 = 5,
+            WALK_TAB16W = 6 };
 
 // Segments for a range of base^e ranks: aim at about 2^target_log lanes.
 // The step counter of the d-ary and packed walks holds 15 base-d digits.
@@ -272,7 +273,7 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
     pp.qmin = M;
     return;
   }
-  if (walk == WALK_PLAIN16 || walk == WALK_WIDE16 || walk == WALK_TAB16) {
+  if (walk == WALK_PLAIN16 || walk == WALK_WIDE16 || walk == WALK_TAB16 || walk == WALK_TAB16W) {
     // two columns per word, low half = even column
     const int NW = ny_pad / 2;
     pp.A.assign((size_t)p.R * NW, 0);
@@ -281,10 +282,10 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
         const uint32_t v = (uint32_t)p.A[(size_t)i * p.C + l] & 0xFFFFu;
         pp.A[(size_t)i * NW + l / 2] |= (int32_t)(v << (16 * (l & 1)));
       }
-    if (walk == WALK_TAB16) {
+    if (walk == WALK_TAB16 || walk == WALK_TAB16W) {
       // subset table of the M fastest rows: word w of subset S holds
       // -sum_{i in S} A[i][col] + 2^15 for its two columns
-      const int M = blk16_rows(p.n, NW);
+      const int M = walk == WALK_TAB16W ? wt16_rows(p.n, NW) : blk16_rows(p.n, NW);
       pp.qmin = M;
       pp.D.assign((size_t)NW << M, 0);
       for (int S = 0; S < (1 << M); ++S)
@@ -323,6 +324,7 @@ hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const Pl
     case WALK_WIDE: return launch_wide(p.kind, mode, a, st);
     case WALK_WIDE16: return launch_wide16(mode, a, st);
     case WALK_TAB16: return p.n == 1 ? launch_tab16_bin(p.kind, mode, a, st, block) : launch_blk16(mode, a, st, block);
+    case WALK_TAB16W: return launch_wide_t16(mode, a, st);
     default: return launch_general(p.kind, mode, a, p.C, st);
   }
 }
@@ -662,7 +664,7 @@ int select_device(const lnorm_hip_opts &o, int &dev) {
 struct Choice {
   int kernel = LNORM_KERNEL_AUTO;
   BlockedPlan bp;
-  int ny_plain = 0, w16 = 0, ny_wide = 0, w16_wide = 0, wtab = 0;
+  int ny_plain = 0, w16 = 0, ny_wide = 0, w16_wide = 0, wtab = 0, wtabw = 0;
 };
 
 int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
@@ -683,6 +685,11 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
   } else {
     c.wtab = (fits16 && p.n >= 2) ? blk16_words(p.n, p.C) : 0;
     if (c.wtab && p.R - 1 < blk16_rows(p.n, c.wtab)) c.wtab = 0;   // fewer enumerated rows than table rows
+    // beyond one wave's tables: the W-wave tabulated walk
+    if (fits16 && p.n >= 2 && !blk16_words(p.n, p.C)) {
+      c.wtabw = wt16_words(p.n, p.C);
+      if (c.wtabw && p.R - 1 < wt16_rows(p.n, c.wtabw)) c.wtabw = 0;
+    }
   }
   if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
     return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
@@ -691,7 +698,7 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
   if (kernel == LNORM_KERNEL_WIDE && !c.ny_wide && !c.w16_wide)
     return fail(LNORM_E_UNSUPPORTED, "W-wave walk needs an instantiated width (%d columns at n = %d) and, for "
                 "n <= 2, at least 3 enumerated rows, for n >= 3, column sums <= 32767", p.C, p.n);
-  if (kernel == LNORM_KERNEL_TABLE16 && !c.wtab)
+  if (kernel == LNORM_KERNEL_TABLE16 && !c.wtab && !c.wtabw)
     return fail(LNORM_E_UNSUPPORTED, "tabulated packed walk needs |.| column sums <= 32767, an "
                 "instantiated width and more enumerated rows than table rows (%d x %d at n = %d)", p.R, p.C, p.n);
   if (kernel == LNORM_KERNEL_PLAIN16 && !c.w16)
@@ -703,7 +710,7 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
     // high steps: 2^29 ran 27x slower), else the tabulated walk, else the
     // 32-bit blocked kernel
     kernel = (can_block && !c.bp.wide && (p.total >= (1ull << 33) || !c.wtab)) ? LNORM_KERNEL_BLOCKED
-                       : c.wtab ? LNORM_KERNEL_TABLE16
+                       : (c.wtab || c.wtabw) ? LNORM_KERNEL_TABLE16
                        : can_block ? LNORM_KERNEL_BLOCKED
                        : (c.w16 ? LNORM_KERNEL_PLAIN16
                                 : (c.ny_plain ? LNORM_KERNEL_PLAIN
@@ -717,7 +724,8 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
 
 void prepare_for(const Problem &p, const Choice &c, Prepared &pp) {
   if (c.kernel == LNORM_KERNEL_PLAIN16) prepare_walk(p, WALK_PLAIN16, 2 * c.w16, pp);
-  else if (c.kernel == LNORM_KERNEL_TABLE16) prepare_walk(p, WALK_TAB16, 2 * c.wtab, pp);
+  else if (c.kernel == LNORM_KERNEL_TABLE16 && c.wtab) prepare_walk(p, WALK_TAB16, 2 * c.wtab, pp);
+  else if (c.kernel == LNORM_KERNEL_TABLE16) prepare_walk(p, WALK_TAB16W, 2 * c.wtabw, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE && c.w16_wide) prepare_walk(p, WALK_WIDE16, 2 * c.w16_wide, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
   else if (c.kernel == LNORM_KERNEL_BLOCKED && !c.ny_plain && c.ny_wide) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
@@ -768,7 +776,7 @@ TuneKey tune_key(int dev, const Problem &p, const Choice &c) {
   } else if (c.kernel == LNORM_KERNEL_PLAIN16) {
     k.width = c.w16;
   } else if (c.kernel == LNORM_KERNEL_TABLE16) {
-    k.width = c.wtab;
+    k.width = c.wtab ? c.wtab : 1000 + c.wtabw;
   } else {
     k.width = c.ny_plain;
   }
@@ -942,6 +950,9 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
                p.kind == KIND_LM ? "LM" : "L1", pp.ny_pad / 2, pp.qmin, g.q);
     else if (pp.walk == WALK_TAB16)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16<d=%d,words=%d,M=%d,q=%d>", p.n, pp.ny_pad / 2,
+               pp.qmin, g.q);
+    else if (pp.walk == WALK_TAB16W)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16w<d=%d,words=%d,M=%d,q=%d>", p.n, pp.ny_pad / 2,
                pp.qmin, g.q);
     else
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain<kind=%d,d=%d,ny=%d,q=%d>", (int)p.kind, p.n,

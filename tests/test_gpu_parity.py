@@ -270,8 +270,8 @@ def test_wide_walk_vs_oracle(pkg, n, marg, R, cols):
     for C in cols:
         A = _rand(rng, R, C, 40)
         print("wide", n, marg, A.shape, flush=True)
-        # AUTO takes the tabulated walk up to 384 columns with 16-bit sums
-        res = _agree(pkg, A, n, marg, **({"kernel": "wide"} if (n == 1 and C - marg <= 384 and R > 5) else {}))
+        # forced: AUTO takes the tabulated walks where the sums fit 16 bits
+        res = _agree(pkg, A, n, marg, kernel="wide")
         assert res["kernel_name"].startswith("wide<"), res["kernel_name"]
     # narrower problems on the forced W-wave walk (AUTO keeps them elsewhere)
     A = _rand(rng, R, 65, 40)
@@ -305,9 +305,10 @@ def test_wide16_walk_vs_oracle(pkg, d, R, cols):
         print("wide16", d, A.shape, flush=True)
         res = _agree(pkg, A, d, 0, kernel="wide")
         assert res["kernel_name"].startswith("wide16<"), res["kernel_name"]
-    # beyond the tabulated walk's widths AUTO takes the W-wave walk
+    # at these widths AUTO takes a W-wave walk (tabulated where its widths
+    # reach)
     A = _rand(rng, R, cols[-1], 60)
-    assert _agree(pkg, A, d, 0)["kernel_name"].startswith("wide16<")
+    assert _agree(pkg, A, d, 0)["kernel_name"].startswith(("table16w<", "wide16<"))
     A = _rand(rng, R, cols[1], 60)
     for q in (2, 3, 5):   # q = 5 at d >= 9 (5 enumerated rows): one segment
         print("wide16 seg_log", d, A.shape, q, flush=True)
@@ -421,6 +422,59 @@ def test_table16_binary_equals_blocked_large(pkg):
         assert oracle.objective(A, 1, marg, t[1]) == t[0]
 
 
+# (d, rows, column counts) for the W-wave tabulated walk: beyond one wave's
+# tables, at 2, 4 and 8 waves (4 at most for d >= 7)
+WIDE_T16 = [
+    (2, 12, [193, 200, 256, 257, 400, 512]),
+    (3, 10, [129, 130, 256, 300, 512]),
+    (4, 9, [97, 128, 200, 256, 512]),
+    (5, 8, [81, 128, 129, 256, 512]),
+    (6, 7, [65, 128, 200, 512]),
+    (7, 7, [65, 128, 129, 256]),
+    (8, 6, [49, 96, 97, 192]),
+    (9, 6, [49, 96, 192]),
+    (10, 6, [33, 64, 65, 128]),
+]
+
+
+@pytest.mark.parametrize("d,R,cols", WIDE_T16)
+def test_table16w_walk_vs_oracle(pkg, d, R, cols):
+    """The tabulated walk over W waves (partial subset tables summed through
+    LDS, the block's labelings split over the waves): bit-exact norm,
+    strategy and rank at every wave count, at forced segment lengths and
+    with the fewest enumerated rows it accepts."""
+    rng = np.random.default_rng(990 + d)
+    for C in cols:
+        A = _rand(rng, R, C, 50)
+        print("table16w", d, A.shape, flush=True)
+        res = _agree(pkg, A, d, 0)
+        assert res["kernel_name"].startswith("table16w<"), res["kernel_name"]
+    A = _rand(rng, R, cols[1], 50)
+    M = int(pkg.calc_lnorm(A, d, 0)[2]["kernel_name"].split("M=")[1].split(",")[0])
+    for q in (M, M + 1, M + 2):
+        if q > R - 1:
+            continue
+        print("table16w seg_log", d, A.shape, q, flush=True)
+        res = _agree(pkg, A, d, 0, seg_log=q, kernel="table16")
+        assert res["kernel_name"].startswith("table16w<") and f",q={q}>" in res["kernel_name"], res["kernel_name"]
+    A = _rand(rng, M + 1, cols[1], 50)
+    assert _agree(pkg, A, d, 0)["kernel_name"].startswith("table16w<")
+
+
+def test_table16w_equals_wide16_larger(pkg):
+    """Larger problems: the tabulated and packed W-wave walks agree on norm,
+    strategy and rank, and the strategy's objective is the norm."""
+    from lnorm_amd import gen
+    for (R, C, d) in ((16, 300, 3), (12, 400, 4), (11, 150, 5), (9, 100, 9)):
+        A = gen.gen_matrix(R, C, 50, 50 + d)
+        print("table16w vs wide16", R, C, d, flush=True)
+        t = pkg.calc_lnorm(A, d, 0)
+        w = pkg.calc_lnorm(A, d, 0, kernel="wide")
+        assert t[2]["kernel_name"].startswith("table16w<") and w[2]["kernel_name"].startswith("wide16<")
+        assert t[:2] == w[:2] and t[2]["best_rank"] == w[2]["best_rank"], (R, C, d)
+        assert oracle.objective(A, d, 0, t[1]) == t[0]
+
+
 def test_table16_equals_plain16_larger(pkg):
     """At sizes the oracle does not reach: the tabulated walk and the packed
     walk agree on norm, strategy and rank (3^23 labelings for the 24 x 32 L3
@@ -441,7 +495,7 @@ def test_wide16_equals_general_larger(pkg):
     for (R, C, d) in ((16, 300, 3), (11, 150, 5), (8, 100, 9)):
         A = gen.gen_matrix(R, C, 50, 30 + d)
         print("wide16 vs general", R, C, d, flush=True)
-        w = pkg.calc_lnorm(A, d, 0)
+        w = pkg.calc_lnorm(A, d, 0, kernel="wide")
         g = pkg.calc_lnorm(A, d, 0, kernel="general")
         assert w[2]["kernel_name"].startswith("wide16<"), w[2]["kernel_name"]
         assert w[:2] == g[:2] and w[2]["best_rank"] == g[2]["best_rank"]
