@@ -832,7 +832,7 @@ int tune_single(DevCtx &cx, const Problem &p, const Choice &c, const lnorm_hip_o
       best.s = 0;
     }
   } else if (c.kernel == LNORM_KERNEL_PLAIN || c.kernel == LNORM_KERNEL_PLAIN16 ||
-             c.kernel == LNORM_KERNEL_TABLE16) {
+             (c.kernel == LNORM_KERNEL_TABLE16 && c.wtab)) {   // the W-wave walks' workgroup is fixed
     Prepared pp;
     prepare_for(p, c, pp);
     if (int r = upload_walk(cx, p, pp, st)) return r;
