@@ -67,6 +67,11 @@ def valu_ops_per_unit(kernel: str, ny: int, marginal: int, P: int = 2) -> float:
     if kernel == "table16":          # tabulated walk: one v_sad_u16 per two |.| columns, half a v_max3,
         words = (ny - marginal + 1) // 2    # the marginal add, and the high step's packed add per 32
         return words + 0.5 + marginal + words / 32.0
+    if kernel == "table16w":         # over W waves: the padded words' v_sad_u16, the high step per 2^M
+        words = (ny - marginal + 1) // 2    # patterns, the W partials' v_add3 and the running maximum
+        nwt = 256 if words <= 256 else 512 if words <= 512 else 1024
+        w, m = (8, 5) if nwt == 256 else (16, 4)
+        return nwt + nwt / 2 ** m + w / 2 + 1 + marginal
     return 2 * ny + 1
 
 
@@ -204,8 +209,9 @@ def cpu_baseline(gen, rows: int = 64, r: int = 127, seed: int = 104, budget_cols
 def single_line(pkg, A, n: int, total: int, ops, int64: bool = False, kernel: str = "auto") -> dict:
     """One call (after a warm-up call): labelings/s, the kernel that ran and,
     with an op count per unit, the VALU fraction by that model.  Used for the
-    wide L1 matrix (36 x 256, 2^35 sign vectors) on AUTO's tabulated walk and
-    on the W-wave walk (2 ops per row-update), and for the 64-bit walk (L1 of
+    wide L1 matrices (36 x 256, 2^35 sign vectors; 34 x 1000, 2^33) on AUTO's
+    tabulated walks and on the binary W-wave walk (2 ops per row-update), and
+    for the 64-bit walk (L1 of
     a 34 x 40 matrix scaled by 2^28 + 3, beyond int32: 2^33 sign vectors)."""
     calc = pkg.calc_lnorm64 if int64 else pkg.calc_lnorm
     calc(A, n, 0, kernel=kernel)
@@ -553,6 +559,15 @@ def main():
             raise RuntimeError("tabulated and W-wave walks disagree on 36x256")
         line["wide_36x256"] = {**wt, "wwave": {k2: ww[k2] for k2 in ("value", "kernel", "kernel_ms", "valu_frac")},
                                "speedup_vs_wwave": wt["value"] / ww["value"]}
+        # L1 of a 34 x 1000 matrix (2^33 sign vectors): the tabulated walk over
+        # 16 waves beside the binary W-wave walk
+        X = gen.gen_matrix(34, 1000, 30, 109)
+        xt = single_line(pkg, X, 1, 2 ** 33, valu_ops_per_unit("table16w", 1000, 0))
+        xw = single_line(pkg, X, 1, 2 ** 33, 2 * 1024 + 1, kernel="wide")
+        if xt["norm"] != xw["norm"]:
+            raise RuntimeError("tabulated and binary W-wave walks disagree on 34x1000")
+        line["wide_34x1000"] = {**xt, "wwave": {k2: xw[k2] for k2 in ("value", "kernel", "kernel_ms", "valu_frac")},
+                                "speedup_vs_wwave": xt["value"] / xw["value"]}
         line["int64_34x40"] = single_line(pkg, gen.gen_matrix(34, 40, 100, 108).astype("int64") * (2 ** 28 + 3),
                                           1, 2 ** 33, None, int64=True)
         line["L3_24x32"] = l3_line(pkg, gen, a.steps, a.warmup, not a.no_cpu)

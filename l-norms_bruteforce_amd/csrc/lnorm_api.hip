@@ -236,12 +236,12 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
   pp.ny_pad = walk == WALK_GENERAL ? 0 : ny_pad;
   pp.qmin = 0;
   pp.D.clear();
-  if (walk == WALK_TAB16 && p.n == 1) {
-    // L1 / LM sign-pattern table (walk_tab16_bin): A = R x NW packed |.|
-    // columns, R x NW packed 2A, (LM) R column-0 entries; D = 2^M x NW words
-    // -A_S + 2^15, (LM) 2^M column-0 sums A_S,0
+  if ((walk == WALK_TAB16 || walk == WALK_TAB16W) && p.n == 1) {
+    // L1 / LM sign-pattern table (walk_tab16_bin, k_wide_tb16): A = R x NW
+    // packed |.| columns, R x NW packed 2A, (LM) R column-0 entries; D = 2^M x
+    // NW words -A_S + 2^15, (LM) 2^M column-0 sums A_S,0
     const int marg = p.kind == KIND_LM ? 1 : 0;
-    const int NW = ny_pad / 2, M = tab_bin_rows(NW);
+    const int NW = ny_pad / 2, M = walk == WALK_TAB16W ? wtb16_rows(NW) : tab_bin_rows(NW);
     auto pack = [&](int32_t *dst, int i, int f) {
       for (int l = marg; l < p.C; ++l) {
         const uint32_t v = (uint32_t)(f * p.A[(size_t)i * p.C + l]) & 0xFFFFu;
@@ -324,7 +324,7 @@ hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const Pl
     case WALK_WIDE: return launch_wide(p.kind, mode, a, st);
     case WALK_WIDE16: return launch_wide16(mode, a, st);
     case WALK_TAB16: return p.n == 1 ? launch_tab16_bin(p.kind, mode, a, st, block) : launch_blk16(mode, a, st, block);
-    case WALK_TAB16W: return launch_wide_t16(mode, a, st);
+    case WALK_TAB16W: return p.n == 1 ? launch_wide_tb16(p.kind, mode, a, st) : launch_wide_t16(mode, a, st);
     default: return launch_general(p.kind, mode, a, p.C, st);
   }
 }
@@ -682,6 +682,11 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
     const int marg = p.kind == KIND_LM ? 1 : 0;
     c.wtab = max_col_abs_sum(p, marg) <= 32767 ? tab16_bin_words(p.C - marg) : 0;
     if (c.wtab && p.R - 1 < tab_bin_rows(c.wtab)) c.wtab = 0;
+    // beyond one wave's table: the W-wave tabulated walk
+    if (max_col_abs_sum(p, marg) <= 32767 && !tab16_bin_words(p.C - marg)) {
+      c.wtabw = wtb16_words(p.C - marg);
+      if (c.wtabw && p.R - 1 < wtb16_rows(c.wtabw)) c.wtabw = 0;
+    }
   } else {
     c.wtab = (fits16 && p.n >= 2) ? blk16_words(p.n, p.C) : 0;
     if (c.wtab && p.R - 1 < blk16_rows(p.n, c.wtab)) c.wtab = 0;   // fewer enumerated rows than table rows
@@ -951,6 +956,9 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
     else if (pp.walk == WALK_TAB16)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16<d=%d,words=%d,M=%d,q=%d>", p.n, pp.ny_pad / 2,
                pp.qmin, g.q);
+    else if (pp.walk == WALK_TAB16W && p.n == 1)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16w<%s,words=%d,M=%d,q=%d>",
+               p.kind == KIND_LM ? "LM" : "L1", pp.ny_pad / 2, pp.qmin, g.q);
     else if (pp.walk == WALK_TAB16W)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16w<d=%d,words=%d,M=%d,q=%d>", p.n, pp.ny_pad / 2,
                pp.qmin, g.q);

@@ -135,6 +135,9 @@ int blk16_rows(int d, int words);     // its tabulated rows M
 int wt16_words(int d, int ny);        // the W-wave tabulated walk: width in words (0: none)
 int wt16_rows(int d, int words);      // its tabulated rows M
 hipError_t launch_wide_t16(Mode mode, const PlainArgs &a, hipStream_t st);
+int wtb16_words(int nabs);            // the W-wave tabulated L1 / LM walk: width in words (0: none)
+int wtb16_rows(int words);            // its tabulated rows M
+hipError_t launch_wide_tb16(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st);
 hipError_t launch_blk16(Mode mode, const PlainArgs &a, hipStream_t st, int block);
 int tab16_bin_words(int nabs);        // tabulated L1/LM walk: width in words for nabs |.| columns (0: none)
 // its table rows: the sign patterns of rows 0..M-1; 4 beyond 64 words keeps

@@ -266,6 +266,217 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
   }
 }
 
+// ---------------------------------------------------------------------------
+// The tabulated L1 / LM walk (walk_tab16_bin) over W waves: wave wv keeps
+// words [wv*NW, (wv+1)*NW) of the high sum u (LM: column 0, outside the
+// words, in every wave; wave 0 adds it).  Per block each wave computes the
+// 2^M patterns' partial sums over its words and stores them in LDS; after a
+// barrier wave wv totals the W partials of its patterns (l = wv, wv + W, ...)
+// and keeps their maximum (MODE_MAX) or first match (MODE_FIRST).
+// Shapes (W, NW): (8, 32), (16, 32), (16, 64) for 512, 1024, 2048 columns.
+// ---------------------------------------------------------------------------
+constexpr int wtb_m(int w) { return w >= 16 ? 4 : 5; }   // partials: W * 2^M * 256 B = 64 KiB
+
+template <int NW, int NWT, int M, bool MARG, int... Ss>
+__device__ __forceinline__ void wtb_partials(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
+                                             const cint32_t *pos0, bool wave0, uint32_t *p, int lane,
+                                             std::integer_sequence<int, Ss...>) {
+  auto one = [&](auto SS) {
+    constexpr int S = decltype(SS)::value;
+    uint32_t acc = (MARG && wave0) ? (uint32_t)(u0 + pos0[S]) : 0u;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) acc = sad16(u[w], neg[(size_t)S * NWT + w], acc);
+    p[S * 64 + lane] = acc;
+  };
+  (one(std::integral_constant<int, Ss>{}), ...);
+}
+
+template <int W, int M, int MODE, bool ODD, int WV, int... Is>
+__device__ __forceinline__ void wtb_eval(const uint32_t *part, int lane, int32_t &best, uint32_t &fb, int32_t target,
+                                         std::integer_sequence<int, Is...>) {
+  auto one = [&](auto II) {
+    constexpr int l = WV + W * decltype(II)::value;
+    // MODE_MAX: any order (a block holds all 2^M patterns); MODE_FIRST: the
+    // pattern of low index l
+    constexpr int S = MODE == MODE_MAX ? l : ((l ^ (l >> 1)) ^ (ODD ? (1 << (M - 1)) : 0));
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w2 = 0; w2 < W; ++w2) tot += part[((size_t)w2 * (1 << M) + S) * 64 + lane];
+    if constexpr (MODE == MODE_MAX) {
+      best = (int32_t)tot > best ? (int32_t)tot : best;
+    } else {
+      const uint32_t c = ((int32_t)tot == target) ? (uint32_t)l : 0xFFFFFFFFu;
+      fb = c < fb ? c : fb;
+    }
+  };
+  (one(std::integral_constant<int, Is>{}), ...);
+}
+
+template <int W, int M, int MODE, bool ODD, int WV>
+struct WtbEval {
+  __device__ __forceinline__ static void run(int wv, const uint32_t *part, int lane, int32_t &best, uint32_t &fb,
+                                             int32_t target) {
+    if constexpr (WV < W) {
+      if (wv == WV) {
+        constexpr int cnt = WV < (1 << M) ? ((1 << M) - WV + W - 1) / W : 0;
+        if constexpr (cnt > 0)
+          wtb_eval<W, M, MODE, ODD, WV>(part, lane, best, fb, target, std::make_integer_sequence<int, cnt>{});
+      } else {
+        WtbEval<W, M, MODE, ODD, WV + 1>::run(wv, part, lane, best, fb, target);
+      }
+    }
+  }
+};
+
+template <int W, int NW, bool MARG, int MODE>
+__global__ __launch_bounds__(64 * W) void k_wide_tb16(
+    const uint32_t *__restrict__ Ap_, const uint32_t *__restrict__ tab_, int R, int q, uint64_t seg0, uint64_t nseg,
+    int32_t target, int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest,
+    unsigned long long *__restrict__ gfirst) {
+  constexpr int NWT = NW * W, M = wtb_m(W);
+  static_assert(W * (1 << M) * 64 * 4 <= 65536, "partials exceed 64 KiB");
+  __shared__ uint32_t part[W * (1 << M) * 64];
+  uint32_t *fin = part;   // per-wave results, once the walk's last barrier has passed
+  const cuint32_t *Ap = as_const(Ap_);
+  const int lane = wave_lane();
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t g = blockIdx.x;
+  if (((g >> 1) << 7) >= nseg) return;   // uniform over the workgroup
+  const uint64_t local = lane_segment(g, lane);
+  const bool active = local < nseg;
+  const uint64_t seg = seg0 + local;
+  const uint32_t pseg = (uint32_t)__builtin_amdgcn_readfirstlane((int)((seg0 ^ g) & 1));
+  const int qh = q - M;
+  const int wbase = wv * NW;
+  const cuint32_t *A2 = Ap + (size_t)R * NWT;                          // packed 2 A
+  const cint32_t *A0 = (const cint32_t *)(Ap + (size_t)2 * R * NWT);   // LM: column 0
+  const cint32_t *pos0 = (const cint32_t *)(as_const(tab_) + ((size_t)NWT << M));
+  uint32_t *mypart = part + (size_t)wv * (1 << M) * 64;
+
+  const uint64_t h0 = seg << qh;
+  const uint64_t g0 = h0 ^ (h0 >> 1);
+  uint32_t u[NW];
+  int32_t u0 = MARG ? A0[R - 1] : 0;
+#pragma unroll
+  for (int l = 0; l < NW; ++l) u[l] = pk_add(kBias16x2, Ap[(size_t)(R - 1) * NWT + wbase + l]);
+  for (int j = 0; j < R - 1 - M; ++j) {
+    const uint32_t pmask = ((g0 >> j) & 1) ? 0xFFFFFFFFu : 0u;   // +row where the bit is 1, -row elsewhere
+    const cuint32_t *row = Ap + (size_t)(M + j) * NWT + wbase;
+#pragma unroll
+    for (int l = 0; l < NW; ++l) {
+      const uint32_t v = row[l];
+      u[l] = pk_add(u[l], v & pmask);
+      u[l] = pk_sub(u[l], v & ~pmask);
+    }
+    if (MARG) u0 += pmask ? A0[M + j] : -A0[M + j];
+  }
+
+  int32_t best = INT32_MIN;
+  uint32_t first = 0xFFFFFFFFu;
+  const uint32_t nblk = 1u << qh;
+  for (uint32_t k = 0; k < nblk; ++k) {   // workgroup-uniform
+    if (k > 0) {
+      const int j = __builtin_ctz(k);
+      const uint32_t hb = (j + 1 < qh) ? ((k >> (j + 1)) & 1u) : pseg;
+      const cuint32_t *row2 = A2 + (size_t)(M + j) * NWT + wbase;
+      if (hb == 0) {
+#pragma unroll
+        for (int l = 0; l < NW; ++l) u[l] = pk_add(u[l], row2[l]);
+        if (MARG) u0 += 2 * A0[M + j];
+      } else {
+#pragma unroll
+        for (int l = 0; l < NW; ++l) u[l] = pk_sub(u[l], row2[l]);
+        if (MARG) u0 -= 2 * A0[M + j];
+      }
+    }
+    const cuint32_t *negk = as_const(tab_);
+    asm volatile("" : "+s"(negk));
+    wtb_partials<NW, NWT, M, MARG>(u, u0, negk + wbase, pos0, wv == 0, mypart, lane,
+                                   std::make_integer_sequence<int, 1 << M>{});
+    __syncthreads();
+    if constexpr (MODE == MODE_MAX) {
+      uint32_t unused = 0;
+      WtbEval<W, M, MODE, false, 0>::run(wv, part, lane, best, unused, target);
+    } else {
+      uint32_t fb = 0xFFFFFFFFu;
+      const uint32_t hp = qh > 0 ? (k & 1u) : pseg;
+      if (hp) WtbEval<W, M, MODE, true, 0>::run(wv, part, lane, best, fb, target);
+      else WtbEval<W, M, MODE, false, 0>::run(wv, part, lane, best, fb, target);
+      const uint32_t c = fb != 0xFFFFFFFFu ? (k << M) + fb : 0xFFFFFFFFu;
+      first = c < first ? c : first;
+    }
+    __syncthreads();
+  }
+  const uint32_t nsteps = nblk << M;
+
+  if constexpr (MODE == MODE_MAX) fin[wv * 64 + lane] = (uint32_t)best;
+  else fin[wv * 64 + lane] = first;
+  __syncthreads();
+  if (wv == 0) {
+    if constexpr (MODE == MODE_MAX) {
+      int32_t m = (int32_t)fin[lane];
+#pragma unroll
+      for (int w2 = 1; w2 < W; ++w2) m = (int32_t)fin[w2 * 64 + lane] > m ? (int32_t)fin[w2 * 64 + lane] : m;
+      if (active) unit_best[local] = m;
+      const int32_t wm = wave_max_i32(active ? m : INT32_MIN);
+      if (wm > __builtin_amdgcn_readfirstlane(*(volatile int32_t *)gbest)) atomicMax(gbest, wm);
+    } else {
+      uint32_t f = fin[lane];
+#pragma unroll
+      for (int w2 = 1; w2 < W; ++w2) f = fin[w2 * 64 + lane] < f ? fin[w2 * 64 + lane] : f;
+      const uint64_t r = (active && f != 0xFFFFFFFFu) ? (seg * (uint64_t)nsteps + f) : ~0ull;
+      const uint64_t m = wave_min_u64(r);
+      if (m != ~0ull) atomicMin(gfirst, (unsigned long long)m);
+    }
+  }
+}
+
+// Widths of the W-wave tabulated L1 / LM walk: 256, 512 or 1024 words (512,
+// 1024 or 2048 |.| columns).
+int wtb16_words(int nabs) {
+  const int need = (nabs + 1) / 2;
+  for (int w : {256, 512, 1024})
+    if (need <= w) return w;
+  return 0;
+}
+
+int wtb16_rows(int words) { return wtb_m(words == 256 ? 8 : 16); }
+
+template <int W, int NW, bool MARG>
+static hipError_t lwtb(Mode mode, const PlainArgs &a, hipStream_t st) {
+  const uint64_t groups = 2 * ((a.nseg + 127) / 128);
+  const dim3 grid((unsigned)groups), blk(64 * W);
+  const uint32_t *Ap = (const uint32_t *)a.A;
+  const uint32_t *tab = (const uint32_t *)a.D;
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_wide_tb16<W, NW, MARG, MODE_MAX>), grid, blk, 0, st, Ap, tab, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
+  else
+    hipLaunchKernelGGL((k_wide_tb16<W, NW, MARG, MODE_FIRST>), grid, blk, 0, st, Ap, tab, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
+  return hipGetLastError();
+}
+
+template <bool MARG>
+static hipError_t lwtb_m(Mode mode, const PlainArgs &a, hipStream_t st) {
+  switch (a.ny_pad / 2) {
+    case 256: return lwtb<8, 32, MARG>(mode, a, st);
+    case 512: return lwtb<16, 32, MARG>(mode, a, st);
+    case 1024: return lwtb<16, 64, MARG>(mode, a, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+hipError_t launch_wide_tb16(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st) {
+  if (!a.A || !a.D || !a.unit_best || !a.gbest || !a.gfirst) return hipErrorInvalidValue;
+  const int words = a.ny_pad / 2;
+  if (words * 2 != a.ny_pad || wtb16_words(2 * words) != words) return hipErrorInvalidValue;
+  if (a.q < wtb16_rows(words) || a.R - 1 < a.q) return hipErrorInvalidValue;
+  if (kind == KIND_L1) return lwtb_m<false>(mode, a, st);
+  if (kind == KIND_LM) return lwtb_m<true>(mode, a, st);
+  return hipErrorInvalidValue;
+}
+
 int wt16_words(int d, int ny) {
   const int nw = (d >= 2 && d <= 10) ? wt16_nw(d) : 0;
   const int need = (ny + 1) / 2;

@@ -408,6 +408,48 @@ def test_table16_binary_wide_vs_oracle(pkg, marg):
     assert t[:2] == w[:2] and t[2]["best_rank"] == w[2]["best_rank"]
 
 
+@pytest.mark.parametrize("marg", [0, 1])
+def test_table16w_binary_vs_oracle(pkg, marg):
+    """The tabulated L1 / LM walk over W waves beyond 384 |.| columns (8 waves
+    of 32 words, 16 of 32, 16 of 64: partial pattern sums through LDS):
+    bit-exact norm, strategy and rank at every shape, at forced segment
+    lengths, with the fewest enumerated rows it accepts, and an error below
+    that."""
+    rng = np.random.default_rng(1010 + marg)
+    for R in (6, 10):
+        for C in (385 + marg, 400, 512 + marg, 513 + marg, 700, 1024 + marg, 1025 + marg, 1500, 2048 + marg):
+            A = _rand(rng, R, C, 60)
+            print("table16w L1", marg, A.shape, flush=True)
+            res = _agree(pkg, A, 1, marg)
+            assert res["kernel_name"].startswith("table16w<L"), res["kernel_name"]
+    for C, M in ((450, 5), (900, 4), (1800, 4)):
+        A = _rand(rng, 14, C, 60)
+        for q in (M, M + 1, M + 3, 13):
+            print("table16w L1 seg_log", marg, A.shape, q, flush=True)
+            res = _agree(pkg, A, 1, marg, kernel="table16", seg_log=q)
+            assert f"M={M},q={q}>" in res["kernel_name"], res["kernel_name"]
+        A = _rand(rng, M + 1, C, 60)
+        assert _agree(pkg, A, 1, marg)["kernel_name"].startswith("table16w<L")
+        with pytest.raises(pkg.LnormError) as e:
+            pkg.calc_lnorm(_rand(rng, M, C, 60), 1, marg, kernel="table16")
+        assert e.value.code == -5
+
+
+def test_table16w_binary_equals_wide_larger(pkg):
+    """2^23 sign vectors of 1000 columns: the tabulated and binary W-wave
+    walks agree on norm, strategy and rank, and the strategy's objective is
+    the norm."""
+    from lnorm_amd import gen
+    for (R, C, marg) in ((24, 1000, 0), (22, 900, 1), (20, 500, 0)):
+        A = gen.gen_matrix(R, C, 60, 80 + marg)
+        print("table16w vs wide", R, C, marg, flush=True)
+        t = pkg.calc_lnorm(A, 1, marg)
+        w = pkg.calc_lnorm(A, 1, marg, kernel="wide")
+        assert t[2]["kernel_name"].startswith("table16w<L") and w[2]["kernel_name"].startswith("wide<")
+        assert t[:2] == w[:2] and t[2]["best_rank"] == w[2]["best_rank"], (R, C, marg)
+        assert oracle.objective(A, 1, marg, t[1]) == t[0]
+
+
 def test_table16_binary_equals_blocked_large(pkg):
     """2^33 sign vectors: the tabulated L1 / LM walk equals the blocked
     kernel (norm, strategy, rank), and the objective of the strategy is the
