@@ -46,7 +46,7 @@ extern "C" {
                                    per-label subset tables for n = 2..10 (column sums <= 32767; one wave
                                    up to 192 columns at d = 2 ... 32 at d = 10, W waves up to 512 at
                                    d <= 6 ... 128 at d = 10), sign-pattern tables for n = 1 (|.| column
-                                   sums <= 32767, up to 384 columns) */
+                                   sums <= 32767, one wave up to 384 columns, 8 or 16 waves up to 2048) */
 
 typedef struct lnorm_hip_opts {
     int32_t kernel;             /* LNORM_KERNEL_* */
