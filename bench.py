@@ -359,12 +359,13 @@ def batch_line(pkg, gen, steps: int) -> dict:
     if AUTOTUNE:
         best, _ = pkg.autotune_batch(probs)
         tuned = {k: best[k] for k in ("batch_streams", "batch_cap_log")}
-    times, dev = [], []
+    times, dev, call = [], [], []
     for _ in range(max(1, steps)):
         t = time.perf_counter()
         out, st = pkg.calc_batch(probs, stats=True, batch_graph=1, autotune=1 if AUTOTUNE else 0)
         times.append(time.perf_counter() - t)
         dev.append(st["ms_device"])
+        call.append(st["ms_total"])
         if out != eager:
             raise RuntimeError("graph replay differs from the eager batch")
     lab = st["labelings"]
@@ -373,6 +374,7 @@ def batch_line(pkg, gen, steps: int) -> dict:
     return {"items": len(probs), "groups": st["groups"], "batched": st["batched"], "labelings": lab,
             "labelings_by_d": {str(k): v for k, v in sorted(per_d.items())},
             "value": lab / wall, "unit": "labelings/s", "ms_per_step": 1e3 * wall, "ms_device": msd,
+            "ms_library_call": sum(call) / len(call),
             "device_labelings_per_s": lab / (msd / 1e3), "eager_ms_device": st0["ms_device"],
             "row_updates_per_s": row_updates / (msd / 1e3),
             "valu_TOPs_model": ops / (msd / 1e3) / 1e12, "valu_frac_model": ops / (msd / 1e3) / VALU_PEAK,

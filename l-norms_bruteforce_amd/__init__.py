@@ -214,16 +214,13 @@ def calc_lnorm_simple(A, n: int, marginal: int = 0, blocks: int = 512, threads: 
 def calc_batch(problems, want_strategy=True, stats=False, **opts):
     """problems: iterable of (A, n, marginal).  Returns [(Lnorm, strategy, status)]
     (and the batch statistics when stats=True)."""
-    items, mats, strats = _items(problems, want_strategy)
+    b = _Batch(problems, want_strategy)
     o = default_opts(**opts)
     st = BatchStats()
-    rc = hip_lib().lnorm_hip_calc_batch_ex(items, len(mats), ctypes.byref(o), ctypes.byref(st))
+    rc = hip_lib().lnorm_hip_calc_batch_ex(b.ptr, b.count, ctypes.byref(o), ctypes.byref(st))
     if rc == -4:
         _check(rc)
-    out = []
-    for it, m, s in zip(items, mats, strats):
-        out.append((int(it.Lnorm), [int(v) for v in s[: m.shape[0] - 1]] if want_strategy else None,
-                    int(it.status)))
+    out = b.results(want_strategy)
     return (out, st.as_dict()) if stats else out
 
 
@@ -246,29 +243,58 @@ def autotune(A, n: int, marginal: int = 0, **opts):
     return _opts_dict(best), [ent[i].as_dict() for i in range(min(n_ent, 64))]
 
 
-def _items(problems, want_strategy):
-    problems = list(problems)
-    mats = [_i32(A) for A, _, _ in problems]
-    strats = [np.zeros(max(m.shape[0] - 1, 1), dtype=np.int32) for m in mats]
-    items = (Item * len(mats))()
-    i32p = ctypes.POINTER(ctypes.c_int32)
-    for it, m, s, (_, n, marg) in zip(items, mats, strats, problems):
-        it.mtx_as_vec = m.ctypes.data_as(i32p)
-        it.iRows, it.iCols = m.shape
-        it.n, it.marginal = n, marg
-        it.strategy = s.ctypes.data_as(i32p) if want_strategy else None
-    return items, mats, strats
+# lnorm_hip_item as a numpy record (the same offsets and size as Item)
+_ITEM_DTYPE = np.dtype({"names": [f for f, _ in Item._fields_],
+                        "formats": [np.uintp, np.int32, np.int32, np.int32, np.int32, np.uintp, np.int32, np.int32],
+                        "offsets": [getattr(Item, f).offset for f, _ in Item._fields_],
+                        "itemsize": ctypes.sizeof(Item)})
+
+
+class _Batch:
+    """The items of one batch call: an array of lnorm_hip_item records built
+    with numpy (no per-item ctypes objects), every matrix in one int32
+    buffer and every strategy (max(R - 1, 1) ints each) in another."""
+
+    def __init__(self, problems, want_strategy: bool):
+        problems = list(problems)
+        mats = [_i32(A) for A, _, _ in problems]
+        k = self.count = len(mats)
+        shape = np.array([m.shape for m in mats], dtype=np.int64).reshape(k, 2)
+        size = shape[:, 0] * shape[:, 1]
+        self.buf = np.concatenate([m.ravel() for m in mats] + [np.zeros(1, np.int32)])
+        moff = np.concatenate(([0], np.cumsum(size)[:-1])) if k else np.zeros(0, np.int64)
+        slen = np.maximum(shape[:, 0] - 1, 1)
+        self.soff = np.concatenate(([0], np.cumsum(slen))).astype(np.int64)
+        self.strat = np.zeros(int(self.soff[-1]) + 1, np.int32)
+        a = np.zeros(k, _ITEM_DTYPE)
+        a["mtx_as_vec"] = self.buf.ctypes.data + 4 * moff
+        a["iRows"], a["iCols"] = shape[:, 0], shape[:, 1]
+        a["n"] = np.fromiter((p[1] for p in problems), np.int32, k)
+        a["marginal"] = np.fromiter((p[2] for p in problems), np.int32, k)
+        if want_strategy:
+            a["strategy"] = self.strat.ctypes.data + 4 * self.soff[:-1]
+        self.a = a
+        self.ptr = ctypes.cast(a.ctypes.data, ctypes.POINTER(Item))
+
+    def results(self, want_strategy: bool) -> list:
+        """[(Lnorm, strategy (R - 1 ints) or None, status)] per item."""
+        L, S = self.a["Lnorm"].tolist(), self.a["status"].tolist()
+        if not want_strategy:
+            return [(l, None, s) for l, s in zip(L, S)]
+        full = self.strat.tolist()
+        n = np.maximum(self.a["iRows"].astype(np.int64) - 1, 0).tolist()
+        return [(l, full[o:o + c], s) for l, o, c, s in zip(L, self.soff[:-1].tolist(), n, S)]
 
 
 def autotune_batch(problems, **opts):
     """lnorm_hip_autotune_batch: times batch_streams x batch_cap_log on a
     sample of the batch and caches the fastest for this batch shape.
     Returns (best options dict, [candidate dicts])."""
-    items, mats, _ = _items(problems, False)
+    b = _Batch(problems, False)
     o = default_opts(**opts)
     best = Opts()
     ent = (TuneEntry * 64)()
-    n_ent = _check(hip_lib().lnorm_hip_autotune_batch(items, len(mats), ctypes.byref(o), ctypes.byref(best),
+    n_ent = _check(hip_lib().lnorm_hip_autotune_batch(b.ptr, b.count, ctypes.byref(o), ctypes.byref(best),
                                                       ent, 64))
     return _opts_dict(best), [ent[i].as_dict() for i in range(min(n_ent, 64))]
 

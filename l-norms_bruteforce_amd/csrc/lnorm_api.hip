@@ -18,6 +18,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -26,6 +27,7 @@
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -255,10 +257,21 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
       if (marg) pp.A[(size_t)2 * p.R * NW + i] = p.A[(size_t)i * p.C];
     }
     pp.D.assign(((size_t)NW << M) + (marg ? (1u << M) : 0), 0);
+    // signed sums of the M fastest rows, each pattern from the one without its
+    // lowest set bit: val[S] = val[S & (S - 1)] + 2 A[ctz S]
+    std::vector<int32_t> val((size_t)p.C << M);
+    for (int l = 0; l < p.C; ++l) {
+      int32_t sum = 0;
+      for (int i = 0; i < M; ++i) sum -= p.A[(size_t)i * p.C + l];
+      val[l] = sum;
+    }
+    for (int S = 1; S < (1 << M); ++S) {
+      const int32_t *prev = val.data() + (size_t)(S & (S - 1)) * p.C, *row = p.A + (size_t)__builtin_ctz(S) * p.C;
+      for (int l = 0; l < p.C; ++l) val[(size_t)S * p.C + l] = prev[l] + 2 * row[l];
+    }
     for (int S = 0; S < (1 << M); ++S) {
       for (int l = 0; l < p.C; ++l) {
-        int32_t sum = 0;
-        for (int i = 0; i < M; ++i) sum += ((S >> i) & 1 ? 1 : -1) * p.A[(size_t)i * p.C + l];
+        const int32_t sum = val[(size_t)S * p.C + l];
         if (marg && l == 0) {
           pp.D[((size_t)NW << M) + S] = sum;
         } else {
@@ -288,12 +301,17 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
       const int M = walk == WALK_TAB16W ? wt16_rows(p.n, NW) : blk16_rows(p.n, NW);
       pp.qmin = M;
       pp.D.assign((size_t)NW << M, 0);
+      // subset sums, each from the subset without its lowest row:
+      // sum[S] = sum[S & (S - 1)] + A[ctz S]; padding columns stay 0
+      std::vector<int32_t> sum((size_t)(2 * NW) << M, 0);
+      for (int S = 1; S < (1 << M); ++S) {
+        const int32_t *prev = sum.data() + (size_t)(S & (S - 1)) * 2 * NW, *row = p.A + (size_t)__builtin_ctz(S) * p.C;
+        int32_t *cur = sum.data() + (size_t)S * 2 * NW;
+        for (int l = 0; l < p.C; ++l) cur[l] = prev[l] + row[l];
+      }
       for (int S = 0; S < (1 << M); ++S)
         for (int l = 0; l < 2 * NW; ++l) {
-          int32_t sum = 0;
-          for (int i = 0; i < M; ++i)
-            if (((S >> i) & 1) && l < p.C) sum += p.A[(size_t)i * p.C + l];
-          const uint32_t v = (uint32_t)(kBias16 - sum) & 0xFFFFu;
+          const uint32_t v = (uint32_t)(kBias16 - sum[(size_t)S * 2 * NW + l]) & 0xFFFFu;
           pp.D[(size_t)S * NW + l / 2] |= (int32_t)(v << (16 * (l & 1)));
         }
     }
@@ -1032,6 +1050,7 @@ constexpr uint64_t kBatchMaxTotal = 1ull << 32;
 // a short tail at the end of each group).
 constexpr int kBatchCapLog = 14;
 constexpr int kBatchStreams = 4;
+constexpr int kPrepThreads = 16;   // host threads preparing batch items (the box's CPU share)
 Geometry batch_geometry(int base, int e, int forced, Walk walk, int cap_log, int qmin = 0) {
   if (forced > 0) return geometry_for(base, e, 63, forced, walk, qmin);
   const double lb = std::log2((double)base);
@@ -1118,6 +1137,49 @@ int tune_batch(const lnorm_hip_item *items, int count, const lnorm_hip_opts &o, 
   return ntimed;
 }
 
+// Classifies batch item i: validates it, picks its walk and prepares its
+// tables and geometry in b.  Returns 0 (batched), 1 (computed on its own) or
+// the item's error code.
+int batch_item(const lnorm_hip_item &it, int i, const lnorm_hip_opts &o, int cap_log, BItem &b) {
+  b.idx = i;
+  b.p = Problem{it.mtx_as_vec, it.iRows, it.iCols, it.n, it.marginal, KIND_L1, 2, 0};
+  if (int r = validate(b.p)) return r;
+  Walk walk;
+  int width;
+  if (b.p.n >= 2) {
+    const bool fits16 = max_col_abs_sum(b.p, 0) <= 32767;
+    const int w16 = fits16 ? plain16_words(b.p.n, b.p.C) : 0;
+    int wtab = fits16 ? blk16_words(b.p.n, b.p.C) : 0;
+    if (wtab > 32 || (wtab && b.p.R - 1 < blk16_rows(b.p.n, wtab))) wtab = 0;   // batched widths: up to 32 words
+    if (wtab) {
+      walk = WALK_TAB16;
+      width = 2 * wtab;
+    } else {
+      if (!w16) return 1;
+      walk = WALK_PLAIN16;
+      width = 2 * w16;
+    }
+  } else {
+    const int marg = b.p.kind == KIND_LM ? 1 : 0;
+    int wtb = max_col_abs_sum(b.p, marg) <= 32767 ? tab16_bin_words(b.p.C - marg) : 0;
+    if (wtb > 64 || (wtb && b.p.R - 1 < tab_bin_rows(wtb))) wtb = 0;   // batched widths: up to 64 words
+    if (wtb) {
+      walk = WALK_TAB16;
+      width = 2 * wtb;
+    } else {
+      width = plain_ny_pad(b.p.kind, 1, b.p.C);
+      if (!width) return 1;
+      walk = WALK_PLAIN;
+    }
+  }
+  if (b.p.total > kBatchMaxTotal) return 1;
+  prepare_walk(b.p, walk, width, b.pp);
+  b.g1 = batch_geometry(b.p.base, b.p.R - 1, o.seg_log, walk, cap_log, b.pp.qmin);
+  b.g2 = geometry_for(b.p.base, b.g1.q, 8, 0, walk, b.pp.qmin);
+  b.work = (double)b.p.total * walk_cost(walk, b.p.n, width);
+  return 0;
+}
+
 int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_hip_batch_stats *stats) {
   const auto t_start = std::chrono::steady_clock::now();
   int first_err = 0;
@@ -1147,52 +1209,45 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
       cap_log = tuned.second;
     }
   }
+  // items are classified and prepared in parallel (tables and packing are
+  // the host's share of a batch); results are gathered in order, so the
+  // statuses, the first error code and the last error message are those of
+  // a sequential pass
+  std::vector<BItem> slot(count);
+  std::vector<int> cls(count, 0);
+  std::vector<std::string> msg(count);
+  {
+    std::atomic<int> next{0};
+    auto worker = [&] {
+      for (;;) {
+        const int lo = next.fetch_add(32);
+        if (lo >= count) break;
+        for (int i = lo; i < std::min(count, lo + 32); ++i) {
+          cls[i] = batch_item(items[i], i, o, cap_log, slot[i]);
+          if (cls[i] < 0) msg[i] = g_err;
+        }
+      }
+    };
+    const int hw = (int)std::max(1u, std::thread::hardware_concurrency());
+    const int nthr = std::min({kPrepThreads, hw, (count + 127) / 128});
+    std::vector<std::thread> pool;
+    for (int t = 1; t < nthr; ++t) pool.emplace_back(worker);
+    worker();
+    for (auto &th : pool) th.join();
+  }
   for (int i = 0; i < count; ++i) {
     lnorm_hip_item &it = items[i];
     it.Lnorm = 0;
     it.status = LNORM_E_HIP;   // until this item's result is in
-    BItem b;
-    b.idx = i;
-    b.p = Problem{it.mtx_as_vec, it.iRows, it.iCols, it.n, it.marginal, KIND_L1, 2, 0};
-    if (int r = validate(b.p)) {
-      it.status = r;
-      if (!first_err) first_err = r;
-      continue;
-    }
-    Walk walk;
-    int width;
-    if (b.p.n >= 2) {
-      const bool fits16 = max_col_abs_sum(b.p, 0) <= 32767;
-      const int w16 = fits16 ? plain16_words(b.p.n, b.p.C) : 0;
-      int wtab = fits16 ? blk16_words(b.p.n, b.p.C) : 0;
-      if (wtab > 32 || (wtab && b.p.R - 1 < blk16_rows(b.p.n, wtab))) wtab = 0;   // batched widths: up to 32 words
-      if (wtab) {
-        walk = WALK_TAB16;
-        width = 2 * wtab;
-      } else {
-        if (!w16) { fallback.push_back(i); continue; }
-        walk = WALK_PLAIN16;
-        width = 2 * w16;
-      }
+    if (cls[i] < 0) {
+      it.status = cls[i];
+      if (!first_err) first_err = cls[i];
+      g_err = msg[i];
+    } else if (cls[i] == 1) {
+      fallback.push_back(i);
     } else {
-      const int marg = b.p.kind == KIND_LM ? 1 : 0;
-      int wtb = max_col_abs_sum(b.p, marg) <= 32767 ? tab16_bin_words(b.p.C - marg) : 0;
-      if (wtb > 64 || (wtb && b.p.R - 1 < tab_bin_rows(wtb))) wtb = 0;   // batched widths: up to 64 words
-      if (wtb) {
-        walk = WALK_TAB16;
-        width = 2 * wtb;
-      } else {
-        width = plain_ny_pad(b.p.kind, 1, b.p.C);
-        if (!width) { fallback.push_back(i); continue; }
-        walk = WALK_PLAIN;
-      }
+      bis.push_back(std::move(slot[i]));
     }
-    if (b.p.total > kBatchMaxTotal) { fallback.push_back(i); continue; }
-    prepare_walk(b.p, walk, width, b.pp);
-    b.g1 = batch_geometry(b.p.base, b.p.R - 1, o.seg_log, walk, cap_log, b.pp.qmin);
-    b.g2 = geometry_for(b.p.base, b.g1.q, 8, 0, walk, b.pp.qmin);
-    b.work = (double)b.p.total * walk_cost(walk, b.p.n, width);
-    bis.push_back(std::move(b));
   }
 
   // groups by instantiation, longest item first within each
