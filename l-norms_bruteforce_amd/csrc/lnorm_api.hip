@@ -1149,8 +1149,8 @@ int batch_item(const lnorm_hip_item &it, int i, const lnorm_hip_opts &o, int cap
   if (b.p.n >= 2) {
     const bool fits16 = max_col_abs_sum(b.p, 0) <= 32767;
     const int w16 = fits16 ? plain16_words(b.p.n, b.p.C) : 0;
-    int wtab = fits16 ? blk16_words(b.p.n, b.p.C) : 0;
-    if (wtab > 32 || (wtab && b.p.R - 1 < blk16_rows(b.p.n, wtab))) wtab = 0;   // batched widths: up to 32 words
+    int wtab = fits16 ? batch_t16_words(b.p.n, b.p.C) : 0;   // 4..32 words in steps of 4
+    if (wtab && b.p.R - 1 < blk16_rows(b.p.n, wtab)) wtab = 0;
     if (wtab) {
       walk = WALK_TAB16;
       width = 2 * wtab;

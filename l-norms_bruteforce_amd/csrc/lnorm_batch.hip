@@ -15,36 +15,9 @@
 // reading the unit index and the target from device memory.  Nothing waits
 // on the host in between, so the whole sequence can be captured in a HIP
 // graph and replayed.
-#include "lnorm_walks.h"
+#include "lnorm_batch.h"
 
 namespace lnorm {
-
-// The wave's next task.  A wave starts on the counter shard of its XCD
-// (workgroups are dispatched to the 8 XCDs round-robin by index) and moves
-// to the next shard when that one is exhausted, so every task is handed out
-// once and the waves of an XCD that finishes early help the others.  Every
-// lane executes the atomic (lane 0 adds 1, the others 0): no control flow
-// depends on the lane index.  A `if (lane == 0)` here, loop-invariant but
-// lane-divergent, let the compiler split the persistent loop by lane, and
-// lanes 1..63 then looped on task 0 forever.
-__device__ __forceinline__ bool next_task(unsigned int *counters, uint64_t ntasks, int &shard, int &left,
-                                          uint32_t &t) {
-  while (left > 0) {
-    // readfirstlane: the shard is wave-uniform, and the compiler must see it
-    // so; with a uniform address it issues one atomic per wave (the lane
-    // addends are summed first) instead of 64 same-address atomics.
-    const int us = __builtin_amdgcn_readfirstlane(shard);
-    const uint32_t k = atomicAdd(counters + (size_t)us * kCounterStride, wave_lane() == 0 ? 1u : 0u);
-    const uint64_t cand = (uint64_t)(uint32_t)__builtin_amdgcn_readfirstlane((int)k) * kCounterShards + shard;
-    if (cand < ntasks) {
-      t = (uint32_t)cand;
-      return true;
-    }
-    shard = (shard + 1) & (kCounterShards - 1);
-    --left;
-  }
-  return false;
-}
 
 template <int KIND, int NY, int MODE>
 __global__ __launch_bounds__(256) void k_batch_bin(const BatchTask *__restrict__ tasks, int nitems,
@@ -93,31 +66,6 @@ __global__ __launch_bounds__(256) void k_batch_p16(const BatchTask *__restrict__
     }
     walk_p16<D, NW, MODE>(as_const((const uint32_t *)it.A), it.R, it.q, seg0, it.nseg, target, it.unit_best, it.gbest,
                           it.gfirst, B, t - it.task0);
-  }
-}
-
-template <int D, int NW, int MODE>
-__global__ __launch_bounds__(256) void k_batch_t16(const BatchTask *__restrict__ tasks, int nitems,
-                                                   uint64_t ntasks, unsigned int *__restrict__ counter) {
-  int shard = (int)(blockIdx.x & (kCounterShards - 1)), left = kCounterShards;
-  uint32_t t;
-  while (next_task(counter, ntasks, shard, left, t)) {
-    int lo = 0, hi = nitems - 1;
-    while (lo < hi) {
-      const int mid = (lo + hi + 1) >> 1;
-      if (tasks[mid].task0 <= t) lo = mid;
-      else hi = mid - 1;
-    }
-    const BatchTask &it = tasks[lo];
-    uint64_t seg0 = 0;
-    int32_t target = 0;
-    if (MODE == MODE_FIRST) {
-      seg0 = *it.unit * it.seg_base;
-      target = *it.gbest;
-    }
-    walk_blk16<D, NW, blk16_low_rows(D, NW), MODE>(as_const((const uint32_t *)it.A), as_const((const uint32_t *)it.D),
-                                                    it.R, it.q, seg0, it.nseg, target, it.unit_best, it.gbest,
-                                                    it.gfirst, t - it.task0);
   }
 }
 
@@ -177,11 +125,6 @@ hipError_t launch_batch_first_unit(const BatchTask *tasks, int nitems, unsigned 
 // ---------------------------------------------------------------------------
 // launchers: grid = min(tasks, resident waves); the waves loop over tasks
 // ---------------------------------------------------------------------------
-static unsigned batch_grid(uint64_t ntasks) {
-  const uint64_t blocks = (ntasks + 3) / 4;            // 4 waves per 256-thread block
-  const uint64_t cap = 256ull * 8;                      // 256 CUs x up to 8 blocks resident
-  return (unsigned)(blocks < cap ? (blocks ? blocks : 1) : cap);
-}
 
 template <int KIND, int NY>
 static hipError_t lb_bin(Mode mode, const BatchLaunch &b, hipStream_t st) {
@@ -205,28 +148,6 @@ static hipError_t lb_p16(Mode mode, const BatchLaunch &b, hipStream_t st) {
     hipLaunchKernelGGL((k_batch_p16<D, NW, MODE_FIRST>), grid, blk, 0, st, b.tasks, b.nitems, b.ntasks,
                        b.counter, kBias16x2);
   return hipGetLastError();
-}
-
-template <int D, int NW>
-static hipError_t lb_t16(Mode mode, const BatchLaunch &b, hipStream_t st) {
-  const dim3 grid(batch_grid(b.ntasks)), blk(256);
-  if (mode == MODE_MAX)
-    hipLaunchKernelGGL((k_batch_t16<D, NW, MODE_MAX>), grid, blk, 0, st, b.tasks, b.nitems, b.ntasks, b.counter);
-  else
-    hipLaunchKernelGGL((k_batch_t16<D, NW, MODE_FIRST>), grid, blk, 0, st, b.tasks, b.nitems, b.ntasks, b.counter);
-  return hipGetLastError();
-}
-
-template <int D>
-static hipError_t lb_t16_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
-  if (blk16_words(D, b.width) * 2 != b.width) return hipErrorInvalidValue;
-  switch (b.width / 2) {
-#define X(n) case n: if constexpr (n <= blk16_max_words(D)) return lb_t16<D, n>(mode, b, st); break;
-    X(8) X(16) X(24) X(32)
-#undef X
-    default: break;
-  }
-  return hipErrorInvalidValue;
 }
 
 template <int NW, bool MARG>
@@ -281,20 +202,7 @@ hipError_t launch_batch_walk(Mode mode, const BatchLaunch &b, hipStream_t st) {
   }
   if (b.table && b.kind == KIND_L1) return lb_tb16_w<false>(mode, b, st);
   if (b.table && b.kind == KIND_LM) return lb_tb16_w<true>(mode, b, st);
-  if (b.table) {
-    switch (b.d) {
-      case 2: return lb_t16_w<2>(mode, b, st);
-      case 3: return lb_t16_w<3>(mode, b, st);
-      case 4: return lb_t16_w<4>(mode, b, st);
-      case 5: return lb_t16_w<5>(mode, b, st);
-      case 6: return lb_t16_w<6>(mode, b, st);
-      case 7: return lb_t16_w<7>(mode, b, st);
-      case 8: return lb_t16_w<8>(mode, b, st);
-      case 9: return lb_t16_w<9>(mode, b, st);
-      case 10: return lb_t16_w<10>(mode, b, st);
-      default: return hipErrorInvalidValue;
-    }
-  }
+  if (b.table) return launch_batch_t16(mode, b, st);
   switch (b.d) {
     case 2: return lb_p16_w<2>(mode, b, st);
     case 3: return lb_p16_w<3>(mode, b, st);

@@ -122,6 +122,8 @@ struct BatchLaunch {
 };
 
 hipError_t launch_batch_walk(Mode mode, const BatchLaunch &b, hipStream_t st);
+hipError_t launch_batch_t16(Mode mode, const BatchLaunch &b, hipStream_t st);   // the tabulated d-ary groups
+int batch_t16_words(int d, int ny);   // a batch item's tabulated width in words (0: none)
 hipError_t launch_batch_first_unit(const BatchTask *tasks, int nitems, unsigned long long *first, hipStream_t st);
 
 hipError_t launch_plain(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block);
