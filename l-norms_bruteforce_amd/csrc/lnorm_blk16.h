@@ -1,0 +1,50 @@
+// lnorm_blk16.h -- the tabulated packed d-ary walk, one lane per segment
+// (walk_blk16): its kernel and launchers, instantiated by lnorm_dary.hip
+// (d = 2..4) and lnorm_dary_hi.hip (d = 5..10), which compile in parallel.
+#pragma once
+
+#include "lnorm_walks.h"
+
+namespace lnorm {
+
+template <int D, int NW, int MODE>
+__global__ __launch_bounds__(256) void k_blk16_dary(
+    const uint32_t *__restrict__ Ap, const uint32_t *__restrict__ neg, int R, int q, uint64_t seg0, uint64_t nseg,
+    int32_t target, int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest,
+    unsigned long long *__restrict__ gfirst) {
+  walk_blk16<D, NW, blk16_low_rows(D, NW), MODE>(as_const(Ap), as_const(neg), R, q, seg0, nseg, target, unit_best,
+                                                  gbest, gfirst, global_wave());
+}
+
+template <int D, int NW>
+static inline hipError_t launch_blk_nw(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
+  const uint64_t waves = 2 * ((a.nseg + 127) / 128);
+  const int wpb = block / 64;
+  const dim3 grid((unsigned)((waves + wpb - 1) / wpb)), blk(block);
+  const uint32_t *Ap = (const uint32_t *)a.A;
+  const uint32_t *neg = (const uint32_t *)a.D;
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_blk16_dary<D, NW, MODE_MAX>), grid, blk, 0, st, Ap, neg, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
+  else
+    hipLaunchKernelGGL((k_blk16_dary<D, NW, MODE_FIRST>), grid, blk, 0, st, Ap, neg, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
+  return hipGetLastError();
+}
+
+template <int D>
+static inline hipError_t launch_blk(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
+  switch (a.ny_pad / 2) {
+#define X(n) case n: if constexpr (n <= blk16_max_words(D)) return launch_blk_nw<D, n>(mode, a, st, block); break;
+    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64) X(80) X(96)
+#undef X
+    default: break;
+  }
+  return hipErrorInvalidValue;
+}
+
+// one translation unit's share of launch_blk16
+hipError_t launch_blk16_lo(Mode mode, const PlainArgs &a, hipStream_t st, int block);   // d = 2..4
+hipError_t launch_blk16_hi(Mode mode, const PlainArgs &a, hipStream_t st, int block);   // d = 5..10
+
+}  // namespace lnorm

@@ -1,0 +1,19 @@
+// lnorm_dary_hi.hip -- the tabulated packed d-ary walk's kernels for
+// d = 5..10 (lnorm_blk16.h; the launch entry is in lnorm_dary.hip).
+#include "lnorm_blk16.h"
+
+namespace lnorm {
+
+hipError_t launch_blk16_hi(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
+  switch (a.d) {
+    case 5: return launch_blk<5>(mode, a, st, block);
+    case 6: return launch_blk<6>(mode, a, st, block);
+    case 7: return launch_blk<7>(mode, a, st, block);
+    case 8: return launch_blk<8>(mode, a, st, block);
+    case 9: return launch_blk<9>(mode, a, st, block);
+    case 10: return launch_blk<10>(mode, a, st, block);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace lnorm

@@ -319,15 +319,15 @@ def test_wide16_walk_vs_oracle(pkg, d, R, cols):
 # (d, rows, column counts) for the tabulated walk: every instantiated width
 # (8..48 words) and odd column counts; rows leave several high digits
 TABLE16 = [
-    (2, 15, [1, 5, 16, 33, 64, 65, 100, 128, 129, 160, 192]),
-    (3, 12, [1, 5, 16, 17, 33, 48, 64, 65, 80, 96, 97, 112, 128]),
-    (4, 10, [3, 16, 31, 48, 64, 65, 80, 96]),
-    (5, 8, [7, 16, 33, 48, 64, 65, 80]),
-    (6, 7, [9, 16, 32, 48, 49, 64]),
-    (7, 6, [5, 16, 32, 33, 48, 64]),
-    (8, 6, [5, 16, 32, 33, 48]),
-    (9, 5, [5, 16, 32, 33, 48]),
-    (10, 5, [5, 16, 32]),
+    (2, 15, [1, 5, 16, 24, 33, 40, 56, 64, 65, 100, 128, 129, 160, 192]),
+    (3, 12, [1, 5, 16, 17, 24, 33, 40, 48, 56, 64, 65, 80, 96, 97, 112, 128]),
+    (4, 10, [3, 16, 24, 31, 40, 48, 56, 64, 65, 80, 96]),
+    (5, 8, [7, 16, 24, 33, 40, 48, 56, 64, 65, 80]),
+    (6, 7, [9, 16, 24, 32, 40, 48, 49, 56, 64]),
+    (7, 6, [5, 16, 24, 32, 33, 40, 48, 56, 64]),
+    (8, 6, [5, 16, 24, 32, 33, 40, 48]),
+    (9, 5, [5, 16, 24, 32, 33, 40, 48]),
+    (10, 5, [5, 16, 24, 32]),
 ]
 
 
