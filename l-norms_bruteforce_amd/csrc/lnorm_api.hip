@@ -1051,11 +1051,11 @@ constexpr uint64_t kBatchMaxTotal = 1ull << 32;
 constexpr int kBatchCapLog = 14;
 constexpr int kBatchStreams = 4;
 constexpr int kPrepThreads = 16;   // host threads preparing batch items (the box's CPU share)
-Geometry batch_geometry(int base, int e, int forced, Walk walk, int cap_log, int qmin = 0) {
+Geometry batch_geometry(int base, int e, int forced, Walk walk, int cap_log, int qmin = 0, int qfloor = 0) {
   if (forced > 0) return geometry_for(base, e, 63, forced, walk, qmin);
   const double lb = std::log2((double)base);
   Geometry g = geometry_for(base, e, 7, 0, walk, qmin);
-  const int qcap = std::max(0, (int)std::floor(cap_log / lb));
+  const int qcap = std::max({0, qfloor, (int)std::floor(cap_log / lb)});
   if (g.q > qcap) g = geometry_for(base, e, 63, qcap, walk, qmin);
   return g;
 }
@@ -1174,7 +1174,17 @@ int batch_item(const lnorm_hip_item &it, int i, const lnorm_hip_opts &o, int cap
   }
   if (b.p.total > kBatchMaxTotal) return 1;
   prepare_walk(b.p, walk, width, b.pp);
-  b.g1 = batch_geometry(b.p.base, b.p.R - 1, o.seg_log, walk, cap_log, b.pp.qmin);
+  // a lane segment of the tabulated d-ary walk starts with all d label tables
+  // (d 2^M NW v_sad_u16) and the seed (about 2 R NW): its segments hold at
+  // least 32 times that work, whatever the cap (3^9 labelings at d = 3 and 16
+  // words, against 3^7 at the default cap)
+  int qfloor = 0;
+  if (walk == WALK_TAB16 && b.p.n >= 2) {
+    const double nw = width / 2, start = (double)b.p.n * (1 << b.pp.qmin) * nw + 2.0 * b.p.R * nw;
+    const double c = walk_cost(walk, b.p.n, width);
+    while (qfloor < b.p.R - 1 && std::pow((double)b.p.n, qfloor) * c < 32 * start) ++qfloor;
+  }
+  b.g1 = batch_geometry(b.p.base, b.p.R - 1, o.seg_log, walk, cap_log, b.pp.qmin, qfloor);
   b.g2 = geometry_for(b.p.base, b.g1.q, 8, 0, walk, b.pp.qmin);
   b.work = (double)b.p.total * walk_cost(walk, b.p.n, width);
   return 0;
