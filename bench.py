@@ -409,14 +409,42 @@ def tune_sweep(pkg, gen) -> dict:
     return out
 
 
+def wide_lines(pkg, gen) -> dict:
+    """The wide L1 matrices on AUTO's tabulated walks beside the binary
+    W-wave walk: 36 x 256 (2^35 sign vectors, one wave per segment) and
+    34 x 1000 (2^33, 16 waves per segment)."""
+    out = {}
+    W = gen.gen_matrix(36, 256, 100, 107)
+    wt = single_line(pkg, W, 1, 2 ** 35, valu_ops_per_unit("table16", 256, 0) + 128 / 16 - 128 / 32)
+    ww = single_line(pkg, W, 1, 2 ** 35, 2 * 256 + 1, kernel="wide")
+    if wt["norm"] != ww["norm"]:
+        raise RuntimeError("tabulated and W-wave walks disagree on 36x256")
+    out["wide_36x256"] = {**wt, "wwave": {k2: ww[k2] for k2 in ("value", "kernel", "kernel_ms", "valu_frac")},
+                          "speedup_vs_wwave": wt["value"] / ww["value"]}
+    # L1 of a 34 x 1000 matrix (2^33 sign vectors): the tabulated walk over
+    # 16 waves beside the binary W-wave walk
+    X = gen.gen_matrix(34, 1000, 30, 109)
+    xt = single_line(pkg, X, 1, 2 ** 33, valu_ops_per_unit("table16w", 1000, 0))
+    xw = single_line(pkg, X, 1, 2 ** 33, 2 * 1024 + 1, kernel="wide")
+    if xt["norm"] != xw["norm"]:
+        raise RuntimeError("tabulated and binary W-wave walks disagree on 34x1000")
+    out["wide_34x1000"] = {**xt, "wwave": {k2: xw[k2] for k2 in ("value", "kernel", "kernel_ms", "valu_frac")},
+                           "speedup_vs_wwave": xt["value"] / xw["value"]}
+    return out
+
+
 def run_case(pkg, gen, case: str, steps: int, warmup: int) -> int:
     """One workload alone: the headline matrix on the blocked, plain or
-    tabulated kernel, the L3 workload, or the 4096-item batch."""
+    tabulated kernel, the L3 workload, the wide L1 matrices or the 4096-item
+    batch."""
     if case == "batch":
         print(json.dumps({"case": case, **batch_line(pkg, gen, steps)}), flush=True)
         return 0
     if case == "l3":
         print(json.dumps({"case": case, **l3_line(pkg, gen, steps, warmup, False)}), flush=True)
+        return 0
+    if case == "wide":
+        print(json.dumps({"case": case, **wide_lines(pkg, gen)}), flush=True)
         return 0
     A = gen.gen_matrix(64, 40, 127, 104).T.copy()
     k, ny = A.shape
@@ -436,7 +464,7 @@ def main():
                     help="run the default launch shapes instead of the autotuner's (tuned before timing)")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the autotuner's candidates on the bench workloads and print them as JSON")
-    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "l3"],
+    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "l3", "wide"],
                     help="run only this workload (for profiler passes) and print its figures")
     ap.add_argument("--copy-calibration", type=int, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB (torch) and exit: the reference "
@@ -554,22 +582,7 @@ def main():
                                 "blocked_valu_frac": s3["valu_frac"], "plain_valu_frac": s4["valu_frac"],
                                 "kernel": s3["kernel_name"], "norm": s3["norm"]}
         line["cli_end_to_end"] = cli_line(pkg, gen)
-        W = gen.gen_matrix(36, 256, 100, 107)
-        wt = single_line(pkg, W, 1, 2 ** 35, valu_ops_per_unit("table16", 256, 0) + 128 / 16 - 128 / 32)
-        ww = single_line(pkg, W, 1, 2 ** 35, 2 * 256 + 1, kernel="wide")
-        if wt["norm"] != ww["norm"]:
-            raise RuntimeError("tabulated and W-wave walks disagree on 36x256")
-        line["wide_36x256"] = {**wt, "wwave": {k2: ww[k2] for k2 in ("value", "kernel", "kernel_ms", "valu_frac")},
-                               "speedup_vs_wwave": wt["value"] / ww["value"]}
-        # L1 of a 34 x 1000 matrix (2^33 sign vectors): the tabulated walk over
-        # 16 waves beside the binary W-wave walk
-        X = gen.gen_matrix(34, 1000, 30, 109)
-        xt = single_line(pkg, X, 1, 2 ** 33, valu_ops_per_unit("table16w", 1000, 0))
-        xw = single_line(pkg, X, 1, 2 ** 33, 2 * 1024 + 1, kernel="wide")
-        if xt["norm"] != xw["norm"]:
-            raise RuntimeError("tabulated and binary W-wave walks disagree on 34x1000")
-        line["wide_34x1000"] = {**xt, "wwave": {k2: xw[k2] for k2 in ("value", "kernel", "kernel_ms", "valu_frac")},
-                                "speedup_vs_wwave": xt["value"] / xw["value"]}
+        line.update(wide_lines(pkg, gen))
         line["int64_34x40"] = single_line(pkg, gen.gen_matrix(34, 40, 100, 108).astype("int64") * (2 ** 28 + 3),
                                           1, 2 ** 33, None, int64=True)
         line["L3_24x32"] = l3_line(pkg, gen, a.steps, a.warmup, not a.no_cpu)

@@ -34,8 +34,8 @@ extern "C" {
 #define LNORM_E_UNSUPPORTED (-5) /* requested kernel cannot run this problem */
 
 /* kernel selection */
-#define LNORM_KERNEL_AUTO 0     /* blocked (n = 1), table16 (n >= 2), packed plain16 (n >= 2), else plain,
-                                   wide, general */
+#define LNORM_KERNEL_AUTO 0     /* n = 1: blocked from 2^33 sign vectors at up to 40 |.| columns, else
+                                   table16; n >= 2: table16, packed plain16; else plain, wide, general */
 #define LNORM_KERNEL_PLAIN 1    /* register-resident Gray walk, lane per segment */
 #define LNORM_KERNEL_BLOCKED 2  /* L1/LM: MFMA-built low-row table + scalar high walk (16- or 32-bit words) */
 #define LNORM_KERNEL_GENERAL 3  /* any width: wave per segment, columns over lanes */

@@ -59,7 +59,7 @@ int fail(int code, const char *fmt, ...) {
 
 constexpr int kMaxLabels = 10;      // RANK_OF_NORM, functions.h:3
 constexpr int kMaxCols = 4096;      // length, functions.h:2
-constexpr uint64_t kUnitsPerChunk = 1ull << 20;
+constexpr uint64_t kUnitsPerChunk = 1ull << 22;   // units per launch (16 MiB of unit maxima)
 // Streams besides the main one over which the kernel groups of a batch run
 // concurrently (each stream at the default priority).
 constexpr int kSideStreams = 3;
@@ -83,6 +83,7 @@ struct DevCtx {
   hipEvent_t bFork[2] = {}, bJoin[2][kSideStreams] = {};
   hipGraphExec_t bGraph = nullptr;
   std::vector<uint64_t> bGraphSig;
+  int cus = 0;   // compute units
   bool init = false;
 };
 
@@ -111,6 +112,9 @@ int ensure_result_bufs(DevCtx &c) {
 
 int ctx_init(DevCtx &c) {
   if (c.init) return 0;
+  int dev = 0;
+  HIPCHK(hipGetDevice(&dev));
+  HIPCHK(hipDeviceGetAttribute(&c.cus, hipDeviceAttributeMultiprocessorCount, dev));
   HIPCHK(hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking));
   for (auto &e : c.ev) HIPCHK(hipEventCreate(&e));
   for (auto &st : c.side) HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
@@ -359,9 +363,9 @@ struct Phase1Out {
 // max_units > 0 limits the walk to the first max_units units (the
 // autotuner's probes); the result is then a partial maximum.
 int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometry &g, hipStream_t st,
-                 int block, Phase1Out &out, uint64_t max_units = 0) {
+                 int block, Phase1Out &out, uint64_t max_units = 0, uint64_t chunk_units = 0) {
   if (int r = ensure_result_bufs(cx)) return r;
-  const uint64_t chunk = kUnitsPerChunk;
+  const uint64_t chunk = chunk_units ? std::min(chunk_units, kUnitsPerChunk) : kUnitsPerChunk;
   const uint64_t nunits = max_units ? std::min(max_units, g.nseg) : g.nseg;
   out.units = nunits;
   out.best = INT32_MIN;
@@ -727,18 +731,25 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
   if (kernel == LNORM_KERNEL_PLAIN16 && !c.w16)
     return fail(LNORM_E_UNSUPPORTED, "packed 16-bit walk needs n >= 2, column sums <= 32767 and an instantiated "
                 "width (%d columns at n = %d)", p.C, p.n);
-  if (kernel == LNORM_KERNEL_AUTO)
-    // n = 1: the 16-bit blocked kernel from 2^33 sign vectors (measured 4-11%
-    // ahead of the tabulated walk at 2^35-2^39; below, its waves get too few
-    // high steps: 2^29 ran 27x slower), else the tabulated walk, else the
-    // 32-bit blocked kernel
-    kernel = (can_block && !c.bp.wide && (p.total >= (1ull << 33) || !c.wtab)) ? LNORM_KERNEL_BLOCKED
-                       : (c.wtab || c.wtabw) ? LNORM_KERNEL_TABLE16
-                       : can_block ? LNORM_KERNEL_BLOCKED
-                       : (c.w16 ? LNORM_KERNEL_PLAIN16
-                                : (c.ny_plain ? LNORM_KERNEL_PLAIN
-                                              : ((c.ny_wide || c.w16_wide) ? LNORM_KERNEL_WIDE
-                                                                            : LNORM_KERNEL_GENERAL)));
+  if (kernel == LNORM_KERNEL_AUTO) {
+    // n = 1: the 16-bit blocked kernel from 2^33 sign vectors at up to 20
+    // words (40 |.| columns; measured 1-2% ahead of the tabulated walk there),
+    // else the tabulated walk (1-5% ahead from 22 words at 2^35-2^39, 6-10% at
+    // 2^31; below 2^33 the blocked kernel's waves get too few high steps: 2^29
+    // ran 27x slower), else the 32-bit blocked kernel
+    if (can_block && !c.bp.wide && ((p.total >= (1ull << 33) && c.wtab <= 20) || !c.wtab))
+      kernel = LNORM_KERNEL_BLOCKED;
+    else if (c.wtab || c.wtabw)
+      kernel = LNORM_KERNEL_TABLE16;
+    else if (can_block)
+      kernel = LNORM_KERNEL_BLOCKED;
+    else if (c.w16)
+      kernel = LNORM_KERNEL_PLAIN16;
+    else if (c.ny_plain)
+      kernel = LNORM_KERNEL_PLAIN;
+    else
+      kernel = (c.ny_wide || c.w16_wide) ? LNORM_KERNEL_WIDE : LNORM_KERNEL_GENERAL;
+  }
   if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_TABLE16)
     return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
   c.kernel = kernel;
@@ -879,6 +890,57 @@ int tune_single(DevCtx &cx, const Problem &p, const Choice &c, const lnorm_hip_o
   return 0;
 }
 
+double walk_cost(Walk walk, int d, int width);
+
+// Segment length and launch size of a single problem's tabulated walk.  All
+// lanes of a launch walk the same number of ranks, so a launch of W
+// workgroups takes ceil(W / resident workgroups) rounds of equal length, and
+// the last round of a launch may run a fraction of the chip.  Among segment
+// exponents from 3 below the default (at least the table rows) to 1 above it,
+// this picks the one with the least modelled time: rounds x (segment work +
+// the lane's start-up: the seed and, for d >= 2, all d label tables) plus
+// 30 us per launch, at launch sizes of whole rounds.  Other walks, and a
+// caller's seg_log, keep the default (about 2^20 segments).
+Geometry single_geometry(const DevCtx &cx, const Problem &p, const Prepared &pp, const lnorm_hip_opts &o,
+                         uint64_t &chunk) {
+  chunk = 0;
+  const Geometry def = geometry_for(p.base, p.R - 1, 20, o.seg_log, pp.walk, pp.qmin);
+  if (o.seg_log > 0 || pp.walk != WALK_TAB16 || cx.cus <= 0) return def;
+  const int block = (o.block > 0 && o.block % 64 == 0 && o.block <= 256) ? o.block : 256;
+  const int words = pp.ny_pad / 2, wpb = block / 64;
+  const int nb = p.n == 1 ? tab16_bin_blocks_per_cu(p.kind, words, block) : blk16_blocks_per_cu(p.n, words, block);
+  if (nb <= 0) return def;
+  const uint64_t slots = (uint64_t)nb * cx.cus;              // resident workgroups
+  const uint64_t per_round = slots * wpb * 64;                 // lane segments per round
+  const uint64_t launch = per_round <= kUnitsPerChunk ? kUnitsPerChunk / per_round * per_round : kUnitsPerChunk;
+  const double c = walk_cost(WALK_TAB16, p.n, pp.ny_pad), nw = words;
+  const double start = (p.n == 1 ? 0.0 : (double)p.n * (1 << pp.qmin) * nw) + 2.0 * p.R * nw;
+  // one round: every lane of nb * wpb / 4 waves per SIMD walks its segment,
+  // 4 cycles per wave64 VALU op at about 2.4 GHz
+  const double round_s = 4.0 * nb * wpb / 4.0 / 2.4e9;
+  const int e = p.R - 1;
+  Geometry best = def;
+  double best_t = -1;
+  for (int q = std::max(pp.qmin, def.q - 3); q <= std::min(e, def.q + 1); ++q) {
+    const Geometry g = geometry_for(p.base, e, 63, q, pp.walk, pp.qmin);
+    if (g.q != q) continue;
+    double rounds = 0, launches = 0;
+    for (uint64_t c0 = 0; c0 < g.nseg; c0 += launch) {
+      const uint64_t n = std::min(launch, g.nseg - c0);
+      const uint64_t wg = (2 * ((n + 127) / 128) + wpb - 1) / wpb;
+      rounds += (double)((wg + slots - 1) / slots);
+      launches += 1;
+    }
+    const double t = rounds * ((double)g.seglen * c + start) * round_s + launches * 30e-6;
+    if (best_t < 0 || t < best_t) {
+      best = g;
+      best_t = t;
+    }
+  }
+  chunk = launch;
+  return best;
+}
+
 int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *optsp,
               int32_t *strategy, lnorm_hip_result *res) {
   const auto t_start = std::chrono::steady_clock::now();
@@ -957,8 +1019,9 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "blocked16<np=%d,P=%d,marg=%d,s=%d>", bp.np, bp.P,
                p.kind == KIND_LM, bp.s);
   } else {
-    const Geometry g = geometry_for(p.base, p.R - 1, 20, o.seg_log, pp.walk, pp.qmin);
-    if (int r = run_walk_max(cx, p, pp, g, st, o.block, p1)) return r;
+    uint64_t chunk = 0;
+    const Geometry g = single_geometry(cx, p, pp, o, chunk);
+    if (int r = run_walk_max(cx, p, pp, g, st, o.block, p1, 0, chunk)) return r;
     unit_e = g.q;
     if (general)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "general<kind=%d,d=%d,q=%d>", (int)p.kind, p.n, g.q);
@@ -1161,8 +1224,8 @@ int batch_item(const lnorm_hip_item &it, int i, const lnorm_hip_opts &o, int cap
     }
   } else {
     const int marg = b.p.kind == KIND_LM ? 1 : 0;
-    int wtb = max_col_abs_sum(b.p, marg) <= 32767 ? tab16_bin_words(b.p.C - marg) : 0;
-    if (wtb > 64 || (wtb && b.p.R - 1 < tab_bin_rows(wtb))) wtb = 0;   // batched widths: up to 64 words
+    int wtb = max_col_abs_sum(b.p, marg) <= 32767 ? batch_tb16_words(b.p.C - marg) : 0;   // up to 64 words
+    if (wtb && b.p.R - 1 < tab_bin_rows(wtb)) wtb = 0;
     if (wtb) {
       walk = WALK_TAB16;
       width = 2 * wtb;

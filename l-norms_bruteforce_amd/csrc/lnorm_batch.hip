@@ -161,6 +161,16 @@ static hipError_t lb_tb16(Mode mode, const BatchLaunch &b, hipStream_t st) {
   return hipGetLastError();
 }
 
+// Width in words of a batch item's tabulated L1 / LM walk: multiples of 4 up
+// to 32 words, then 40..64 in steps of 8 (the instantiations below).
+int batch_tb16_words(int nabs) {
+  const int need = std::max(1, (nabs + 1) / 2);
+  if (need <= 32) return (need + 3) / 4 * 4;
+  for (int w : {40, 48, 56, 64})
+    if (need <= w) return w;
+  return 0;
+}
+
 template <bool MARG>
 static hipError_t lb_tb16_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
   switch (b.width / 2) {

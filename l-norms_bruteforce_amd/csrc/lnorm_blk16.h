@@ -43,8 +43,30 @@ static inline hipError_t launch_blk(Mode mode, const PlainArgs &a, hipStream_t s
   return hipErrorInvalidValue;
 }
 
-// one translation unit's share of launch_blk16
+// workgroups of k_blk16_dary<D, words, MODE_MAX> resident per CU at this
+// block size (0: unknown)
+template <int D>
+static inline int blocks_per_cu_blk(int words, int block) {
+  int nb = 0;
+  switch (words) {
+#define X(n)                                                                                           \
+  case n:                                                                                              \
+    if constexpr (n <= blk16_max_words(D))                                                             \
+      if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_blk16_dary<D, n, MODE_MAX>, block, 0) != \
+          hipSuccess)                                                                                  \
+        nb = 0;                                                                                        \
+    break;
+    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64) X(80) X(96)
+#undef X
+    default: break;
+  }
+  return nb;
+}
+
+// one translation unit's share of launch_blk16 and blk16_blocks_per_cu
 hipError_t launch_blk16_lo(Mode mode, const PlainArgs &a, hipStream_t st, int block);   // d = 2..4
 hipError_t launch_blk16_hi(Mode mode, const PlainArgs &a, hipStream_t st, int block);   // d = 5..10
+int blocks_per_cu_blk16_lo(int d, int words, int block);
+int blocks_per_cu_blk16_hi(int d, int words, int block);
 
 }  // namespace lnorm

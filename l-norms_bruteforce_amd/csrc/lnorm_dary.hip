@@ -28,6 +28,19 @@ hipError_t launch_blk16_lo(Mode mode, const PlainArgs &a, hipStream_t st, int bl
   }
 }
 
+int blocks_per_cu_blk16_lo(int d, int words, int block) {
+  switch (d) {
+    case 2: return blocks_per_cu_blk<2>(words, block);
+    case 3: return blocks_per_cu_blk<3>(words, block);
+    case 4: return blocks_per_cu_blk<4>(words, block);
+    default: return 0;
+  }
+}
+
+int blk16_blocks_per_cu(int d, int words, int block) {
+  return d <= 4 ? blocks_per_cu_blk16_lo(d, words, block) : blocks_per_cu_blk16_hi(d, words, block);
+}
+
 hipError_t launch_blk16(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
   if (!a.A || !a.D || !a.unit_best || !a.gbest || !a.gfirst) return hipErrorInvalidValue;
   if (block <= 0 || block % 64 || block > 256) block = 256;

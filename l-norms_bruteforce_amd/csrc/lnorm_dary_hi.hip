@@ -16,4 +16,16 @@ hipError_t launch_blk16_hi(Mode mode, const PlainArgs &a, hipStream_t st, int bl
   }
 }
 
+int blocks_per_cu_blk16_hi(int d, int words, int block) {
+  switch (d) {
+    case 5: return blocks_per_cu_blk<5>(words, block);
+    case 6: return blocks_per_cu_blk<6>(words, block);
+    case 7: return blocks_per_cu_blk<7>(words, block);
+    case 8: return blocks_per_cu_blk<8>(words, block);
+    case 9: return blocks_per_cu_blk<9>(words, block);
+    case 10: return blocks_per_cu_blk<10>(words, block);
+    default: return 0;
+  }
+}
+
 }  // namespace lnorm

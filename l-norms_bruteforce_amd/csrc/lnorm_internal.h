@@ -124,6 +124,7 @@ struct BatchLaunch {
 hipError_t launch_batch_walk(Mode mode, const BatchLaunch &b, hipStream_t st);
 hipError_t launch_batch_t16(Mode mode, const BatchLaunch &b, hipStream_t st);   // the tabulated d-ary groups
 int batch_t16_words(int d, int ny);   // a batch item's tabulated width in words (0: none)
+int batch_tb16_words(int nabs);       // a batch L1 / LM item's tabulated width in words (0: none)
 hipError_t launch_batch_first_unit(const BatchTask *tasks, int nitems, unsigned long long *first, hipStream_t st);
 
 hipError_t launch_plain(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block);
@@ -137,6 +138,8 @@ int blk16_rows(int d, int words);     // its tabulated rows M
 int wt16_words(int d, int ny);        // the W-wave tabulated walk: width in words (0: none)
 int wt16_rows(int d, int words);      // its tabulated rows M
 hipError_t launch_wide_t16(Mode mode, const PlainArgs &a, hipStream_t st);
+int blk16_blocks_per_cu(int d, int words, int block);             // resident workgroups of the
+int tab16_bin_blocks_per_cu(Kind kind, int words, int block);     // tabulated walks' kernels (0: unknown)
 int wtb16_words(int nabs);            // the W-wave tabulated L1 / LM walk: width in words (0: none)
 int wtb16_rows(int words);            // its tabulated rows M
 hipError_t launch_wide_tb16(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st);

@@ -13,7 +13,8 @@ __global__ __launch_bounds__(256) void k_tab16_bin(
                                  global_wave());
 }
 
-static const int kTabBinWords[] = {4, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96, 128, 160, 192};
+// every even width up to 32 words (the blocked kernel's widths), then steps of 8 and 16
+static const int kTabBinWords[] = {4, 6, 8, 10, 12, 14, 16, 18, 20, 22, 24, 26, 28, 30, 32, 40, 48, 56, 64, 80, 96, 128, 160, 192};
 
 int tab16_bin_words(int nabs) {
   const int need = (nabs + 1) / 2;
@@ -41,11 +42,37 @@ template <bool MARG>
 static hipError_t launch_tb(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
   switch (a.ny_pad / 2) {
 #define X(n) case n: return launch_tb_nw<n, MARG>(mode, a, st, block);
-    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64) X(80) X(96) X(128) X(160) X(192)
+    X(4) X(6) X(8) X(10) X(12) X(14) X(16) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) X(40) X(48) X(56) X(64) \
+    X(80) X(96) X(128) X(160) X(192)
 #undef X
     default: break;
   }
   return hipErrorInvalidValue;
+}
+
+// workgroups of k_tab16_bin<words, MARG, MODE_MAX> resident per CU at this
+// block size (0: unknown)
+template <bool MARG>
+static int blocks_per_cu_tb(int words, int block) {
+  int nb = 0;
+  switch (words) {
+#define X(n)                                                                                                      \
+  case n:                                                                                                         \
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_tab16_bin<n, MARG, MODE_MAX>, block, 0) != hipSuccess) \
+      nb = 0;                                                                                                     \
+    break;
+    X(4) X(6) X(8) X(10) X(12) X(14) X(16) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) X(40) X(48) X(56) X(64) \
+    X(80) X(96) X(128) X(160) X(192)
+#undef X
+    default: break;
+  }
+  return nb;
+}
+
+int tab16_bin_blocks_per_cu(Kind kind, int words, int block) {
+  if (kind == KIND_L1) return blocks_per_cu_tb<false>(words, block);
+  if (kind == KIND_LM) return blocks_per_cu_tb<true>(words, block);
+  return 0;
 }
 
 hipError_t launch_tab16_bin(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block) {

@@ -365,13 +365,14 @@ def test_table16_walk_vs_oracle(pkg, d, R, cols):
 @pytest.mark.parametrize("marg", [0, 1])
 def test_table16_binary_vs_oracle(pkg, marg):
     """The tabulated L1 / LM walk (the 5 fastest rows as a sign-pattern
-    table): bit-exact norm, strategy and rank at every width (4..64 words of
+    table): bit-exact norm, strategy and rank at every width (4..32 words in
+    steps of 2, then 40..64 words of
     |.| columns, odd counts, LM's column 0 outside the words), at forced
     segment lengths, with exactly 5 enumerated rows, and AUTO's choice of it
     beyond the 16-bit blocked kernel's 64 |.| columns."""
     rng = np.random.default_rng(950 + marg)
     for R in (6, 9, 14):
-        for C in (1 + marg, 2, 7, 8, 9, 16, 17, 31, 33, 64, 65, 100, 128 + marg):
+        for C in (1 + marg, 2, 7, 8, 9, 11, 16, 17, 19, 27, 31, 33, 35, 43, 51, 59, 64, 65, 100, 128 + marg):
             A = _rand(rng, R, C, 60)
             print("table16 L1", marg, A.shape, flush=True)
             res = _agree(pkg, A, 1, marg, kernel="table16")
