@@ -34,11 +34,12 @@ MFMA_I8_PEAK = 5.0e15                   # dense int8 ops/s
 # rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE on `bench.py --case blocked`
 # (with and without --no-autotune), each scaled by the counter's reading for a
 # 1 GiB device-to-device copy (x2.0 for FETCH_SIZE, x1.0 for WRITE_SIZE):
-#   P=2, s=11 (autotuned): 128.5 KiB and 40960 KiB per launch of 2^20 waves;
+#   P=2, s=11 (autotuned): 130.1 KiB and 81920 KiB per launch of 2^21 waves
+#                          (one launch per call);
 #   P=4, s=12 (default):   207.6 KiB and 20480 KiB per launch of 2^19 waves.
 # Only the per-wave maxima reach HBM; the matrix and the pattern table stay
 # on chip.
-PROFILED_TRAFFIC = {"blocked16<np=32,P=2,marg=0,s=11>": 263168 + 41943040,
+PROFILED_TRAFFIC = {"blocked16<np=32,P=2,marg=0,s=11>": 266445 + 83886080,
                     "blocked16<np=32,P=4,marg=0,s=12>": 425216 + 20971520}
 
 
@@ -529,7 +530,7 @@ def main():
             "unit": "TOP/s",
             "frac": sb["valu_frac"],
             "traffic": PROFILED_TRAFFIC.get(sb["kernel_name"]),
-            "traffic_per": "launch (2^20 waves at most per launch)",
+            "traffic_per": "launch (2^22 waves at most per launch; one per call at the tuned shape)",
             "ops_per_sign_vector": sb["valu_ops_per_sign_vector"],
             "kernel_ms": sb["kernel_ms"],
         },

@@ -80,13 +80,22 @@ __global__ __launch_bounds__(256) void k_blocked16(
 
   const uint32_t nsteps = 1u << s;
   const uint32_t nu = (uint32_t)(w & 1) << s;   // bit s of N_high
-  for (uint32_t t = 1; t < nsteps; ++t) {
+  // the packed delta of step t: row j = ctz t flips to 1 ^ (bit j+1 of N_high)
+  auto delta = [&](uint32_t t) {
     const int j = __builtin_ctz(t);
     const uint32_t up = (((nu | t) >> (j + 1)) & 1u) ^ 1u;
-    const uint32_t *dp = dpk + ((size_t)up * (uint32_t)nh + (uint32_t)j) * NW;
+    return dpk + ((size_t)up * (uint32_t)nh + (uint32_t)j) * NW;
+  };
+  // each step's delta address comes from the step before, and a scheduling
+  // barrier keeps its scalar loads at the top of the iteration: their latency
+  // hides behind eval() instead of stalling the wave just before the adds
+  const uint32_t *dp = delta(1);
+  for (uint32_t t = 1; t < nsteps; ++t) {
     uint32_t dl[NW];
 #pragma unroll
     for (int c = 0; c < NW; ++c) dl[c] = dp[c];
+    dp = delta(t + 1 < nsteps ? t + 1 : 1);
+    __builtin_amdgcn_sched_barrier(0);
     eval();                      // state of rank t-1
 #pragma unroll
     for (int c = 0; c < NW; ++c) S[c] += dl[c];
