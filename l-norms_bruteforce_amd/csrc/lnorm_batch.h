@@ -40,4 +40,43 @@ static inline unsigned batch_grid(uint64_t ntasks) {
   return (unsigned)(blocks < cap ? (blocks ? blocks : 1) : cap);
 }
 
+// The batched tabulated d-ary walk (walk_blk16 per task), instantiated by
+// lnorm_batch_tab.hip (d = 2..4) and lnorm_batch_tab_hi.hip (d = 5..10).
+template <int D, int NW, int MODE>
+__global__ __launch_bounds__(256) void k_batch_t16(const BatchTask *__restrict__ tasks, int nitems,
+                                                   uint64_t ntasks, unsigned int *__restrict__ counter) {
+  int shard = (int)(blockIdx.x & (kCounterShards - 1)), left = kCounterShards;
+  uint32_t t;
+  while (next_task(counter, ntasks, shard, left, t)) {
+    int lo = 0, hi = nitems - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (tasks[mid].task0 <= t) lo = mid;
+      else hi = mid - 1;
+    }
+    const BatchTask &it = tasks[lo];
+    uint64_t seg0 = 0;
+    int32_t target = 0;
+    if (MODE == MODE_FIRST) {
+      seg0 = *it.unit * it.seg_base;
+      target = *it.gbest;
+    }
+    walk_blk16<D, NW, blk16_low_rows(D, NW), MODE>(as_const((const uint32_t *)it.A), as_const((const uint32_t *)it.D),
+                                                    it.R, it.q, seg0, it.nseg, target, it.unit_best, it.gbest,
+                                                    it.gfirst, t - it.task0);
+  }
+}
+
+template <int D, int NW>
+static inline hipError_t lb_t16(Mode mode, const BatchLaunch &b, hipStream_t st) {
+  const dim3 grid(batch_grid(b.ntasks)), blk(256);
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_batch_t16<D, NW, MODE_MAX>), grid, blk, 0, st, b.tasks, b.nitems, b.ntasks, b.counter);
+  else
+    hipLaunchKernelGGL((k_batch_t16<D, NW, MODE_FIRST>), grid, blk, 0, st, b.tasks, b.nitems, b.ntasks, b.counter);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_t16_hi(Mode mode, const BatchLaunch &b, hipStream_t st);   // d = 5..10
+
 }  // namespace lnorm

@@ -161,11 +161,11 @@ static hipError_t lb_tb16(Mode mode, const BatchLaunch &b, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Width in words of a batch item's tabulated L1 / LM walk: multiples of 4 up
-// to 32 words, then 40..64 in steps of 8 (the instantiations below).
+// Width in words of a batch item's tabulated L1 / LM walk: even counts from 4
+// up to 32 words, then 40..64 in steps of 8 (the instantiations below).
 int batch_tb16_words(int nabs) {
   const int need = std::max(1, (nabs + 1) / 2);
-  if (need <= 32) return (need + 3) / 4 * 4;
+  if (need <= 32) return std::max(4, (need + 1) / 2 * 2);
   for (int w : {40, 48, 56, 64})
     if (need <= w) return w;
   return 0;
@@ -175,7 +175,7 @@ template <bool MARG>
 static hipError_t lb_tb16_w(Mode mode, const BatchLaunch &b, hipStream_t st) {
   switch (b.width / 2) {
 #define X(n) case n: return lb_tb16<n, MARG>(mode, b, st);
-    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64)
+    X(4) X(6) X(8) X(10) X(12) X(14) X(16) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) X(40) X(48) X(56) X(64)
 #undef X
     default: break;
   }

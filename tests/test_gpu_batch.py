@@ -47,20 +47,25 @@ def test_batch_mixed_vs_oracle(pkg):
 
 
 def test_batch_table_widths_vs_oracle(pkg):
-    """The batched tabulated walk at every width (4..32 words in steps of 4,
-    up to one wave's table width for each d = 2..10), at column counts on
-    both sides of each width and with the fewest rows it tabulates."""
+    """The batched tabulated walks at every width (d = 2..4 and L1 / LM: 4..32
+    words in steps of 2; d = 5..10: steps of 4, up to one wave's table
+    width), at column counts on both sides of each width and with the fewest
+    rows they tabulate."""
     print("batch table widths vs oracle", flush=True)
     rng = np.random.default_rng(47)
-    rows = {2: (6, 12), 3: (6, 8), 4: (5, 7), 5: (4, 6), 6: (4, 5), 7: (3, 5), 8: (3, 4), 9: (3, 4), 10: (3, 4)}
-    maxw = {2: 32, 3: 32, 4: 32, 5: 32, 6: 32, 7: 32, 8: 24, 9: 24, 10: 16}
+    rows = {1: (6, 14), 2: (6, 12), 3: (6, 8), 4: (5, 7), 5: (4, 6), 6: (4, 5), 7: (3, 5), 8: (3, 4), 9: (3, 4),
+            10: (3, 4)}
+    maxw = {1: 64, 2: 32, 3: 32, 4: 32, 5: 32, 6: 32, 7: 32, 8: 24, 9: 24, 10: 16}
     probs = []
     for d, (r0, r1) in rows.items():
-        for C in (1, 7, 8, 9, 16, 17, 24, 25, 33, 40, 41, 48, 49, 56, 57, 64):
+        for C in (1, 7, 8, 9, 11, 16, 17, 21, 24, 25, 29, 33, 40, 41, 45, 48, 49, 53, 56, 57, 61, 64):
             if (C + 1) // 2 > maxw[d]:
                 continue
-            R = int(rng.integers(r0, r1 + 1))
-            probs.append((_rand(rng, R, C, 40), d, 0))
+            for marg in ((0, 1) if d == 1 else (0,)):
+                if C <= marg:
+                    continue
+                R = int(rng.integers(r0, r1 + 1))
+                probs.append((_rand(rng, R, C, 40), d, marg))
     out, st = pkg.calc_batch(probs, stats=True)
     assert st["fallback"] == 0, st
     for (A, n, marg), (norm, strat, status) in zip(probs, out):
