@@ -924,13 +924,14 @@ Geometry single_geometry(const DevCtx &cx, const Problem &p, const Prepared &pp,
   for (int q = std::max(pp.qmin, def.q - 3); q <= std::min(e, def.q + 1); ++q) {
     const Geometry g = geometry_for(p.base, e, 63, q, pp.walk, pp.qmin);
     if (g.q != q) continue;
-    double rounds = 0, launches = 0;
-    for (uint64_t c0 = 0; c0 < g.nseg; c0 += launch) {
-      const uint64_t n = std::min(launch, g.nseg - c0);
+    // rounds of a launch of n lane segments; full launches, then the rest
+    auto launch_rounds = [&](uint64_t n) {
       const uint64_t wg = (2 * ((n + 127) / 128) + wpb - 1) / wpb;
-      rounds += (double)((wg + slots - 1) / slots);
-      launches += 1;
-    }
+      return (double)((wg + slots - 1) / slots);
+    };
+    const uint64_t full = g.nseg / launch, rest = g.nseg % launch;
+    const double rounds = (double)full * launch_rounds(launch) + (rest ? launch_rounds(rest) : 0.0);
+    const double launches = (double)full + (rest ? 1.0 : 0.0);
     const double t = rounds * ((double)g.seglen * c + start) * round_s + launches * 30e-6;
     if (best_t < 0 || t < best_t) {
       best = g;
