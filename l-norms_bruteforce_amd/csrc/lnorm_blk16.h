@@ -32,11 +32,20 @@ static inline hipError_t launch_blk_nw(Mode mode, const PlainArgs &a, hipStream_
   return hipGetLastError();
 }
 
+// instantiated widths in words: multiples of 4 up to 32 (every even count for
+// d <= 4), then 40..64 in steps of 8, 80 and 96; at most one wave's table
+constexpr bool blk16_has_width(int d, int nw) {
+  const bool listed = nw <= 32 ? nw >= 4 && (nw % 4 == 0 || (d <= 4 && nw % 2 == 0))
+                               : (nw == 40 || nw == 48 || nw == 56 || nw == 64 || nw == 80 || nw == 96);
+  return listed && nw <= blk16_max_words(d);
+}
+
 template <int D>
 static inline hipError_t launch_blk(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
   switch (a.ny_pad / 2) {
-#define X(n) case n: if constexpr (n <= blk16_max_words(D)) return launch_blk_nw<D, n>(mode, a, st, block); break;
-    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64) X(80) X(96)
+#define X(n) case n: if constexpr (blk16_has_width(D, n)) return launch_blk_nw<D, n>(mode, a, st, block); break;
+    X(4) X(6) X(8) X(10) X(12) X(14) X(16) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) X(40) X(48) X(56) \
+    X(64) X(80) X(96)
 #undef X
     default: break;
   }
@@ -51,12 +60,13 @@ static inline int blocks_per_cu_blk(int words, int block) {
   switch (words) {
 #define X(n)                                                                                           \
   case n:                                                                                              \
-    if constexpr (n <= blk16_max_words(D))                                                             \
+    if constexpr (blk16_has_width(D, n))                                                               \
       if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_blk16_dary<D, n, MODE_MAX>, block, 0) != \
           hipSuccess)                                                                                  \
         nb = 0;                                                                                        \
     break;
-    X(4) X(8) X(12) X(16) X(20) X(24) X(28) X(32) X(40) X(48) X(56) X(64) X(80) X(96)
+    X(4) X(6) X(8) X(10) X(12) X(14) X(16) X(18) X(20) X(22) X(24) X(26) X(28) X(30) X(32) X(40) X(48) X(56) \
+    X(64) X(80) X(96)
 #undef X
     default: break;
   }

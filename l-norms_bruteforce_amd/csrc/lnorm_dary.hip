@@ -6,14 +6,12 @@
 
 namespace lnorm {
 
-// widths in words: steps of 4 up to 32 words, then of 8 and 16
-static const int kBlkWords[] = {4, 8, 12, 16, 20, 24, 28, 32, 40, 48, 56, 64, 80, 96};
-
+// the smallest instantiated width (blk16_has_width) holding the columns
 int blk16_words(int d, int ny) {
-  const int need = (ny + 1) / 2;
-  const int mx = blk16_max_words(d);
   if (d < 2 || d > 10) return 0;
-  for (int w : kBlkWords) if (w >= need && w <= mx) return w;
+  const int need = (ny + 1) / 2;
+  for (int w = 4; w <= 96; w += 2)
+    if (w >= need && blk16_has_width(d, w)) return w;
   return 0;
 }
 
