@@ -297,6 +297,8 @@ TABLE_MAX_WORDS = {2: 32, 3: 32, 4: 32, 5: 32, 6: 32, 7: 32, 8: 24, 9: 24, 10: 1
 
 
 def table16_rows(d: int, words: int) -> int:
+    if d == 3 and words <= 16:
+        return 6
     if d <= 3:
         return 5 if words <= 32 else 4
     return {4: 4 if words <= 32 else 3 if words <= 40 else 2, 5: 3 if words <= 32 else 2,

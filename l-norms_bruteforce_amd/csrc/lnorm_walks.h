@@ -558,7 +558,7 @@ constexpr int blk16_low_rows(int d, int nw) {
   if (nw > blk16_max_words(d)) return 0;
   switch (d) {
     case 2: return nw <= 32 ? 5 : 4;
-    case 3: return nw <= 32 ? 5 : 4;
+    case 3: return nw <= 16 ? 6 : nw <= 32 ? 5 : 4;   // 6 rows: 729 labelings per block (247 VGPRs at 16 words)
     case 4: return nw <= 32 ? 4 : nw <= 40 ? 3 : 2;
     case 5: return nw <= 32 ? 3 : 2;
     case 6: return nw <= 24 ? 3 : 2;

@@ -9,9 +9,9 @@ from oracle import host_oracle as ho
 
 pytestmark = pytest.mark.gpu
 
-# rows the tabulated walk keeps in its subset tables, per d (up to 64 columns;
-# 4 at d = 3 beyond 64 columns)
-TABLE_ROWS = {2: 5, 3: 5, 4: 4, 5: 3, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
+# rows the tabulated walk keeps in its subset tables, per d, at up to 16 words
+# (32 columns); d = 3 keeps 5 rows from 17 to 32 words and 4 beyond
+TABLE_ROWS = {2: 5, 3: 6, 4: 4, 5: 3, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
 
 
 def _laid_out(case):
