@@ -301,7 +301,7 @@ def table16_rows(d: int, words: int) -> int:
         return 6
     if d <= 3:
         return 5 if words <= 32 else 4
-    return {4: 4 if words <= 32 else 3 if words <= 40 else 2, 5: 3 if words <= 32 else 2,
+    return {4: 4 if words <= 32 else 3 if words <= 40 else 2, 5: 4 if words <= 16 else 3 if words <= 32 else 2,
             6: 3 if words <= 24 else 2, 7: 2 if words <= 24 else 1, 8: 2, 9: 2 if words <= 16 else 1,
             10: 2}[d]
 

@@ -560,7 +560,7 @@ constexpr int blk16_low_rows(int d, int nw) {
     case 2: return nw <= 32 ? 5 : 4;
     case 3: return nw <= 16 ? 6 : nw <= 32 ? 5 : 4;   // 6 rows: 729 labelings per block (247 VGPRs at 16 words)
     case 4: return nw <= 32 ? 4 : nw <= 40 ? 3 : 2;
-    case 5: return nw <= 32 ? 3 : 2;
+    case 5: return nw <= 16 ? 4 : nw <= 32 ? 3 : 2;   // 4 rows: 625 labelings per block (234 VGPRs at 16 words)
     case 6: return nw <= 24 ? 3 : 2;
     case 7: return nw <= 24 ? 2 : 1;
     case 8: return 2;

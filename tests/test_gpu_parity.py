@@ -10,8 +10,9 @@ from oracle import host_oracle as ho
 pytestmark = pytest.mark.gpu
 
 # rows the tabulated walk keeps in its subset tables, per d, at up to 16 words
-# (32 columns); d = 3 keeps 5 rows from 17 to 32 words and 4 beyond
-TABLE_ROWS = {2: 5, 3: 6, 4: 4, 5: 3, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
+# (32 columns); d = 3 keeps 5 rows from 17 to 32 words and 4 beyond, d = 5 3
+# rows from 17 to 32 words
+TABLE_ROWS = {2: 5, 3: 6, 4: 4, 5: 4, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
 
 
 def _laid_out(case):
