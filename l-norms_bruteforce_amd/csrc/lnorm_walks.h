@@ -561,8 +561,8 @@ constexpr int blk16_low_rows(int d, int nw) {
     case 3: return nw <= 16 ? 6 : nw <= 32 ? 5 : 4;   // 6 rows: 729 labelings per block (247 VGPRs at 16 words)
     case 4: return nw <= 32 ? 4 : nw <= 40 ? 3 : 2;
     case 5: return nw <= 16 ? 4 : nw <= 32 ? 3 : 2;   // 4 rows: 625 labelings per block (234 VGPRs at 16 words)
-    case 6: return nw <= 24 ? 3 : 2;
-    case 7: return nw <= 24 ? 2 : 1;
+    case 6: return nw <= 8 ? 4 : nw <= 24 ? 3 : 2;   // 4 rows: 1296 labelings per block (230 VGPRs at 8 words)
+    case 7: return nw <= 16 ? 3 : nw <= 24 ? 2 : 1;   // 3 rows: 343 labelings per block (192 VGPRs at 16 words)
     case 8: return 2;
     case 9: return nw <= 16 ? 2 : 1;
     case 10: return 2;

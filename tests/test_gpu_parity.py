@@ -9,10 +9,11 @@ from oracle import host_oracle as ho
 
 pytestmark = pytest.mark.gpu
 
-# rows the tabulated walk keeps in its subset tables, per d, at up to 16 words
-# (32 columns); d = 3 keeps 5 rows from 17 to 32 words and 4 beyond, d = 5 3
-# rows from 17 to 32 words
-TABLE_ROWS = {2: 5, 3: 6, 4: 4, 5: 4, 6: 3, 7: 2, 8: 2, 9: 2, 10: 2}
+# rows the tabulated walk keeps in its subset tables, per d, at up to 8 words
+# (16 columns); beyond, d = 3 keeps 6 rows up to 16 words, 5 up to 32 and 4
+# beyond, d = 5 4 rows up to 16 words and 3 up to 32, d = 6 3 rows up to 24
+# words, d = 7 3 rows up to 16 words and 2 up to 24
+TABLE_ROWS = {2: 5, 3: 6, 4: 4, 5: 4, 6: 4, 7: 3, 8: 2, 9: 2, 10: 2}
 
 
 def _laid_out(case):
