@@ -302,9 +302,8 @@ def table16_rows(d: int, words: int) -> int:
     if d <= 3:
         return 5 if words <= 32 else 4
     return {4: 4 if words <= 32 else 3 if words <= 40 else 2, 5: 4 if words <= 16 else 3 if words <= 32 else 2,
-            6: 4 if words <= 8 else 3 if words <= 24 else 2, 7: 3 if words <= 16 else 2 if words <= 24 else 1, 8: 2,
-            9: 2 if words <= 16 else 1,
-            10: 2}[d]
+            6: 4 if words <= 8 else 3 if words <= 24 else 2, 7: 3 if words <= 16 else 2 if words <= 24 else 1,
+            8: 3 if words <= 16 else 2, 9: 3 if words <= 16 else 1, 10: 3 if words <= 8 else 2}[d]
 
 
 def table16_ops(d: int, words: int) -> float:

@@ -563,9 +563,9 @@ constexpr int blk16_low_rows(int d, int nw) {
     case 5: return nw <= 16 ? 4 : nw <= 32 ? 3 : 2;   // 4 rows: 625 labelings per block (234 VGPRs at 16 words)
     case 6: return nw <= 8 ? 4 : nw <= 24 ? 3 : 2;   // 4 rows: 1296 labelings per block (230 VGPRs at 8 words)
     case 7: return nw <= 16 ? 3 : nw <= 24 ? 2 : 1;   // 3 rows: 343 labelings per block (192 VGPRs at 16 words)
-    case 8: return 2;
-    case 9: return nw <= 16 ? 2 : 1;
-    case 10: return 2;
+    case 8: return nw <= 16 ? 3 : 2;   // 3 rows: 512 labelings per block (216 VGPRs at 16 words)
+    case 9: return nw <= 16 ? 3 : 1;   // 729 labelings per block (244 VGPRs at 16 words)
+    case 10: return nw <= 8 ? 3 : 2;   // 1000 labelings per block up to 8 words
     default: return 0;
   }
 }

@@ -12,8 +12,8 @@ pytestmark = pytest.mark.gpu
 # rows the tabulated walk keeps in its subset tables, per d, at up to 8 words
 # (16 columns); beyond, d = 3 keeps 6 rows up to 16 words, 5 up to 32 and 4
 # beyond, d = 5 4 rows up to 16 words and 3 up to 32, d = 6 3 rows up to 24
-# words, d = 7 3 rows up to 16 words and 2 up to 24
-TABLE_ROWS = {2: 5, 3: 6, 4: 4, 5: 4, 6: 4, 7: 3, 8: 2, 9: 2, 10: 2}
+# words, d = 7, 8, 9 3 rows up to 16 words, d = 10 2 rows
+TABLE_ROWS = {2: 5, 3: 6, 4: 4, 5: 4, 6: 4, 7: 3, 8: 3, 9: 3, 10: 3}
 
 
 def _laid_out(case):
