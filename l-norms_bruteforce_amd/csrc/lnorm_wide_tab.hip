@@ -23,9 +23,16 @@ namespace lnorm {
 // d = 5, 6 (register room).
 constexpr int wt16_nw(int d) { return d <= 7 ? 32 : (d == 8 || d == 9) ? 24 : d == 10 ? 16 : 0; }
 constexpr int wt16_wmax(int d) { return d <= 6 ? 8 : 4; }
+// the table rows of the W-wave walk: the single wave's rows at its width,
+// capped by LDS and registers.  d = 3: the W waves' partial tables of two
+// labels fill up to 64 KiB (W * 2 * 2^M * 256 B), 5 rows at 2 and 4 waves
+// and 4 at 8 (13.8x the packed W-wave walk at 256 columns, 9x with one row
+// less).  Other d keep 32 KiB: 5, 4 and 3 rows (2 for d >= 5 over 8 waves;
+// more spill at 8 waves, and at d = 2 both labels change every step, so a
+// row more does not cut the table work per labeling).
 constexpr int wt16_m(int d, int w) {
   const int m = blk16_low_rows(d, wt16_nw(d));
-  const int cap = w >= 8 ? (d >= 5 ? 2 : 3) : w >= 4 ? 4 : 5;
+  const int cap = d == 3 ? (w >= 8 ? 4 : w >= 4 ? 5 : 6) : w >= 8 ? (d >= 5 ? 2 : 3) : w >= 4 ? 4 : 5;
   return m < cap ? m : cap;
 }
 
@@ -136,8 +143,11 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
   constexpr int NWT = NW * W;
   constexpr int NPAT = LowCode<D, M>::NPAT;
   constexpr int PW = 2 * (1 << M) * 64;   // one wave's partial tables (words)
+  static_assert(W * PW * 4 <= 65536, "partial tables exceed 64 KiB");
   __shared__ uint32_t part[W * PW];
-  __shared__ uint32_t fin[W * 64];
+  // per-wave results, once the last partial tables are summed: every use of
+  // part ends with a barrier
+  uint32_t *fin = part;
   const cuint32_t *Ap = as_const(Ap_);
   const int lane = wave_lane();
   const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
