@@ -24,17 +24,27 @@ namespace lnorm {
 constexpr int wt16_nw(int d) { return d <= 7 ? 32 : (d == 8 || d == 9) ? 24 : d == 10 ? 16 : 0; }
 constexpr int wt16_wmax(int d) { return d <= 6 ? 8 : 4; }
 // the table rows of the W-wave walk: the single wave's rows at its width,
-// capped by LDS and registers.  d = 3: the W waves' partial tables of two
-// labels fill up to 64 KiB (W * 2 * 2^M * 256 B), 5 rows at 2 and 4 waves
-// and 4 at 8 (13.8x the packed W-wave walk at 256 columns, 9x with one row
-// less).  Other d keep 32 KiB: 5, 4 and 3 rows (2 for d >= 5 over 8 waves;
-// more spill at 8 waves, and at d = 2 both labels change every step, so a
-// row more does not cut the table work per labeling).
+// capped by LDS and registers.  LDS: the W waves' partial tables of two
+// labels (W * 2 * 2^M * 256 B) fill up to 64 KiB for d = 3 and 32 KiB for
+// other d (at d = 2 both labels change every step, so a row more does not
+// cut the table work per labeling: it ran 1.7x slower).  Registers: above 4
+// waves a workgroup puts 2 waves on a SIMD (256 VGPRs each), where 4 rows fit
+// at d = 3, 3 at d = 2 and 4 and 2 beyond.  d = 3 over 4 waves: 13.8x the
+// packed W-wave walk at 256 columns, 9x with one row less.
 constexpr int wt16_m(int d, int w) {
   const int m = blk16_low_rows(d, wt16_nw(d));
-  const int cap = d == 3 ? (w >= 8 ? 4 : w >= 4 ? 5 : 6) : w >= 8 ? (d >= 5 ? 2 : 3) : w >= 4 ? 4 : 5;
+  const int lds = d == 3 ? 128 : 64;   // 2^M * W at most
+  int capl = 0;
+  while ((2 << capl) * w <= lds) ++capl;
+  const int capr = w > 4 ? (d == 3 ? 4 : d >= 5 ? 2 : 3) : 6;
+  const int cap = capl < capr ? capl : capr;
   return m < cap ? m : cap;
 }
+
+// wave counts of the W-wave tabulated walk: at most wt16_wmax(d), and not 5
+// at d = 6 (its 2 table rows spilled there)
+constexpr int kWtWaves[] = {2, 3, 4, 5, 6, 8};
+constexpr bool wt16_has(int d, int w) { return w <= wt16_wmax(d) && !(d == 6 && w == 5); }
 
 // This wave's partial tables of labels C and C + 1 into its LDS slice
 // (p[e][S][lane], e = 0 for C, 1 for C + 1).
@@ -490,8 +500,8 @@ hipError_t launch_wide_tb16(Kind kind, Mode mode, const PlainArgs &a, hipStream_
 int wt16_words(int d, int ny) {
   const int nw = (d >= 2 && d <= 10) ? wt16_nw(d) : 0;
   const int need = (ny + 1) / 2;
-  for (int w = 2; nw > 0 && w <= wt16_wmax(d); w *= 2)
-    if (need <= nw * w) return nw * w;
+  for (int w : kWtWaves)
+    if (nw > 0 && wt16_has(d, w) && need <= nw * w) return nw * w;
   return 0;
 }
 
@@ -520,8 +530,13 @@ template <int D>
 static hipError_t lwt_d(Mode mode, const PlainArgs &a, hipStream_t st) {
   const int words = a.ny_pad / 2;
   if (words == 2 * wt16_nw(D)) return lwt<D, 2>(mode, a, st);
+  if (words == 3 * wt16_nw(D)) return lwt<D, 3>(mode, a, st);
   if (words == 4 * wt16_nw(D)) return lwt<D, 4>(mode, a, st);
-  if constexpr (wt16_wmax(D) >= 8)
+  if constexpr (wt16_has(D, 5))
+    if (words == 5 * wt16_nw(D)) return lwt<D, 5>(mode, a, st);
+  if constexpr (wt16_has(D, 6))
+    if (words == 6 * wt16_nw(D)) return lwt<D, 6>(mode, a, st);
+  if constexpr (wt16_has(D, 8))
     if (words == 8 * wt16_nw(D)) return lwt<D, 8>(mode, a, st);
   return hipErrorInvalidValue;
 }
