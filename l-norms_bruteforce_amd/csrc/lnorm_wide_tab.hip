@@ -293,9 +293,10 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
 // 2^M patterns' partial sums over its words and stores them in LDS; after a
 // barrier wave wv totals the W partials of its patterns (l = wv, wv + W, ...)
 // and keeps their maximum (MODE_MAX) or first match (MODE_FIRST).
-// Shapes (W, NW): (8, 32), (16, 32), (16, 64) for 512, 1024, 2048 columns.
+// Shapes (W, NW): (8, 32), (10, 32), (12, 32), (16, 32), (16, 64) for 512, 640,
+// 768, 1024, 2048 columns.
 // ---------------------------------------------------------------------------
-constexpr int wtb_m(int w) { return w >= 16 ? 4 : 5; }   // partials: W * 2^M * 256 B = 64 KiB
+constexpr int wtb_m(int w) { return w > 8 ? 4 : 5; }   // partials: W * 2^M * 256 B <= 64 KiB
 
 template <int NW, int NWT, int M, bool MARG, int... Ss>
 __device__ __forceinline__ void wtb_partials(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
@@ -451,16 +452,16 @@ __global__ __launch_bounds__(64 * W) void k_wide_tb16(
   }
 }
 
-// Widths of the W-wave tabulated L1 / LM walk: 256, 512 or 1024 words (512,
-// 1024 or 2048 |.| columns).
+// Widths of the W-wave tabulated L1 / LM walk: 256, 320, 384, 512 or 1024
+// words (8, 10, 12 or 16 waves of 32 words, 16 of 64; up to 2048 |.| columns).
 int wtb16_words(int nabs) {
   const int need = (nabs + 1) / 2;
-  for (int w : {256, 512, 1024})
+  for (int w : {256, 320, 384, 512, 1024})
     if (need <= w) return w;
   return 0;
 }
 
-int wtb16_rows(int words) { return wtb_m(words == 256 ? 8 : 16); }
+int wtb16_rows(int words) { return wtb_m(words == 1024 ? 16 : words / 32); }
 
 template <int W, int NW, bool MARG>
 static hipError_t lwtb(Mode mode, const PlainArgs &a, hipStream_t st) {
@@ -481,6 +482,8 @@ template <bool MARG>
 static hipError_t lwtb_m(Mode mode, const PlainArgs &a, hipStream_t st) {
   switch (a.ny_pad / 2) {
     case 256: return lwtb<8, 32, MARG>(mode, a, st);
+    case 320: return lwtb<10, 32, MARG>(mode, a, st);
+    case 384: return lwtb<12, 32, MARG>(mode, a, st);
     case 512: return lwtb<16, 32, MARG>(mode, a, st);
     case 1024: return lwtb<16, 64, MARG>(mode, a, st);
     default: return hipErrorInvalidValue;

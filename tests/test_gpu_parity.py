@@ -413,8 +413,8 @@ def test_table16_binary_wide_vs_oracle(pkg, marg):
 
 @pytest.mark.parametrize("marg", [0, 1])
 def test_table16w_binary_vs_oracle(pkg, marg):
-    """The tabulated L1 / LM walk over W waves beyond 384 |.| columns (8 waves
-    of 32 words, 16 of 32, 16 of 64: partial pattern sums through LDS):
+    """The tabulated L1 / LM walk over W waves beyond 384 |.| columns (8, 10,
+    12 or 16 waves of 32 words, 16 of 64: partial pattern sums through LDS):
     bit-exact norm, strategy and rank at every shape, at forced segment
     lengths, with the fewest enumerated rows it accepts, and an error below
     that."""
@@ -425,7 +425,7 @@ def test_table16w_binary_vs_oracle(pkg, marg):
             print("table16w L1", marg, A.shape, flush=True)
             res = _agree(pkg, A, 1, marg)
             assert res["kernel_name"].startswith("table16w<L"), res["kernel_name"]
-    for C, M in ((450, 5), (900, 4), (1800, 4)):
+    for C, M in ((450, 5), (620, 4), (740, 4), (900, 4), (1800, 4)):
         A = _rand(rng, 14, C, 60)
         for q in (M, M + 1, M + 3, 13):
             print("table16w L1 seg_log", marg, A.shape, q, flush=True)
