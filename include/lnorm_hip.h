@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define LNORM_HIP_ABI_VERSION 4
+#define LNORM_HIP_ABI_VERSION 5
 
 /* error codes */
 #define LNORM_OK 0
@@ -103,6 +103,35 @@ int lnorm_hip_calc_Lnorm(const int32_t *mtx_as_vec, int iRows, int iCols, int n,
  * strategy may be NULL when opts->want_strategy == 0; res may be NULL). */
 int lnorm_hip_calc_ex(const int32_t *mtx_as_vec, int iRows, int iCols, int n, int marginal,
                       const lnorm_hip_opts *opts, int32_t *strategy, lnorm_hip_result *res);
+
+/*
+ * lnorm_hip_calc_ex without device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code that
+ * must not call hipMalloc/hipFree, such as tight serving loops or harnesses
+ * that forbid it).  Arguments, results and kernels are those of
+ * lnorm_hip_calc_ex; the caller still passes the matrix, the strategy and
+ * the result in host memory.
+ *   Device memory: every buffer (matrix copy, tables, unit maxima, results,
+ *   general-kernel state) is carved from one static array of
+ *   LNORM_NOALLOC_SCRATCH_BYTES per device, which each device that loads the
+ *   library holds until the process ends.  The call never calls hipMalloc or
+ *   hipFree.  Its first call on a device creates that device's stream and
+ *   timing events and looks up the array's address.
+ *   Limits: enumeration launches hold at most 2^20 units (longer problems
+ *   run in more launches); the general kernel runs as many concurrent waves
+ *   as its per-wave state fits in the rest of the array.  A problem whose
+ *   tables do not fit returns LNORM_E_SIZE with the bytes needed; within the
+ *   reference's limits (63 rows, 4096 columns, d <= 10) every int32 problem
+ *   fits.
+ *   Sharing: all calls on one device share its array, which keeps the last
+ *   call's contents; each call writes every buffer it reads before reading
+ *   it.  Calls that overlapped on one device would race on the array: the
+ *   library runs them one at a time (a per-device lock, and each call's
+ *   device work completes before it returns).
+ */
+#define LNORM_NOALLOC_SCRATCH_BYTES (8u << 20)
+int lnorm_hip_calc_noalloc(const int32_t *mtx_as_vec, int iRows, int iCols, int n, int marginal,
+                           const lnorm_hip_opts *opts, int32_t *strategy, lnorm_hip_result *res);
 
 /*
  * The int_type = long long variant of calc_Lnorm (README.md:100-105:

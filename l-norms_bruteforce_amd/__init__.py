@@ -82,7 +82,8 @@ HIP_SYMBOLS = ["lnorm_hip_calc_Lnorm", "lnorm_hip_calc_ex", "lnorm_hip_calc_batc
                "lnorm_hip_default_opts", "lnorm_hip_last_error", "lnorm_hip_abi_version",
                "lnorm_hip_device_count", "lnorm_hip_release", "lnorm_hip_selftest",
                "lnorm_hip_debug_table", "lnorm_hip_calc_batch_ex", "lnorm_hip_autotune",
-               "lnorm_hip_autotune_batch", "lnorm_hip_calc_Lnorm64", "lnorm_hip_calc_ex64"]
+               "lnorm_hip_autotune_batch", "lnorm_hip_calc_Lnorm64", "lnorm_hip_calc_ex64",
+               "lnorm_hip_calc_noalloc"]
 
 _hip = None
 
@@ -107,6 +108,9 @@ def hip_lib():
         L.lnorm_hip_calc_ex.restype = ctypes.c_int
         L.lnorm_hip_calc_ex.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                         ctypes.POINTER(Opts), i32p, ctypes.POINTER(Result)]
+        L.lnorm_hip_calc_noalloc.restype = ctypes.c_int
+        L.lnorm_hip_calc_noalloc.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.POINTER(Opts), i32p, ctypes.POINTER(Result)]
         L.lnorm_hip_calc_batch.restype = ctypes.c_int
         L.lnorm_hip_calc_batch.argtypes = [ctypes.POINTER(Item), ctypes.c_int, ctypes.POINTER(Opts)]
         L.lnorm_hip_calc_batch_ex.restype = ctypes.c_int
@@ -168,17 +172,20 @@ def default_opts(**kw) -> Opts:
     return o
 
 
-def calc_lnorm(A, n: int, marginal: int = 0, want_strategy: bool = True, **opts):
+def calc_lnorm(A, n: int, marginal: int = 0, want_strategy: bool = True, noalloc: bool = False, **opts):
     """The reference's calc_Lnorm on the GPU for a laid-out matrix
-    (mtx_as_vec, R x C).  Returns (Lnorm, strategy list, result dict)."""
+    (mtx_as_vec, R x C).  Returns (Lnorm, strategy list, result dict).
+    noalloc: through lnorm_hip_calc_noalloc (device buffers carved from the
+    library's static per-device scratch array, no device allocation)."""
     a = _i32(A)
     R, C = a.shape
     o = default_opts(want_strategy=int(want_strategy), **opts)
     strat = np.zeros(max(R - 1, 1), dtype=np.int32)
     res = Result()
     i32p = ctypes.POINTER(ctypes.c_int32)
-    _check(hip_lib().lnorm_hip_calc_ex(a.ctypes.data_as(i32p), R, C, n, marginal, ctypes.byref(o),
-                                       strat.ctypes.data_as(i32p), ctypes.byref(res)))
+    fn = hip_lib().lnorm_hip_calc_noalloc if noalloc else hip_lib().lnorm_hip_calc_ex
+    _check(fn(a.ctypes.data_as(i32p), R, C, n, marginal, ctypes.byref(o), strat.ctypes.data_as(i32p),
+              ctypes.byref(res)))
     return int(res.Lnorm), [int(v) for v in strat[: R - 1]] if want_strategy else None, res.as_dict()
 
 

@@ -55,7 +55,18 @@ def _run(cmd):
     return p
 
 
-def build_hip(jobs: int = 8) -> str:
+def _stale(obj: str, src: str) -> bool:
+    """True when obj is missing or older than src or any engine header."""
+    if not os.path.exists(obj):
+        return True
+    deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
+    deps.append(os.path.join(INC, "lnorm_hip.h"))
+    return os.path.getmtime(obj) < max(os.path.getmtime(d) for d in deps)
+
+
+def build_hip(jobs: int = 8, incremental: bool = False) -> str:
+    """Compiles every HIP translation unit (incremental: only those older
+    than their source or a header) and links lib/liblnorm_hip.so."""
     os.makedirs(OBJ, exist_ok=True)
     os.makedirs(LIB, exist_ok=True)
     objs = []
@@ -63,6 +74,8 @@ def build_hip(jobs: int = 8) -> str:
     for src in HIP_SOURCES:
         obj = os.path.join(OBJ, src.replace(".hip", ".o"))
         objs.append(obj)
+        if incremental and not _stale(obj, os.path.join(CSRC, src)):
+            continue
         cmds.append([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
     with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
         for f in [ex.submit(_run, c) for c in cmds]:
@@ -88,12 +101,13 @@ def build_host() -> tuple:
     return lib, exe
 
 
-def build(jobs: int = 8) -> None:
-    build_hip(jobs)
+def build(jobs: int = 8, incremental: bool = False) -> None:
+    build_hip(jobs, incremental)
     if os.path.exists(os.path.join(HOST, "lnorm_host.c")):
         build_host()
 
 
 if __name__ == "__main__":
-    build(int(sys.argv[1]) if len(sys.argv) > 1 else 8)
+    args = [a for a in sys.argv[1:] if a != "--incremental"]
+    build(int(args[0]) if args else 8, incremental="--incremental" in sys.argv[1:])
     print("built", LIB, BIN)

@@ -67,6 +67,7 @@ constexpr int kSideStreams = 3;
 struct Buf {
   void *p = nullptr;
   size_t cap = 0;
+  bool fixed = false;   // a carve-out of the no-allocation scratch: never reallocated
 };
 
 struct DevCtx {
@@ -85,6 +86,11 @@ struct DevCtx {
   std::vector<uint64_t> bGraphSig;
   int cus = 0;   // compute units
   bool init = false;
+  // units per enumeration launch (the capacity of `units`) and the general
+  // kernel's concurrent waves (its per-wave state slices)
+  uint64_t unit_chunk = kUnitsPerChunk;
+  uint64_t general_slices = kGeneralSlices;
+  bool noalloc = false;   // every buffer is a fixed carve-out of `scratch`
 };
 
 std::mutex g_ctx_mu;
@@ -93,6 +99,9 @@ DevCtx g_ctx[64];
 int ensure(Buf &b, size_t bytes) {
   if (bytes == 0) bytes = 8;
   if (b.cap >= bytes) return 0;
+  if (b.fixed)
+    return fail(LNORM_E_SIZE, "no-allocation call: %zu bytes of device scratch needed where %zu are reserved",
+                bytes, b.cap);
   if (b.p) (void)hipFree(b.p);
   b.p = nullptr;
   b.cap = 0;
@@ -106,7 +115,7 @@ int ensure(Buf &b, size_t bytes) {
 // enumeration writes.  Each run_* function makes sure of them itself: the
 // autotuner's probes run before a call's own uploads.
 int ensure_result_bufs(DevCtx &c) {
-  if (int r = ensure(c.units, kUnitsPerChunk * 4)) return r;
+  if (int r = ensure(c.units, c.unit_chunk * 4)) return r;
   return ensure(c.scal, 256);
 }
 
@@ -123,6 +132,57 @@ int ctx_init(DevCtx &c) {
     for (auto &e : c.bJoin[ph]) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   }
   c.init = true;
+  return 0;
+}
+
+// ---- device memory of the no-allocation entry point ----------------------------
+// lnorm_hip_calc_noalloc takes every device buffer of a single-problem call
+// from this one array, which each device that loads the library holds until
+// the process ends.  In host code `scratch` names a host-side stand-in, not a
+// device address: the contexts below keep each device's own address, looked
+// up once per device by hipGetSymbolAddress, and only that address reaches
+// kernels and copies.  The layout (bytes, in this order):
+//   unit maxima     4 MiB   2^20 units per launch (launches are chunked)
+//   results         4 KiB   the maximum and the first rank/unit
+//   matrix          1 MiB   R x C int32 (general kernel: 63 x 4096 at most)
+//   deltas/tables 512 KiB   [2][R-1][1024] deltas of the widest binary walk
+//   blocked       468 KiB   int8 planes, low-pattern table, seed, high deltas
+//   general state   rest    per-wave label vectors: as many concurrent waves
+//                           as whole slices fit (at most kGeneralSlices)
+constexpr size_t kScratchBytes = LNORM_NOALLOC_SCRATCH_BYTES;
+constexpr uint64_t kNoallocUnits = 1ull << 20;
+static __device__ __align__(256) uint8_t scratch[kScratchBytes];
+
+DevCtx g_ctx_noalloc[64];
+
+// Creates the context's stream and events and carves its buffers out of this
+// device's copy of `scratch` (the calling thread's current device).
+int noalloc_init(DevCtx &c) {
+  if (c.noalloc) return 0;
+  if (int r = ctx_init(c)) return r;
+  void *base = nullptr;
+  HIPCHK(hipGetSymbolAddress(&base, HIP_SYMBOL(scratch)));
+  if (!base) return fail(LNORM_E_HIP, "hipGetSymbolAddress(scratch) returned NULL");
+  size_t off = 0;
+  auto carve = [&](Buf &b, size_t bytes) {
+    b.p = (char *)base + off;
+    b.cap = bytes;
+    b.fixed = true;
+    off += (bytes + 255) & ~(size_t)255;
+  };
+  carve(c.units, kNoallocUnits * 4);
+  carve(c.scal, 4096);
+  carve(c.A, (size_t)1 << 20);
+  carve(c.D, (size_t)512 << 10);
+  carve(c.lo, (size_t)16 << 10);
+  carve(c.table, (size_t)256 << 10);
+  carve(c.seed, (size_t)4 << 10);
+  carve(c.dfull, (size_t)64 << 10);
+  carve(c.dpk, (size_t)128 << 10);
+  if (off >= kScratchBytes) return fail(LNORM_E_HIP, "scratch layout exceeds %zu bytes", kScratchBytes);
+  carve(c.state, kScratchBytes - off);
+  c.unit_chunk = kNoallocUnits;
+  c.noalloc = true;
   return 0;
 }
 
@@ -365,7 +425,7 @@ struct Phase1Out {
 int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometry &g, hipStream_t st,
                  int block, Phase1Out &out, uint64_t max_units = 0, uint64_t chunk_units = 0) {
   if (int r = ensure_result_bufs(cx)) return r;
-  const uint64_t chunk = chunk_units ? std::min(chunk_units, kUnitsPerChunk) : kUnitsPerChunk;
+  const uint64_t chunk = chunk_units ? std::min(chunk_units, cx.unit_chunk) : cx.unit_chunk;
   const uint64_t nunits = max_units ? std::min(max_units, g.nseg) : g.nseg;
   out.units = nunits;
   out.best = INT32_MIN;
@@ -390,6 +450,7 @@ int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometr
     a.gbest = gbest;
     a.gfirst = gfirst;
     a.state = (uint32_t *)cx.state.p;
+    a.slices = cx.general_slices;
     HIPCHK(hipEventRecord(cx.ev[4], st));
     hipError_t e = launch_walk(p, pp, MODE_MAX, a, st, block);
     HIPCHK(hipEventRecord(cx.ev[5], st));
@@ -432,6 +493,7 @@ int run_walk_first(DevCtx &cx, const Problem &p, const Prepared &pp, uint64_t u,
   a.gbest = (int32_t *)cx.scal.p;
   a.gfirst = gfirst;
   a.state = (uint32_t *)cx.state.p;
+  a.slices = cx.general_slices;
   hipError_t err = launch_walk(p, pp, MODE_FIRST, a, st, block);
   if (err != hipSuccess) return fail(LNORM_E_HIP, "strategy launch failed: %s", hipGetErrorString(err));
   unsigned long long hf;
@@ -630,8 +692,8 @@ int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t
   out.units = nunits;
   out.best = INT32_MIN;
   out.first_unit = ~0ull;
-  for (uint64_t c0 = 0; c0 < nunits; c0 += kUnitsPerChunk) {
-    const uint64_t n = std::min(kUnitsPerChunk, nunits - c0);
+  for (uint64_t c0 = 0; c0 < nunits; c0 += cx.unit_chunk) {
+    const uint64_t n = std::min(cx.unit_chunk, nunits - c0);
     HIPCHK(launch_fill_i32(gbest, INT32_MIN, 1, st));
     HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
     BlockedArgs a{};
@@ -771,10 +833,16 @@ int upload_walk(DevCtx &cx, const Problem &p, const Prepared &pp, hipStream_t st
   const int nvec = (p.kind == KIND_L1 || p.kind == KIND_LM) ? 1 : p.n;
   if (int r = ensure(cx.A, pp.A.size() * 4)) return r;
   if (int r = ensure(cx.D, pp.D.size() * 4)) return r;
-  if (int r = ensure(cx.units, kUnitsPerChunk * 4)) return r;
+  if (int r = ensure(cx.units, cx.unit_chunk * 4)) return r;
   if (int r = ensure(cx.scal, 256)) return r;
-  if (pp.walk == WALK_GENERAL)
-    if (int r = ensure(cx.state, (size_t)kGeneralSlices * nvec * p.C * 4)) return r;
+  if (pp.walk == WALK_GENERAL) {
+    // one state slice (nvec x C words) per concurrent wave of the general
+    // kernel; a fixed scratch runs as many waves as its slices hold
+    const size_t slice = (size_t)nvec * p.C * 4;
+    cx.general_slices = cx.state.fixed ? std::min<uint64_t>(kGeneralSlices, cx.state.cap / slice) : kGeneralSlices;
+    if (cx.general_slices == 0) return ensure(cx.state, slice);   // the size error
+    if (int r = ensure(cx.state, (size_t)cx.general_slices * slice)) return r;
+  }
   HIPCHK(hipMemcpyAsync(cx.A.p, pp.A.data(), pp.A.size() * 4, hipMemcpyHostToDevice, st));
   if (!pp.D.empty()) HIPCHK(hipMemcpyAsync(cx.D.p, pp.D.data(), pp.D.size() * 4, hipMemcpyHostToDevice, st));
   return 0;
@@ -912,7 +980,7 @@ Geometry single_geometry(const DevCtx &cx, const Problem &p, const Prepared &pp,
   if (nb <= 0) return def;
   const uint64_t slots = (uint64_t)nb * cx.cus;              // resident workgroups
   const uint64_t per_round = slots * wpb * 64;                 // lane segments per round
-  const uint64_t launch = per_round <= kUnitsPerChunk ? kUnitsPerChunk / per_round * per_round : kUnitsPerChunk;
+  const uint64_t launch = per_round <= cx.unit_chunk ? cx.unit_chunk / per_round * per_round : cx.unit_chunk;
   const double c = walk_cost(WALK_TAB16, p.n, pp.ny_pad), nw = words;
   const double start = (p.n == 1 ? 0.0 : (double)p.n * (1 << pp.qmin) * nw) + 2.0 * p.R * nw;
   // one round: every lane of nb * wpb / 4 waves per SIMD walks its segment,
@@ -943,7 +1011,7 @@ Geometry single_geometry(const DevCtx &cx, const Problem &p, const Prepared &pp,
 }
 
 int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *optsp,
-              int32_t *strategy, lnorm_hip_result *res) {
+              int32_t *strategy, lnorm_hip_result *res, bool noalloc = false) {
   const auto t_start = std::chrono::steady_clock::now();
   lnorm_hip_opts o;
   lnorm_hip_default_opts(&o);
@@ -954,9 +1022,9 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
   if (o.want_strategy && R > 1 && !strategy) return fail(LNORM_E_ARG, "strategy output is NULL");
   int dev = 0;
   if (int r = select_device(o, dev)) return r;
-  DevCtx &cx = g_ctx[dev];
+  DevCtx &cx = noalloc ? g_ctx_noalloc[dev] : g_ctx[dev];
   std::lock_guard<std::mutex> lock(cx.mu);
-  if (int r = ctx_init(cx)) return r;
+  if (int r = noalloc ? noalloc_init(cx) : ctx_init(cx)) return r;
   hipStream_t st = o.stream ? (hipStream_t)o.stream : cx.stream;
 
   lnorm_hip_result rr;
@@ -1751,6 +1819,11 @@ void lnorm_hip_default_opts(lnorm_hip_opts *o) {
 int lnorm_hip_calc_ex(const int32_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *opts,
                       int32_t *strategy, lnorm_hip_result *res) {
   return calc_impl(A, R, C, n, marginal, opts, strategy, res);
+}
+
+int lnorm_hip_calc_noalloc(const int32_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *opts,
+                           int32_t *strategy, lnorm_hip_result *res) {
+  return calc_impl(A, R, C, n, marginal, opts, strategy, res, true);
 }
 
 int lnorm_hip_calc_ex64(const int64_t *A, int R, int C, int n, int marginal, const lnorm_hip_opts *opts,

@@ -119,7 +119,8 @@ hipError_t launch_general(Kind kind, Mode mode, const PlainArgs &a, int ny, hipS
   uint64_t seglen = 1;
   const int base = a.d < 2 ? 2 : a.d;
   for (int i = 0; i < a.q; ++i) seglen *= (uint64_t)base;
-  const uint64_t grid = a.nseg < kGeneralSlices ? a.nseg : kGeneralSlices;
+  const uint64_t slices = a.slices ? a.slices : kGeneralSlices;
+  const uint64_t grid = a.nseg < slices ? a.nseg : slices;
   if (mode == MODE_MAX)
     hipLaunchKernelGGL(k_general<MODE_MAX>, dim3((unsigned)grid), dim3(64), 0, st, a.A, a.R, ny, (int)kind,
                        a.d, seglen, a.seg0, a.nseg, a.state, a.target, a.unit_best, a.gbest,

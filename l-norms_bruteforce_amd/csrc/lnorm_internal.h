@@ -35,7 +35,8 @@ struct PlainArgs {
   int32_t *unit_best;      // MAX mode: best per segment [nseg]
   int32_t *gbest;          // MAX mode: atomic max
   unsigned long long *gfirst;  // FIRST mode: atomic min of the rank
-  uint32_t *state;         // general kernel: per-wave scratch, kGeneralSlices * nvec * ny words
+  uint32_t *state;         // general kernel: per-wave scratch, slices * nvec * ny words
+  uint64_t slices;         // general kernel: concurrent waves (state slices); 0 = kGeneralSlices
 };
 
 // The int_type = long long variant: the same geometry with 64-bit values.
