@@ -35,7 +35,8 @@ extern "C" {
 
 /* kernel selection */
 #define LNORM_KERNEL_AUTO 0     /* n = 1: blocked from 2^33 sign vectors at up to 40 |.| columns, else
-                                   table16; n >= 2: table16, packed plain16; else plain, wide, general */
+                                   table16; n >= 2: table16; then table32, blocked (32-bit words), packed
+                                   plain16, plain, wide, general */
 #define LNORM_KERNEL_PLAIN 1    /* register-resident Gray walk, lane per segment */
 #define LNORM_KERNEL_BLOCKED 2  /* L1/LM: MFMA-built low-row table + scalar high walk (16- or 32-bit words) */
 #define LNORM_KERNEL_GENERAL 3  /* any width: wave per segment, columns over lanes */
@@ -46,7 +47,12 @@ extern "C" {
                                    per-label subset tables for n = 2..10 (column sums <= 32767; one wave
                                    up to 192 columns at d = 2 ... 32 at d = 10, W waves up to 512 at
                                    d <= 6 ... 128 at d = 10), sign-pattern tables for n = 1 (|.| column
-                                   sums <= 32767, one wave up to 384 columns, 8 or 16 waves up to 2048) */
+                                   sums <= 32767, one wave up to 384 columns, 8 to 16 waves up to 2048) */
+#define LNORM_KERNEL_TABLE32 7  /* the tabulated walks in 32-bit words (one column per word, v_sad_u32): any
+                                   int32 input; n = 1 up to 192 |.| columns in one wave, 1024 over W waves;
+                                   n = 2..10 up to 96 (d = 2) .. 16 (d = 10) columns in one wave, 256
+                                   (d <= 6) .. 64 (d = 10) over W waves.  AUTO takes it where the 16-bit
+                                   walks do not apply (column sums beyond 32767) */
 
 typedef struct lnorm_hip_opts {
     int32_t kernel;             /* LNORM_KERNEL_* */

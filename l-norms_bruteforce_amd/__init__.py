@@ -22,8 +22,9 @@ HIP_LIB = os.path.join(LIB_DIR, "liblnorm_hip.so")
 HOST_LIB = os.path.join(LIB_DIR, "liblnorm_host.so")
 CLI = os.path.join(PKG_DIR, "bin", "L_HIP")
 
-KERNEL_AUTO, KERNEL_PLAIN, KERNEL_BLOCKED, KERNEL_GENERAL, KERNEL_PLAIN16, KERNEL_WIDE, KERNEL_TABLE16 = range(7)
-KERNELS = {"auto": 0, "plain": 1, "blocked": 2, "general": 3, "plain16": 4, "wide": 5, "table16": 6}
+(KERNEL_AUTO, KERNEL_PLAIN, KERNEL_BLOCKED, KERNEL_GENERAL, KERNEL_PLAIN16, KERNEL_WIDE, KERNEL_TABLE16,
+ KERNEL_TABLE32) = range(8)
+KERNELS = {"auto": 0, "plain": 1, "blocked": 2, "general": 3, "plain16": 4, "wide": 5, "table16": 6, "table32": 7}
 ERRORS = {0: "OK", -1: "E_ARG", -2: "E_BOUND", -3: "E_SIZE", -4: "E_HIP", -5: "E_UNSUPPORTED"}
 
 

@@ -257,7 +257,10 @@ struct Geometry {
 
 // The lane- or wave-per-segment walks.
 enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2, WALK_WIDE = 3, WALK_WIDE16 = 4, WALK_TAB16 = 5,
-            WALK_TAB16W = 6 };
+            WALK_TAB16W = 6, WALK_TAB32 = 7, WALK_TAB32W = 8 };
+
+// The tabulated walks in 32-bit words: one column per word (ny_pad = words).
+bool is_tab32(Walk w) { return w == WALK_TAB32 || w == WALK_TAB32W; }
 
 // Segments for a range of base^e ranks: aim at about 2^target_log lanes.
 // The step counter of the d-ary and packed walks holds 15 base-d digits.
@@ -282,7 +285,7 @@ Geometry geometry_for(int base, int e, int target_log, int forced, Walk walk, in
 
 struct Prepared {
   Walk walk = WALK_PLAIN;
-  int ny_pad = 0;      // kernel width in columns (packed walk: 2 x words)
+  int ny_pad = 0;      // kernel width in columns (packed walks: 2 x words; 32-bit tabulated: words)
   int qmin = 0;        // smallest segment exponent (tabulated walk: its table rows)
   std::vector<int32_t> A, D;
 };
@@ -297,11 +300,73 @@ int64_t max_col_abs_sum(const Problem &p, int first_col) {
   return mx;
 }
 
+// Tables of the tabulated walks in 32-bit words (walk_blk16 / walk_tab16_bin
+// with WB = 32).  NW = ny_pad words, one column each, padding columns 0.
+//   n = 1: A = R x NW |.| columns, R x NW of 2 A, (LM) the R column-0
+//          entries; D = 2^M x NW words 2^31 - A_S (A_S = sum_{i<M} s_i A_i,
+//          s_i = 2 S_i - 1), (LM) the 2^M column-0 sums A_S,0.
+//   n >= 2: A = R x NW columns; D = 2^M x NW words 2^31 - sum_{i in S} A_i.
+void prepare_tab32(const Problem &p, Walk walk, int NW, Prepared &pp) {
+  const int marg = p.kind == KIND_LM ? 1 : 0;
+  int M;
+  if (p.n == 1) M = walk == WALK_TAB32W ? wtb32_rows(NW) : tab32_bin_rows(NW);
+  else M = walk == WALK_TAB32W ? wt32_rows(p.n, NW) : blk32_rows(p.n, NW);
+  pp.qmin = M;
+  const int C = p.C - marg;   // the columns held in words
+  auto at = [&](int i, int c) { return p.A[(size_t)i * p.C + marg + c]; };
+  // signed (n = 1) or subset (n >= 2) sums of the M fastest rows, each from
+  // the pattern without its lowest row: one add per column.  Every sum is
+  // bounded by sum |A| <= INT32_MAX; the adds wrap (unsigned) on the way.
+  const int VC = p.n == 1 ? p.C : C;   // n = 1 keeps column 0 too (LM's marginal column)
+  std::vector<uint32_t> val((size_t)VC << M, 0u);
+  if (p.n == 1)
+    for (int l = 0; l < p.C; ++l) {
+      uint32_t sum = 0;
+      for (int i = 0; i < M; ++i) sum -= (uint32_t)p.A[(size_t)i * p.C + l];
+      val[l] = sum;
+    }
+  for (int S = 1; S < (1 << M); ++S) {
+    const uint32_t *prev = val.data() + (size_t)(S & (S - 1)) * VC;
+    uint32_t *cur = val.data() + (size_t)S * VC;
+    const int i = __builtin_ctz(S);
+    for (int l = 0; l < VC; ++l) {
+      const uint32_t a = (uint32_t)(p.n == 1 ? p.A[(size_t)i * p.C + l] : at(i, l));
+      cur[l] = prev[l] + (p.n == 1 ? 2u * a : a);
+    }
+  }
+  if (p.n == 1) {
+    pp.A.assign((size_t)2 * p.R * NW + (marg ? p.R : 0), 0);
+    for (int i = 0; i < p.R; ++i) {
+      for (int c = 0; c < C; ++c) {
+        pp.A[(size_t)i * NW + c] = at(i, c);
+        pp.A[(size_t)(p.R + i) * NW + c] = (int32_t)(2u * (uint32_t)at(i, c));
+      }
+      if (marg) pp.A[(size_t)2 * p.R * NW + i] = p.A[(size_t)i * p.C];
+    }
+    pp.D.assign(((size_t)NW << M) + (marg ? (1u << M) : 0), (int32_t)kBias32);
+    for (int S = 0; S < (1 << M); ++S) {
+      for (int c = 0; c < C; ++c) pp.D[(size_t)S * NW + c] = (int32_t)(kBias32 - val[(size_t)S * VC + marg + c]);
+      if (marg) pp.D[((size_t)NW << M) + S] = (int32_t)val[(size_t)S * VC];
+    }
+  } else {
+    pp.A.assign((size_t)p.R * NW, 0);
+    for (int i = 0; i < p.R; ++i)
+      for (int c = 0; c < C; ++c) pp.A[(size_t)i * NW + c] = at(i, c);
+    pp.D.assign((size_t)NW << M, (int32_t)kBias32);
+    for (int S = 0; S < (1 << M); ++S)
+      for (int c = 0; c < C; ++c) pp.D[(size_t)S * NW + c] = (int32_t)(kBias32 - val[(size_t)S * VC + c]);
+  }
+}
+
 void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
   pp.walk = walk;
   pp.ny_pad = walk == WALK_GENERAL ? 0 : ny_pad;
   pp.qmin = 0;
   pp.D.clear();
+  if (is_tab32(walk)) {
+    prepare_tab32(p, walk, ny_pad, pp);
+    return;
+  }
   if ((walk == WALK_TAB16 || walk == WALK_TAB16W) && p.n == 1) {
     // L1 / LM sign-pattern table (walk_tab16_bin, k_wide_tb16): A = R x NW
     // packed |.| columns, R x NW packed 2A, (LM) R column-0 entries; D = 2^M x
@@ -407,6 +472,8 @@ hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const Pl
     case WALK_WIDE16: return launch_wide16(mode, a, st);
     case WALK_TAB16: return p.n == 1 ? launch_tab16_bin(p.kind, mode, a, st, block) : launch_blk16(mode, a, st, block);
     case WALK_TAB16W: return p.n == 1 ? launch_wide_tb16(p.kind, mode, a, st) : launch_wide_t16(mode, a, st);
+    case WALK_TAB32: return p.n == 1 ? launch_tab32_bin(p.kind, mode, a, st, block) : launch_blk32(mode, a, st, block);
+    case WALK_TAB32W: return p.n == 1 ? launch_wide_tb32(p.kind, mode, a, st) : launch_wide_t32(mode, a, st);
     default: return launch_general(p.kind, mode, a, p.C, st);
   }
 }
@@ -749,6 +816,7 @@ struct Choice {
   int kernel = LNORM_KERNEL_AUTO;
   BlockedPlan bp;
   int ny_plain = 0, w16 = 0, ny_wide = 0, w16_wide = 0, wtab = 0, wtabw = 0;
+  int wtab32 = 0, wtab32w = 0;   // the 32-bit tabulated walks: one wave or W waves (words = columns)
 };
 
 int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
@@ -780,8 +848,33 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
       if (c.wtabw && p.R - 1 < wt16_rows(p.n, c.wtabw)) c.wtabw = 0;
     }
   }
+  // the 32-bit tabulated walks: any int32 input, at widths of up to 192 |.|
+  // columns in one wave and 1024 over W waves (n = 1), up to 96 (d = 2) ..
+  // 16 (d = 10) columns in one wave and 256 (d <= 6) .. 64 (d = 10) over W
+  // waves (n >= 2), with more enumerated rows than table rows
+  {
+    const int marg = p.kind == KIND_LM ? 1 : 0;
+    if (p.n == 1) {
+      c.wtab32 = tab32_bin_words(p.C - marg);
+      if (c.wtab32 && p.R - 1 < tab32_bin_rows(c.wtab32)) c.wtab32 = 0;
+      if (!tab32_bin_words(p.C - marg)) {
+        c.wtab32w = wtb32_words(p.C - marg);
+        if (c.wtab32w && p.R - 1 < wtb32_rows(c.wtab32w)) c.wtab32w = 0;
+      }
+    } else {
+      c.wtab32 = blk32_words(p.n, p.C);
+      if (c.wtab32 && p.R - 1 < blk32_rows(p.n, c.wtab32)) c.wtab32 = 0;
+      if (!blk32_words(p.n, p.C)) {
+        c.wtab32w = wt32_words(p.n, p.C);
+        if (c.wtab32w && p.R - 1 < wt32_rows(p.n, c.wtab32w)) c.wtab32w = 0;
+      }
+    }
+  }
   if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
     return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
+  if (kernel == LNORM_KERNEL_TABLE32 && !c.wtab32 && !c.wtab32w)
+    return fail(LNORM_E_UNSUPPORTED, "32-bit tabulated walk needs an instantiated width and more enumerated rows "
+                "than table rows (%d x %d at n = %d)", p.R, p.C, p.n);
   if (kernel == LNORM_KERNEL_PLAIN && !c.ny_plain)
     return fail(LNORM_E_UNSUPPORTED, "no plain instantiation for %d columns at n = %d", p.C, p.n);
   if (kernel == LNORM_KERNEL_WIDE && !c.ny_wide && !c.w16_wide)
@@ -799,10 +892,14 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
     // else the tabulated walk (1-5% ahead from 22 words at 2^35-2^39, 6-10% at
     // 2^31; below 2^33 the blocked kernel's waves get too few high steps: 2^29
     // ran 27x slower), else the 32-bit blocked kernel
+    // Where the 16-bit walks do not apply (column sums beyond 32767, or too
+    // few rows for their tables), the 32-bit tabulated walk.
     if (can_block && !c.bp.wide && ((p.total >= (1ull << 33) && c.wtab <= 20) || !c.wtab))
       kernel = LNORM_KERNEL_BLOCKED;
     else if (c.wtab || c.wtabw)
       kernel = LNORM_KERNEL_TABLE16;
+    else if (c.wtab32 || c.wtab32w)
+      kernel = LNORM_KERNEL_TABLE32;
     else if (can_block)
       kernel = LNORM_KERNEL_BLOCKED;
     else if (c.w16)
@@ -812,7 +909,7 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
     else
       kernel = (c.ny_wide || c.w16_wide) ? LNORM_KERNEL_WIDE : LNORM_KERNEL_GENERAL;
   }
-  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_TABLE16)
+  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_TABLE32)
     return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
   c.kernel = kernel;
   return 0;
@@ -822,6 +919,8 @@ void prepare_for(const Problem &p, const Choice &c, Prepared &pp) {
   if (c.kernel == LNORM_KERNEL_PLAIN16) prepare_walk(p, WALK_PLAIN16, 2 * c.w16, pp);
   else if (c.kernel == LNORM_KERNEL_TABLE16 && c.wtab) prepare_walk(p, WALK_TAB16, 2 * c.wtab, pp);
   else if (c.kernel == LNORM_KERNEL_TABLE16) prepare_walk(p, WALK_TAB16W, 2 * c.wtabw, pp);
+  else if (c.kernel == LNORM_KERNEL_TABLE32 && c.wtab32) prepare_walk(p, WALK_TAB32, c.wtab32, pp);
+  else if (c.kernel == LNORM_KERNEL_TABLE32) prepare_walk(p, WALK_TAB32W, c.wtab32w, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE && c.w16_wide) prepare_walk(p, WALK_WIDE16, 2 * c.w16_wide, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
   else if (c.kernel == LNORM_KERNEL_BLOCKED && !c.ny_plain && c.ny_wide) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
@@ -879,6 +978,8 @@ TuneKey tune_key(int dev, const Problem &p, const Choice &c) {
     k.width = c.w16;
   } else if (c.kernel == LNORM_KERNEL_TABLE16) {
     k.width = c.wtab ? c.wtab : 1000 + c.wtabw;
+  } else if (c.kernel == LNORM_KERNEL_TABLE32) {
+    k.width = c.wtab32 ? c.wtab32 : 1000 + c.wtab32w;
   } else {
     k.width = c.ny_plain;
   }
@@ -934,7 +1035,8 @@ int tune_single(DevCtx &cx, const Problem &p, const Choice &c, const lnorm_hip_o
       best.s = 0;
     }
   } else if (c.kernel == LNORM_KERNEL_PLAIN || c.kernel == LNORM_KERNEL_PLAIN16 ||
-             (c.kernel == LNORM_KERNEL_TABLE16 && c.wtab)) {   // the W-wave walks' workgroup is fixed
+             (c.kernel == LNORM_KERNEL_TABLE16 && c.wtab) ||
+             (c.kernel == LNORM_KERNEL_TABLE32 && c.wtab32)) {   // the W-wave walks' workgroup is fixed
     Prepared pp;
     prepare_for(p, c, pp);
     if (int r = upload_walk(cx, p, pp, st)) return r;
@@ -973,15 +1075,18 @@ Geometry single_geometry(const DevCtx &cx, const Problem &p, const Prepared &pp,
                          uint64_t &chunk) {
   chunk = 0;
   const Geometry def = geometry_for(p.base, p.R - 1, 20, o.seg_log, pp.walk, pp.qmin);
-  if (o.seg_log > 0 || pp.walk != WALK_TAB16 || cx.cus <= 0) return def;
+  if (o.seg_log > 0 || (pp.walk != WALK_TAB16 && pp.walk != WALK_TAB32) || cx.cus <= 0) return def;
   const int block = (o.block > 0 && o.block % 64 == 0 && o.block <= 256) ? o.block : 256;
-  const int words = pp.ny_pad / 2, wpb = block / 64;
-  const int nb = p.n == 1 ? tab16_bin_blocks_per_cu(p.kind, words, block) : blk16_blocks_per_cu(p.n, words, block);
+  const bool w32 = pp.walk == WALK_TAB32;
+  const int words = w32 ? pp.ny_pad : pp.ny_pad / 2, wpb = block / 64;
+  int nb;
+  if (w32) nb = p.n == 1 ? tab32_bin_blocks_per_cu(p.kind, words, block) : blk32_blocks_per_cu(p.n, words, block);
+  else nb = p.n == 1 ? tab16_bin_blocks_per_cu(p.kind, words, block) : blk16_blocks_per_cu(p.n, words, block);
   if (nb <= 0) return def;
   const uint64_t slots = (uint64_t)nb * cx.cus;              // resident workgroups
   const uint64_t per_round = slots * wpb * 64;                 // lane segments per round
   const uint64_t launch = per_round <= cx.unit_chunk ? cx.unit_chunk / per_round * per_round : cx.unit_chunk;
-  const double c = walk_cost(WALK_TAB16, p.n, pp.ny_pad), nw = words;
+  const double c = walk_cost(pp.walk, p.n, pp.ny_pad), nw = words;
   const double start = (p.n == 1 ? 0.0 : (double)p.n * (1 << pp.qmin) * nw) + 2.0 * p.R * nw;
   // one round: every lane of nb * wpb / 4 waves per SIMD walks its segment,
   // 4 cycles per wave64 VALU op at about 2.4 GHz
@@ -1034,7 +1139,8 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
   Choice ch;
   if (int r = choose_kernel(p, o, ch)) return r;
   if (o.autotune && (ch.kernel == LNORM_KERNEL_BLOCKED || ch.kernel == LNORM_KERNEL_PLAIN ||
-                     ch.kernel == LNORM_KERNEL_PLAIN16 || ch.kernel == LNORM_KERNEL_TABLE16)) {
+                     ch.kernel == LNORM_KERNEL_PLAIN16 || ch.kernel == LNORM_KERNEL_TABLE16 ||
+                     ch.kernel == LNORM_KERNEL_TABLE32)) {
     // launch shapes from the cache (tuned on the first miss); explicit
     // options of the caller win
     const TuneKey key = tune_key(dev, p, ch);
@@ -1112,6 +1218,12 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
     else if (pp.walk == WALK_TAB16W)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16w<d=%d,words=%d,M=%d,q=%d>", p.n, pp.ny_pad / 2,
                pp.qmin, g.q);
+    else if (is_tab32(pp.walk) && p.n == 1)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "table32%s<%s,words=%d,M=%d,q=%d>",
+               pp.walk == WALK_TAB32W ? "w" : "", p.kind == KIND_LM ? "LM" : "L1", pp.ny_pad, pp.qmin, g.q);
+    else if (is_tab32(pp.walk))
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "table32%s<d=%d,words=%d,M=%d,q=%d>",
+               pp.walk == WALK_TAB32W ? "w" : "", p.n, pp.ny_pad, pp.qmin, g.q);
     else
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain<kind=%d,d=%d,ny=%d,q=%d>", (int)p.kind, p.n,
                pp.ny_pad, g.q);
@@ -1197,6 +1309,11 @@ Geometry batch_geometry(int base, int e, int forced, Walk walk, int cap_log, int
 // the tabulated walk 1.5 plus its table sums 2 + 2^(M+1) per word over d^M.
 double walk_cost(Walk walk, int d, int width) {
   if (walk == WALK_PLAIN) return 2.0 * width;
+  if (walk == WALK_TAB32 && d == 1) return width + 1.5 + width / 32.0;
+  if (walk == WALK_TAB32) {
+    const int M = blk32_rows(d, width);
+    return 1.5 + (2.0 + (double)(2 << M)) * width / std::pow((double)d, (double)M);
+  }
   const int nw = width / 2;
   if (walk == WALK_TAB16 && d == 1) return nw + 1.5 + nw / 32.0;
   if (walk == WALK_TAB16) {

@@ -19,18 +19,18 @@ int blk16_rows(int d, int words) { return blk16_low_rows(d, words); }
 
 hipError_t launch_blk16_lo(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
   switch (a.d) {
-    case 2: return launch_blk<2>(mode, a, st, block);
-    case 3: return launch_blk<3>(mode, a, st, block);
-    case 4: return launch_blk<4>(mode, a, st, block);
+    case 2: return launch_blk<2, 16>(mode, a, st, block);
+    case 3: return launch_blk<3, 16>(mode, a, st, block);
+    case 4: return launch_blk<4, 16>(mode, a, st, block);
     default: return hipErrorInvalidValue;
   }
 }
 
 int blocks_per_cu_blk16_lo(int d, int words, int block) {
   switch (d) {
-    case 2: return blocks_per_cu_blk<2>(words, block);
-    case 3: return blocks_per_cu_blk<3>(words, block);
-    case 4: return blocks_per_cu_blk<4>(words, block);
+    case 2: return blocks_per_cu_blk<2, 16>(words, block);
+    case 3: return blocks_per_cu_blk<3, 16>(words, block);
+    case 4: return blocks_per_cu_blk<4, 16>(words, block);
     default: return 0;
   }
 }

@@ -6,24 +6,24 @@ namespace lnorm {
 
 hipError_t launch_blk16_hi(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
   switch (a.d) {
-    case 5: return launch_blk<5>(mode, a, st, block);
-    case 6: return launch_blk<6>(mode, a, st, block);
-    case 7: return launch_blk<7>(mode, a, st, block);
-    case 8: return launch_blk<8>(mode, a, st, block);
-    case 9: return launch_blk<9>(mode, a, st, block);
-    case 10: return launch_blk<10>(mode, a, st, block);
+    case 5: return launch_blk<5, 16>(mode, a, st, block);
+    case 6: return launch_blk<6, 16>(mode, a, st, block);
+    case 7: return launch_blk<7, 16>(mode, a, st, block);
+    case 8: return launch_blk<8, 16>(mode, a, st, block);
+    case 9: return launch_blk<9, 16>(mode, a, st, block);
+    case 10: return launch_blk<10, 16>(mode, a, st, block);
     default: return hipErrorInvalidValue;
   }
 }
 
 int blocks_per_cu_blk16_hi(int d, int words, int block) {
   switch (d) {
-    case 5: return blocks_per_cu_blk<5>(words, block);
-    case 6: return blocks_per_cu_blk<6>(words, block);
-    case 7: return blocks_per_cu_blk<7>(words, block);
-    case 8: return blocks_per_cu_blk<8>(words, block);
-    case 9: return blocks_per_cu_blk<9>(words, block);
-    case 10: return blocks_per_cu_blk<10>(words, block);
+    case 5: return blocks_per_cu_blk<5, 16>(words, block);
+    case 6: return blocks_per_cu_blk<6, 16>(words, block);
+    case 7: return blocks_per_cu_blk<7, 16>(words, block);
+    case 8: return blocks_per_cu_blk<8, 16>(words, block);
+    case 9: return blocks_per_cu_blk<9, 16>(words, block);
+    case 10: return blocks_per_cu_blk<10, 16>(words, block);
     default: return 0;
   }
 }

@@ -150,6 +150,24 @@ int tab16_bin_words(int nabs);        // tabulated L1/LM walk: width in words fo
 // the table (2^M x NW scalar words) and the unrolled block within bounds
 constexpr int tab_bin_rows(int nw) { return nw <= 64 ? 5 : 4; }
 hipError_t launch_tab16_bin(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block);
+// The tabulated walks in 32-bit words (one column per word, biased by 2^31,
+// v_sad_u32): int32 inputs whose column sums exceed the 16-bit walks' bound.
+// Their widths are in words, one column each (PlainArgs::ny_pad = words).
+// L1 / LM table rows: 5 up to 128 words, 4 beyond.
+constexpr int tab32_bin_rows(int nw) { return nw <= 128 ? 5 : 4; }
+int tab32_bin_words(int nabs);        // one wave, up to 192 |.| columns (0: none)
+int tab32_bin_blocks_per_cu(Kind kind, int words, int block);
+hipError_t launch_tab32_bin(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block);
+int wtb32_words(int nabs);            // W waves, 256..1024 |.| columns (0: none)
+int wtb32_rows(int words);
+hipError_t launch_wide_tb32(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st);
+int blk32_words(int d, int ny);       // d-ary, one wave: up to 96 columns at d = 2 ... 16 at d = 10 (0: none)
+int blk32_rows(int d, int words);
+int blk32_blocks_per_cu(int d, int words, int block);
+hipError_t launch_blk32(Mode mode, const PlainArgs &a, hipStream_t st, int block);
+int wt32_words(int d, int ny);        // d-ary, W waves of 32 (d = 8, 9: 24; 10: 16) words (0: none)
+int wt32_rows(int d, int words);
+hipError_t launch_wide_t32(Mode mode, const PlainArgs &a, hipStream_t st);
 hipError_t launch_wide16(Mode mode, const PlainArgs &a, hipStream_t st);
 int plain16_words(int d, int ny);               // packed width instantiated for ny columns, 0 if none
 bool plain_supported(Kind kind, int d, int ny_pad);

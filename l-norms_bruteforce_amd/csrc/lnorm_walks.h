@@ -338,8 +338,50 @@ __device__ __forceinline__ uint32_t pk_sub(uint32_t a, uint32_t b) {
   return __builtin_bit_cast(uint32_t, __builtin_bit_cast(u16x2, a) - __builtin_bit_cast(u16x2, b));
 }
 
-// The packed counterpart of seed_add32.
-template <int D, int NW>
+// Word formats of the tabulated walks.
+//   WB = 16: two columns per word, each half biased by 2^15: v_pk_add_u16,
+//            v_pk_sub_u16, v_sad_u16 (two |.| per instruction).  Exact while
+//            every column's absolute sum is <= 2^15 - 1.
+//   WB = 32: one column per word, biased by 2^31: v_add_u32, v_sub_u32,
+//            v_sad_u32.  Exact under the int32 contract sum |A| <= 2^31 - 1,
+//            which bounds every label sum, subset sum and objective.
+// w_zero: the first |.| of a 32-bit chain accumulates onto a zero the
+// compiler cannot see (a literal 0 turns v_sad_u32 into three instructions).
+template <int WB>
+__device__ __forceinline__ uint32_t w_add(uint32_t a, uint32_t b) {
+  if constexpr (WB == 16) return pk_add(a, b);
+  else return a + b;
+}
+template <int WB>
+__device__ __forceinline__ uint32_t w_sub(uint32_t a, uint32_t b) {
+  if constexpr (WB == 16) return pk_sub(a, b);
+  else return a - b;
+}
+template <int WB>
+__device__ __forceinline__ uint32_t w_sad(uint32_t a, uint32_t b, uint32_t c) {
+  if constexpr (WB == 16) return sad16(a, b, c);
+  else return sad32(a, b, c);
+}
+template <int WB>
+__device__ __forceinline__ uint32_t w_zero() {
+  if constexpr (WB == 16) return 0u;
+  else return opaque_zero();
+}
+template <int WB>
+constexpr uint32_t w_bias() { return WB == 16 ? kBias16x2 : kBias32; }
+// A finished 32-bit |.| sum, made opaque.  v_sad_u16 is an intrinsic the
+// compiler cannot look into; the 32-bit |a - b| + c is plain arithmetic, and
+// without this the compiler re-associated the labelings' sums of table
+// entries into the |.| chains themselves and kept hundreds of partial sums
+// live (spills from d = 4 on).
+template <int WB>
+__device__ __forceinline__ uint32_t w_done(uint32_t x) {
+  if constexpr (WB == 32) asm("" : "+v"(x));
+  return x;
+}
+
+// The packed counterpart of seed_add32 (WB = 32: the same in 32-bit words).
+template <int D, int NW, int WB = 16>
 __device__ __forceinline__ void seed_add16(uint32_t (&t)[D][NW], const cuint32_t *row, int lab) {
   uint32_t mask[D];
 #pragma unroll
@@ -348,7 +390,7 @@ __device__ __forceinline__ void seed_add16(uint32_t (&t)[D][NW], const cuint32_t
   for (int l = 0; l < NW; ++l) {
     const uint32_t v = row[l];
 #pragma unroll
-    for (int c = 0; c < D; ++c) t[c][l] = pk_add(t[c][l], v & mask[c]);
+    for (int c = 0; c < D; ++c) t[c][l] = w_add<WB>(t[c][l], v & mask[c]);
   }
 }
 
@@ -629,73 +671,78 @@ __device__ __forceinline__ void blk16_eval(const uint32_t (&T)[D][1 << M], uint3
 
 // Table of label C from its sums u[C]: T[C][0] = |u_C|, and T[C][S] = F_C(S)
 // (d <= 3) or F_C(S) - |u_C| (d >= 4, the accumulator starts at -|u_C|).
-template <int C, int D, int NW, int M>
+template <int C, int D, int NW, int M, int WB>
 __device__ __forceinline__ void blk16_table(const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
                                             const cuint32_t *neg) {
-  uint32_t U = 0;
+  const uint32_t z = w_zero<WB>();
+  uint32_t U = z;
 #pragma unroll
-  for (int w = 0; w < NW; ++w) U = sad16(u[C][w], neg[w], U);
+  for (int w = 0; w < NW; ++w) U = w_sad<WB>(u[C][w], neg[w], U);
+  U = w_done<WB>(U);
   T[C][0] = U;
-  const uint32_t base = D <= 3 ? 0u : 0u - U;
+  const uint32_t base = D <= 3 ? z : z - U;
 #pragma unroll
   for (int S = 1; S < (1 << M); ++S) {
     uint32_t acc = base;
 #pragma unroll
-    for (int w = 0; w < NW; ++w) acc = sad16(u[C][w], neg[(size_t)S * NW + w], acc);
-    T[C][S] = acc;
+    for (int w = 0; w < NW; ++w) acc = w_sad<WB>(u[C][w], neg[(size_t)S * NW + w], acc);
+    T[C][S] = w_done<WB>(acc);
   }
 }
 
-template <int D, int NW, int M, int... Cs>
+template <int D, int NW, int M, int WB, int... Cs>
 __device__ __forceinline__ void blk16_tables_all(const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
                                                  const cuint32_t *neg, std::integer_sequence<int, Cs...>) {
-  (blk16_table<Cs, D, NW, M>(u, T, neg), ...);
+  (blk16_table<Cs, D, NW, M, WB>(u, T, neg), ...);
 }
 
 // Tables of the two labels a high step changes, C and C + 1, in one pass
 // over the subset table: each scalar word feeds two v_sad_u16 (one per
 // label), half the scalar loads of two single-label passes, and the two
 // accumulator chains are independent.
-template <int C, int D, int NW, int M>
+template <int C, int D, int NW, int M, int WB>
 __device__ __forceinline__ void blk16_table_pair(const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
                                                  const cuint32_t *neg) {
-  uint32_t U0 = 0, U1 = 0;
+  const uint32_t z = w_zero<WB>();
+  uint32_t U0 = z, U1 = z;
 #pragma unroll
   for (int w = 0; w < NW; ++w) {
     const uint32_t n = neg[w];
-    U0 = sad16(u[C][w], n, U0);
-    U1 = sad16(u[C + 1][w], n, U1);
+    U0 = w_sad<WB>(u[C][w], n, U0);
+    U1 = w_sad<WB>(u[C + 1][w], n, U1);
   }
+  U0 = w_done<WB>(U0);
+  U1 = w_done<WB>(U1);
   T[C][0] = U0;
   T[C + 1][0] = U1;
-  const uint32_t b0 = D <= 3 ? 0u : 0u - U0, b1 = D <= 3 ? 0u : 0u - U1;
+  const uint32_t b0 = D <= 3 ? z : z - U0, b1 = D <= 3 ? z : z - U1;
 #pragma unroll
   for (int S = 1; S < (1 << M); ++S) {
     uint32_t a0 = b0, a1 = b1;
 #pragma unroll
     for (int w = 0; w < NW; ++w) {
       const uint32_t n = neg[(size_t)S * NW + w];
-      a0 = sad16(u[C][w], n, a0);
-      a1 = sad16(u[C + 1][w], n, a1);
+      a0 = w_sad<WB>(u[C][w], n, a0);
+      a1 = w_sad<WB>(u[C + 1][w], n, a1);
     }
-    T[C][S] = a0;
-    T[C + 1][S] = a1;
+    T[C][S] = w_done<WB>(a0);
+    T[C + 1][S] = w_done<WB>(a1);
   }
 }
 
-template <int C, int D, int NW, int M>
+template <int C, int D, int NW, int M, int WB>
 struct Blk16TablePair {
   __device__ __forceinline__ static void run(int lo, const uint32_t (&u)[D][NW], uint32_t (&T)[D][1 << M],
                                              const cuint32_t *neg) {
     if constexpr (C + 1 < D) {
-      if (lo == C) blk16_table_pair<C, D, NW, M>(u, T, neg);
-      else Blk16TablePair<C + 1, D, NW, M>::run(lo, u, T, neg);
+      if (lo == C) blk16_table_pair<C, D, NW, M, WB>(u, T, neg);
+      else Blk16TablePair<C + 1, D, NW, M, WB>::run(lo, u, T, neg);
     }
   }
 };
 
-// Moves one packed row from label `from` to from + 1 (up) or from - 1.
-template <int C, int D, int NW>
+// Moves one row (WB-bit words) from label `from` to from + 1 (up) or from - 1.
+template <int C, int D, int NW, int WB = 16>
 struct Move16 {
   __device__ __forceinline__ static void run(uint32_t (&u)[D][NW], const cuint32_t *row, int from, int up) {
     if (from == C) {
@@ -704,28 +751,29 @@ struct Move16 {
         if constexpr (TU != C) {
 #pragma unroll
           for (int w = 0; w < NW; ++w) {
-            u[C][w] = pk_sub(u[C][w], row[w]);
-            u[TU][w] = pk_add(u[TU][w], row[w]);
+            u[C][w] = w_sub<WB>(u[C][w], row[w]);
+            u[TU][w] = w_add<WB>(u[TU][w], row[w]);
           }
         }
       } else {
         if constexpr (TD != C) {
 #pragma unroll
           for (int w = 0; w < NW; ++w) {
-            u[C][w] = pk_sub(u[C][w], row[w]);
-            u[TD][w] = pk_add(u[TD][w], row[w]);
+            u[C][w] = w_sub<WB>(u[C][w], row[w]);
+            u[TD][w] = w_add<WB>(u[TD][w], row[w]);
           }
         }
       }
     } else if constexpr (C + 1 < D) {
-      Move16<C + 1, D, NW>::run(u, row, from, up);
+      Move16<C + 1, D, NW, WB>::run(u, row, from, up);
     }
   }
 };
 
 // Ap: R x NW packed rows (as walk_p16); neg: 2^M x NW packed words -A_S +
 // 2^15.  q >= M: a segment is d^(q-M) high steps of d^M labelings each.
-template <int D, int NW, int M, int MODE>
+// WB = 32: Ap holds R x NW int32 columns and neg the words -A_S + 2^31.
+template <int D, int NW, int M, int MODE, int WB = 16>
 __device__ __forceinline__ void walk_blk16(
     const cuint32_t *Ap, const cuint32_t *neg, int R, int q, uint64_t seg0, uint64_t nseg, int32_t target,
     int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest, unsigned long long *__restrict__ gfirst,
@@ -749,7 +797,7 @@ __device__ __forceinline__ void walk_blk16(
 #pragma unroll
   for (int c = 0; c < D; ++c)
 #pragma unroll
-    for (int l = 0; l < NW; ++l) u[c][l] = pk_add(kBias16x2, c == 0 ? Ap[(size_t)(R - 1) * NW + l] : 0u);
+    for (int l = 0; l < NW; ++l) u[c][l] = w_add<WB>(w_bias<WB>(), c == 0 ? Ap[(size_t)(R - 1) * NW + l] : 0u);
   uint64_t x = seg;
   for (int i = 0; i < R - 1 - M; ++i) {
     int lab;
@@ -760,10 +808,10 @@ __device__ __forceinline__ void walk_blk16(
       lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
       x /= (uint64_t)D;
     }
-    seed_add16<D, NW>(u, Ap + (size_t)(M + i) * NW, lab);
+    seed_add16<D, NW, WB>(u, Ap + (size_t)(M + i) * NW, lab);
   }
   uint32_t T[D][1 << M];
-  blk16_tables_all<D, NW, M>(u, T, neg, std::make_integer_sequence<int, D>{});
+  blk16_tables_all<D, NW, M, WB>(u, T, neg, std::make_integer_sequence<int, D>{});
   uint32_t totU = 0;
 #pragma unroll
   for (int c = 0; c < D; ++c) totU += T[c][0];
@@ -788,13 +836,13 @@ __device__ __forceinline__ void walk_blk16(
       const int from = p ? D - v : v - 1;
       const int up = p ? 0 : 1;
       pm ^= (1u << j) - 1u;
-      Move16<0, D, NW>::run(u, Ap + (size_t)(M + j) * NW, from, up);
+      Move16<0, D, NW, WB>::run(u, Ap + (size_t)(M + j) * NW, from, up);
       // The table's scalar loads are loop-invariant; hoisted out of the
       // loop, 2^M * NW values would not fit in SGPRs and be spilled to VGPR
       // lanes (one v_readlane per use), so the pointer is laundered here.
       const cuint32_t *negk = neg;
       asm volatile("" : "+s"(negk));
-      Blk16TablePair<0, D, NW, M>::run(up ? from : from - 1, u, T, negk);
+      Blk16TablePair<0, D, NW, M, WB>::run(up ? from : from - 1, u, T, negk);
       totU = 0;
 #pragma unroll
       for (int c = 0; c < D; ++c) totU += T[c][0];
@@ -857,50 +905,55 @@ __device__ __forceinline__ void walk_blk16(
 // then (LM) the 2^M column-0 sums A_S,0.
 // ---------------------------------------------------------------------------
 
-template <int NW, int M, bool MARG, int MODE, int S>
+template <int NW, int M, bool MARG, int WB, int MODE, int S>
 __device__ __forceinline__ int32_t tb16_obj(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
                                             const cint32_t *pos0) {
-  uint32_t acc = MARG ? (uint32_t)(u0 + pos0[S]) : 0u;
+  uint32_t acc = MARG ? (uint32_t)(u0 + pos0[S]) : w_zero<WB>();
 #pragma unroll
-  for (int w = 0; w < NW; ++w) acc = sad16(u[w], neg[(size_t)S * NW + w], acc);
+  for (int w = 0; w < NW; ++w) acc = w_sad<WB>(u[w], neg[(size_t)S * NW + w], acc);
   return (int32_t)acc;
 }
 
 // MODE_MAX: patterns in table order, two per v_max3.
-template <int NW, int M, bool MARG, int... Ps>
+template <int NW, int M, bool MARG, int WB, int... Ps>
 __device__ __forceinline__ void tb16_max(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
                                          const cint32_t *pos0, int32_t &best, std::integer_sequence<int, Ps...>) {
   auto pair = [&](auto P) {
     constexpr int S = decltype(P)::value;
-    const int32_t a = tb16_obj<NW, M, MARG, MODE_MAX, 2 * S>(u, u0, neg, pos0);
-    const int32_t b = tb16_obj<NW, M, MARG, MODE_MAX, 2 * S + 1>(u, u0, neg, pos0);
+    const int32_t a = tb16_obj<NW, M, MARG, WB, MODE_MAX, 2 * S>(u, u0, neg, pos0);
+    const int32_t b = tb16_obj<NW, M, MARG, WB, MODE_MAX, 2 * S + 1>(u, u0, neg, pos0);
     const int32_t m = a > b ? a : b;
     best = m > best ? m : best;
   };
   (pair(std::integral_constant<int, Ps>{}), ...);
 }
 
-// MODE_FIRST: patterns in the order of l; fb = the first l matching.
-template <int NW, int M, bool MARG, bool ODD, int... Ls>
+// MODE_FIRST: patterns in the order of l; fb = the first l matching.  It
+// re-walks a single unit, so the patterns run as a loop (the wave-uniform
+// pattern index addresses the scalar table) rather than unrolled: half the
+// code of the two unrolled orders.
+template <int NW, int M, bool MARG, int WB>
 __device__ __forceinline__ void tb16_first(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
-                                           const cint32_t *pos0, int32_t target, uint32_t &fb,
-                                           std::integer_sequence<int, Ls...>) {
-  auto one = [&](auto L) {
-    constexpr int l = decltype(L)::value;
-    constexpr int S = (l ^ (l >> 1)) ^ (ODD ? (1 << (M - 1)) : 0);
-    const int32_t a = tb16_obj<NW, M, MARG, MODE_FIRST, S>(u, u0, neg, pos0);
-    const uint32_t c = (a == target) ? (uint32_t)l : 0xFFFFFFFFu;
+                                           const cint32_t *pos0, int32_t target, uint32_t odd, uint32_t &fb) {
+  const uint32_t flip = odd ? (1u << (M - 1)) : 0u;
+#pragma unroll 1
+  for (uint32_t l = 0; l < (1u << M); ++l) {
+    const uint32_t S = (l ^ (l >> 1)) ^ flip;
+    const cuint32_t *row = neg + (size_t)S * NW;
+    uint32_t acc = MARG ? (uint32_t)(u0 + pos0[S]) : w_zero<WB>();
+#pragma unroll
+    for (int w = 0; w < NW; ++w) acc = w_sad<WB>(u[w], row[w], acc);
+    const uint32_t c = ((int32_t)acc == target) ? l : 0xFFFFFFFFu;
     fb = c < fb ? c : fb;
-  };
-  (one(std::integral_constant<int, Ls>{}), ...);
+  }
 }
 
-template <int NW, bool MARG, int MODE>
+template <int NW, bool MARG, int MODE, int WB = 16>
 __device__ __forceinline__ void walk_tab16_bin(
     const cuint32_t *Ap, const cuint32_t *Tab, int R, int q, uint64_t seg0, uint64_t nseg, int32_t target,
     int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest, unsigned long long *__restrict__ gfirst,
     uint64_t w) {
-  constexpr int M = tab_bin_rows(NW);
+  constexpr int M = WB == 16 ? tab_bin_rows(NW) : tab32_bin_rows(NW);
   const int lane = wave_lane();
   if (((w >> 1) << 7) >= nseg) return;
   const uint64_t local = lane_segment(w, lane);
@@ -918,12 +971,12 @@ __device__ __forceinline__ void walk_tab16_bin(
   uint32_t u[NW];
   int32_t u0 = MARG ? A0[R - 1] : 0;
 #pragma unroll
-  for (int l = 0; l < NW; ++l) u[l] = pk_add(kBias16x2, Ap[(size_t)(R - 1) * NW + l]);
+  for (int l = 0; l < NW; ++l) u[l] = w_add<WB>(w_bias<WB>(), Ap[(size_t)(R - 1) * NW + l]);
   for (int j = 0; j < R - 1 - M; ++j) {
     const bool b = (g0 >> j) & 1;
     const cuint32_t *row = Ap + (size_t)(M + j) * NW;
 #pragma unroll
-    for (int l = 0; l < NW; ++l) u[l] = b ? pk_add(u[l], row[l]) : pk_sub(u[l], row[l]);
+    for (int l = 0; l < NW; ++l) u[l] = b ? w_add<WB>(u[l], row[l]) : w_sub<WB>(u[l], row[l]);
     if (MARG) u0 += b ? A0[M + j] : -A0[M + j];
   }
 
@@ -939,11 +992,11 @@ __device__ __forceinline__ void walk_tab16_bin(
       const cuint32_t *row2 = A2 + (size_t)(M + j) * NW;
       if (hb == 0) {
 #pragma unroll
-        for (int l = 0; l < NW; ++l) u[l] = pk_add(u[l], row2[l]);
+        for (int l = 0; l < NW; ++l) u[l] = w_add<WB>(u[l], row2[l]);
         if (MARG) u0 += 2 * A0[M + j];
       } else {
 #pragma unroll
-        for (int l = 0; l < NW; ++l) u[l] = pk_sub(u[l], row2[l]);
+        for (int l = 0; l < NW; ++l) u[l] = w_sub<WB>(u[l], row2[l]);
         if (MARG) u0 -= 2 * A0[M + j];
       }
     }
@@ -952,12 +1005,11 @@ __device__ __forceinline__ void walk_tab16_bin(
     const cuint32_t *neg = Tab;
     asm volatile("" : "+s"(neg));
     if constexpr (MODE == MODE_MAX) {
-      tb16_max<NW, M, MARG>(u, u0, neg, pos0, best, std::make_integer_sequence<int, (1 << M) / 2>{});
+      tb16_max<NW, M, MARG, WB>(u, u0, neg, pos0, best, std::make_integer_sequence<int, (1 << M) / 2>{});
     } else {
       uint32_t fb = 0xFFFFFFFFu;
       const uint32_t hp = qh > 0 ? (k & 1u) : pseg;
-      if (hp) tb16_first<NW, M, MARG, true>(u, u0, neg, pos0, target, fb, std::make_integer_sequence<int, 1 << M>{});
-      else tb16_first<NW, M, MARG, false>(u, u0, neg, pos0, target, fb, std::make_integer_sequence<int, 1 << M>{});
+      tb16_first<NW, M, MARG, WB>(u, u0, neg, pos0, target, hp, fb);
       const uint32_t c = fb != 0xFFFFFFFFu ? (k << M) + fb : 0xFFFFFFFFu;
       first = c < first ? c : first;
     }

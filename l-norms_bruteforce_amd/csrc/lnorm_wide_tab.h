@@ -1,0 +1,573 @@
+// lnorm_wide_tab.h -- the tabulated d-ary walk (walk_blk16) over W waves,
+// for widths beyond one wave's registers, in both word formats (WB = 16:
+// packed halves; 32: one column per word), instantiated by
+// lnorm_wide_tab.hip (16-bit) and lnorm_wide_tab32.hip (32-bit).
+//
+// A workgroup of W waves walks the 64 lane segments of one group (as the
+// other W-wave walks); wave wv keeps words [wv*NW, (wv+1)*NW) of every label
+// sum.  After a high step each wave computes the two changed labels' subset
+// tables over its words (2 * 2^M * NW v_sad_u16, as walk_blk16) and stores
+// these partial tables in LDS; after a barrier every wave sums the W
+// partials, so each wave holds the full tables.  The block's d^M low
+// labelings are split over the waves (wave wv takes l = wv, wv + W, ...), and
+// the per-wave maxima (MODE_MAX) or first matching ranks (MODE_FIRST) are
+// combined once at the end.  Labels, ranks and the pattern order are those of
+// walk_blk16.
+#pragma once
+
+#include "lnorm_walks.h"
+
+namespace lnorm {
+
+// Words per wave of the W-wave tabulated walk (the widest single-wave
+// table shape up to 32 words), its widest wave count (8 for d <= 6, 4
+// beyond: at 8 waves of 512 threads a wave has 256 registers and no AGPRs)
+// and its table rows: the single-wave rule, with 2^M <= 64 / W so the W
+// waves' partial tables take 32 KiB of LDS, and M <= 2 at 8 waves for
+// d = 5, 6 (register room).
+constexpr int wt16_nw(int d) { return d <= 7 ? 32 : (d == 8 || d == 9) ? 24 : d == 10 ? 16 : 0; }
+constexpr int wt16_wmax(int d) { return d <= 6 ? 8 : 4; }
+// the table rows of the W-wave walk: the single wave's rows at its width,
+// capped by LDS and registers.  LDS: the W waves' partial tables of two
+// labels (W * 2 * 2^M * 256 B) fill up to 64 KiB for d = 3 and 32 KiB for
+// other d (at d = 2 both labels change every step, so a row more does not
+// cut the table work per labeling: it ran 1.7x slower).  Registers: above 4
+// waves a workgroup puts 2 waves on a SIMD (256 VGPRs each), where 4 rows fit
+// at d = 3, 3 at d = 2 and 4 and 2 beyond.  d = 3 over 4 waves: 13.8x the
+// packed W-wave walk at 256 columns, 9x with one row less.
+constexpr int wt16_m(int d, int w) {
+  const int m = blk16_low_rows(d, wt16_nw(d));
+  const int lds = d == 3 ? 128 : 64;   // 2^M * W at most
+  int capl = 0;
+  while ((2 << capl) * w <= lds) ++capl;
+  const int capr = w > 4 ? (d == 3 ? 4 : d >= 5 ? 2 : 3) : 6;
+  const int cap = capl < capr ? capl : capr;
+  return m < cap ? m : cap;
+}
+
+// wave counts of the W-wave tabulated walk: at most wt16_wmax(d), and not 5
+// at d = 6 (its 2 table rows spilled there)
+constexpr int kWtWaves[] = {2, 3, 4, 5, 6, 8};
+constexpr bool wt16_has(int d, int w) { return w <= wt16_wmax(d) && !(d == 6 && w == 5); }
+
+// This wave's partial tables of labels C and C + 1 into its LDS slice
+// (p[e][S][lane], e = 0 for C, 1 for C + 1).
+template <int C, int D, int NW, int NWT, int M, int WB>
+__device__ __forceinline__ void wt_partial_pair(const uint32_t (&u)[D][NW], const cuint32_t *neg,
+                                                uint32_t *p, int lane) {
+  const uint32_t z = w_zero<WB>();
+#pragma unroll
+  for (int S = 0; S < (1 << M); ++S) {
+    uint32_t a0 = z, a1 = z;
+#pragma unroll
+    for (int w = 0; w < NW; ++w) {
+      const uint32_t n = neg[(size_t)S * NWT + w];
+      a0 = w_sad<WB>(u[C][w], n, a0);
+      a1 = w_sad<WB>(u[C + 1][w], n, a1);
+    }
+    a0 = w_done<WB>(a0);
+    a1 = w_done<WB>(a1);
+    p[S * 64 + lane] = a0;
+    p[((1 << M) + S) * 64 + lane] = a1;
+  }
+}
+
+// The full tables of labels C and C + 1 from the W waves' partials
+// (T[c][0] = |u_c|; d <= 3: F_c(S); d >= 4: F_c(S) - |u_c|).
+template <int C, int D, int W, int M>
+__device__ __forceinline__ void wt_sum_pair(uint32_t (&T)[D][1 << M], const uint32_t *part, int lane) {
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    uint32_t f[1 << M];
+#pragma unroll
+    for (int S = 0; S < (1 << M); ++S) {
+      uint32_t acc = 0;
+#pragma unroll
+      for (int w2 = 0; w2 < W; ++w2) acc += part[(((size_t)w2 * 2 + e) * (1 << M) + S) * 64 + lane];
+      f[S] = acc;
+    }
+#pragma unroll
+    for (int S = 0; S < (1 << M); ++S) T[C + e][S] = (D <= 3 || S == 0) ? f[S] : f[S] - f[0];
+  }
+}
+
+template <int C, int D, int NW, int NWT, int W, int M, int WB>
+struct WtPair {
+  __device__ __forceinline__ static void partial(int lo, const uint32_t (&u)[D][NW], const cuint32_t *neg,
+                                                 uint32_t *p, int lane) {
+    if constexpr (C + 1 < D) {
+      if (lo == C) wt_partial_pair<C, D, NW, NWT, M, WB>(u, neg, p, lane);
+      else WtPair<C + 1, D, NW, NWT, W, M, WB>::partial(lo, u, neg, p, lane);
+    }
+  }
+  __device__ __forceinline__ static void sum(int lo, uint32_t (&T)[D][1 << M], const uint32_t *part, int lane) {
+    if constexpr (C + 1 < D) {
+      if (lo == C) wt_sum_pair<C, D, W, M>(T, part, lane);
+      else WtPair<C + 1, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
+    }
+  }
+};
+
+// Wave WV's share of the block's labelings: l = WV, WV + W, ..., in pairs
+// (one v_max3 per pair in MODE_MAX).
+template <int D, int M, bool ODD, int MODE, int W, int L>
+__device__ __forceinline__ void wt_eval_pair(const uint32_t (&T)[D][1 << M], uint32_t totU, int32_t &best,
+                                             uint32_t &fb, int32_t target) {
+  constexpr int NPAT = LowCode<D, M>::NPAT;
+  constexpr int L2 = L + W < NPAT ? L + W : L;
+  const auto cs = std::make_integer_sequence<int, D>{};
+  const int32_t a = blk16_obj<D, M, ODD, L>(T, totU, cs);
+  const int32_t b = blk16_obj<D, M, ODD, L2>(T, totU, cs);
+  if constexpr (MODE == MODE_MAX) {
+    const int32_t m = a > b ? a : b;
+    best = m > best ? m : best;
+  } else {
+    const uint32_t ca = (a == target) ? (uint32_t)L : 0xFFFFFFFFu;
+    const uint32_t cb = (b == target) ? (uint32_t)L2 : 0xFFFFFFFFu;
+    const uint32_t c = ca < cb ? ca : cb;
+    fb = c < fb ? c : fb;
+  }
+}
+
+template <int D, int M, bool ODD, int MODE, int W, int WV, int... Is>
+__device__ __forceinline__ void wt_eval(const uint32_t (&T)[D][1 << M], uint32_t totU, int32_t &best, uint32_t &fb,
+                                        int32_t target, std::integer_sequence<int, Is...>) {
+  (wt_eval_pair<D, M, ODD, MODE, W, WV + 2 * W * Is>(T, totU, best, fb, target), ...);
+}
+
+template <int D, int M, bool ODD, int MODE, int W, int WV>
+struct WtEval {
+  __device__ __forceinline__ static void run(int wv, const uint32_t (&T)[D][1 << M], uint32_t totU,
+                                             int32_t &best, uint32_t &fb, int32_t target) {
+    if constexpr (WV < W) {
+      if (wv == WV) {
+        constexpr int NPAT = LowCode<D, M>::NPAT;
+        constexpr int cnt = WV < NPAT ? (NPAT - WV + W - 1) / W : 0;
+        if constexpr (cnt > 0)
+          wt_eval<D, M, ODD, MODE, W, WV>(T, totU, best, fb, target, std::make_integer_sequence<int, (cnt + 1) / 2>{});
+      } else {
+        WtEval<D, M, ODD, MODE, W, WV + 1>::run(wv, T, totU, best, fb, target);
+      }
+    }
+  }
+};
+
+template <int D, int NW, int W, int M, int MODE, int WB>
+__global__ __launch_bounds__(64 * W) void k_wide_t16(
+    const uint32_t *__restrict__ Ap_, const uint32_t *__restrict__ neg_, int R, int q, uint64_t seg0, uint64_t nseg,
+    int32_t target, int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest,
+    unsigned long long *__restrict__ gfirst) {
+  constexpr int NWT = NW * W;
+  constexpr int NPAT = LowCode<D, M>::NPAT;
+  constexpr int PW = 2 * (1 << M) * 64;   // one wave's partial tables (words)
+  static_assert(W * PW * 4 <= 65536, "partial tables exceed 64 KiB");
+  __shared__ uint32_t part[W * PW];
+  // per-wave results, once the last partial tables are summed: every use of
+  // part ends with a barrier
+  uint32_t *fin = part;
+  const cuint32_t *Ap = as_const(Ap_);
+  const int lane = wave_lane();
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t g = blockIdx.x;
+  if (((g >> 1) << 7) >= nseg) return;   // uniform over the workgroup
+  const uint64_t local = lane_segment(g, lane);
+  const bool active = local < nseg;
+  const uint64_t seg = seg0 + local;
+  const uint32_t pseg = (uint32_t)__builtin_amdgcn_readfirstlane((int)((seg0 ^ g) & 1));
+  const int qh = q - M;
+  const int wbase = wv * NW;
+  uint32_t *mypart = part + (size_t)wv * PW;
+
+  uint32_t par_mask;
+  if (D & 1) par_mask = pseg ? ((1u << qh) - 1u) : 0u;
+  else par_mask = (qh > 0 && pseg) ? (1u << (qh - 1)) : 0u;
+  const uint32_t hpar0 = ((D & 1) || qh == 0) ? pseg : 0u;
+
+  uint32_t u[D][NW];
+#pragma unroll
+  for (int c = 0; c < D; ++c)
+#pragma unroll
+    for (int l = 0; l < NW; ++l)
+      u[c][l] = w_add<WB>(w_bias<WB>(), c == 0 ? Ap[(size_t)(R - 1) * NWT + wbase + l] : 0u);
+  uint64_t x = seg;
+  for (int i = 0; i < R - 1 - M; ++i) {
+    int lab;
+    if (i < qh) {
+      lab = ((par_mask >> i) & 1) ? D - 1 : 0;
+    } else {
+      const uint32_t r = (uint32_t)(x % (uint64_t)(2 * D));
+      lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
+      x /= (uint64_t)D;
+    }
+    seed_add16<D, NW, WB>(u, Ap + (size_t)(M + i) * NWT + wbase, lab);
+  }
+
+  // full tables of every label, two labels at a time (odd d: the last pair
+  // is (d-2, d-1) again)
+  uint32_t T[D][1 << M];
+#pragma unroll 1
+  for (int c0 = 0; c0 < D; c0 += 2) {
+    const int lo = c0 + 1 < D ? c0 : D - 2;
+    const cuint32_t *negk = as_const(neg_);
+    asm volatile("" : "+s"(negk));
+    WtPair<0, D, NW, NWT, W, M, WB>::partial(lo, u, negk + wbase, mypart, lane);
+    __syncthreads();
+    WtPair<0, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
+    __syncthreads();
+  }
+  uint32_t totU = 0;
+#pragma unroll
+  for (int c = 0; c < D; ++c) totU += T[c][0];
+
+  uint64_t biaspat = 0;
+  for (int i = 0; i < qh; ++i) biaspat |= (uint64_t)(16 - D) << (4 * i);
+  uint64_t cnt = biaspat;
+  uint32_t pm = par_mask;
+  uint32_t nblk = 1;
+  for (int i = 0; i < qh; ++i) nblk *= (uint32_t)D;
+
+  int32_t best = INT32_MIN;
+  uint32_t first = 0xFFFFFFFFu;
+  for (uint32_t k = 0; k < nblk; ++k) {   // nblk, j, from and up are workgroup-uniform
+    if (k > 0) {
+      const int j = __builtin_ctzll(~cnt) >> 2;
+      const uint64_t low = (1ull << (4 * j)) - 1ull;
+      cnt = (cnt + 1ull) | (biaspat & low);
+      const int v = (int)((cnt >> (4 * j)) & 15ull) - (16 - D);
+      const int p = (pm >> j) & 1;
+      const int from = p ? D - v : v - 1;
+      const int up = p ? 0 : 1;
+      pm ^= (1u << j) - 1u;
+      Move16<0, D, NW, WB>::run(u, Ap + (size_t)(M + j) * NWT + wbase, from, up);
+      const int lo = up ? from : from - 1;
+      const cuint32_t *negk = as_const(neg_);
+      asm volatile("" : "+s"(negk));
+      WtPair<0, D, NW, NWT, W, M, WB>::partial(lo, u, negk + wbase, mypart, lane);
+      __syncthreads();
+      WtPair<0, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
+      __syncthreads();
+      totU = 0;
+#pragma unroll
+      for (int c = 0; c < D; ++c) totU += T[c][0];
+    }
+    if constexpr (MODE == MODE_MAX) {
+      uint32_t unused = 0;
+      WtEval<D, M, false, MODE, W, 0>::run(wv, T, totU, best, unused, target);
+    } else {
+      uint32_t fb = 0xFFFFFFFFu;
+      if ((hpar0 ^ k) & 1u) {
+#pragma unroll
+        for (int c = 0; c < D; ++c)
+#pragma unroll
+          for (int S = 0; S < (1 << M); ++S) asm volatile("" : "+v"(T[c][S]));
+        asm volatile("" : "+v"(totU));
+        WtEval<D, M, true, MODE, W, 0>::run(wv, T, totU, best, fb, target);
+      } else {
+        WtEval<D, M, false, MODE, W, 0>::run(wv, T, totU, best, fb, target);
+      }
+      const uint32_t c = fb != 0xFFFFFFFFu ? k * (uint32_t)NPAT + fb : 0xFFFFFFFFu;
+      first = c < first ? c : first;
+    }
+  }
+  const uint32_t nsteps = nblk * (uint32_t)NPAT;
+
+  if constexpr (MODE == MODE_MAX) fin[wv * 64 + lane] = (uint32_t)best;
+  else fin[wv * 64 + lane] = first;
+  __syncthreads();
+  if (wv == 0) {
+    if constexpr (MODE == MODE_MAX) {
+      int32_t m = (int32_t)fin[lane];
+#pragma unroll
+      for (int w2 = 1; w2 < W; ++w2) m = (int32_t)fin[w2 * 64 + lane] > m ? (int32_t)fin[w2 * 64 + lane] : m;
+      if (active) unit_best[local] = m;
+      const int32_t wm = wave_max_i32(active ? m : INT32_MIN);
+      if (wm > __builtin_amdgcn_readfirstlane(*(volatile int32_t *)gbest)) atomicMax(gbest, wm);
+    } else {
+      uint32_t f = fin[lane];
+#pragma unroll
+      for (int w2 = 1; w2 < W; ++w2) f = fin[w2 * 64 + lane] < f ? fin[w2 * 64 + lane] : f;
+      const uint64_t r = (active && f != 0xFFFFFFFFu) ? (seg * (uint64_t)nsteps + f) : ~0ull;
+      const uint64_t m = wave_min_u64(r);
+      if (m != ~0ull) atomicMin(gfirst, (unsigned long long)m);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// The tabulated L1 / LM walk (walk_tab16_bin) over W waves: wave wv keeps
+// words [wv*NW, (wv+1)*NW) of the high sum u (LM: column 0, outside the
+// words, in every wave; wave 0 adds it).  Per block each wave computes the
+// 2^M patterns' partial sums over its words and stores them in LDS; after a
+// barrier wave wv totals the W partials of its patterns (l = wv, wv + W, ...)
+// and keeps their maximum (MODE_MAX) or first match (MODE_FIRST).
+// Shapes (W, NW): (8, 32), (10, 32), (12, 32), (16, 32), (16, 64) for 512, 640,
+// 768, 1024, 2048 columns.
+// ---------------------------------------------------------------------------
+constexpr int wtb_m(int w) { return w > 8 ? 4 : 5; }   // partials: W * 2^M * 256 B <= 64 KiB
+
+template <int NW, int NWT, int M, bool MARG, int WB, int... Ss>
+__device__ __forceinline__ void wtb_partials(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
+                                             const cint32_t *pos0, bool wave0, uint32_t *p, int lane,
+                                             std::integer_sequence<int, Ss...>) {
+  auto one = [&](auto SS) {
+    constexpr int S = decltype(SS)::value;
+    uint32_t acc = (MARG && wave0) ? (uint32_t)(u0 + pos0[S]) : w_zero<WB>();
+#pragma unroll
+    for (int w = 0; w < NW; ++w) acc = w_sad<WB>(u[w], neg[(size_t)S * NWT + w], acc);
+    p[S * 64 + lane] = acc;
+  };
+  (one(std::integral_constant<int, Ss>{}), ...);
+}
+
+template <int W, int M, int MODE, bool ODD, int WV, int... Is>
+__device__ __forceinline__ void wtb_eval(const uint32_t *part, int lane, int32_t &best, uint32_t &fb, int32_t target,
+                                         std::integer_sequence<int, Is...>) {
+  auto one = [&](auto II) {
+    constexpr int l = WV + W * decltype(II)::value;
+    // MODE_MAX: any order (a block holds all 2^M patterns); MODE_FIRST: the
+    // pattern of low index l
+    constexpr int S = MODE == MODE_MAX ? l : ((l ^ (l >> 1)) ^ (ODD ? (1 << (M - 1)) : 0));
+    uint32_t tot = 0;
+#pragma unroll
+    for (int w2 = 0; w2 < W; ++w2) tot += part[((size_t)w2 * (1 << M) + S) * 64 + lane];
+    if constexpr (MODE == MODE_MAX) {
+      best = (int32_t)tot > best ? (int32_t)tot : best;
+    } else {
+      const uint32_t c = ((int32_t)tot == target) ? (uint32_t)l : 0xFFFFFFFFu;
+      fb = c < fb ? c : fb;
+    }
+  };
+  (one(std::integral_constant<int, Is>{}), ...);
+}
+
+template <int W, int M, int MODE, bool ODD, int WV>
+struct WtbEval {
+  __device__ __forceinline__ static void run(int wv, const uint32_t *part, int lane, int32_t &best, uint32_t &fb,
+                                             int32_t target) {
+    if constexpr (WV < W) {
+      if (wv == WV) {
+        constexpr int cnt = WV < (1 << M) ? ((1 << M) - WV + W - 1) / W : 0;
+        if constexpr (cnt > 0)
+          wtb_eval<W, M, MODE, ODD, WV>(part, lane, best, fb, target, std::make_integer_sequence<int, cnt>{});
+      } else {
+        WtbEval<W, M, MODE, ODD, WV + 1>::run(wv, part, lane, best, fb, target);
+      }
+    }
+  }
+};
+
+template <int W, int NW, bool MARG, int MODE, int WB>
+__global__ __launch_bounds__(64 * W) void k_wide_tb16(
+    const uint32_t *__restrict__ Ap_, const uint32_t *__restrict__ tab_, int R, int q, uint64_t seg0, uint64_t nseg,
+    int32_t target, int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest,
+    unsigned long long *__restrict__ gfirst) {
+  constexpr int NWT = NW * W, M = wtb_m(W);
+  static_assert(W * (1 << M) * 64 * 4 <= 65536, "partials exceed 64 KiB");
+  __shared__ uint32_t part[W * (1 << M) * 64];
+  uint32_t *fin = part;   // per-wave results, once the walk's last barrier has passed
+  const cuint32_t *Ap = as_const(Ap_);
+  const int lane = wave_lane();
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t g = blockIdx.x;
+  if (((g >> 1) << 7) >= nseg) return;   // uniform over the workgroup
+  const uint64_t local = lane_segment(g, lane);
+  const bool active = local < nseg;
+  const uint64_t seg = seg0 + local;
+  const uint32_t pseg = (uint32_t)__builtin_amdgcn_readfirstlane((int)((seg0 ^ g) & 1));
+  const int qh = q - M;
+  const int wbase = wv * NW;
+  const cuint32_t *A2 = Ap + (size_t)R * NWT;                          // packed 2 A
+  const cint32_t *A0 = (const cint32_t *)(Ap + (size_t)2 * R * NWT);   // LM: column 0
+  const cint32_t *pos0 = (const cint32_t *)(as_const(tab_) + ((size_t)NWT << M));
+  uint32_t *mypart = part + (size_t)wv * (1 << M) * 64;
+
+  const uint64_t h0 = seg << qh;
+  const uint64_t g0 = h0 ^ (h0 >> 1);
+  uint32_t u[NW];
+  int32_t u0 = MARG ? A0[R - 1] : 0;
+#pragma unroll
+  for (int l = 0; l < NW; ++l) u[l] = w_add<WB>(w_bias<WB>(), Ap[(size_t)(R - 1) * NWT + wbase + l]);
+  for (int j = 0; j < R - 1 - M; ++j) {
+    const uint32_t pmask = ((g0 >> j) & 1) ? 0xFFFFFFFFu : 0u;   // +row where the bit is 1, -row elsewhere
+    const cuint32_t *row = Ap + (size_t)(M + j) * NWT + wbase;
+#pragma unroll
+    for (int l = 0; l < NW; ++l) {
+      const uint32_t v = row[l];
+      u[l] = w_add<WB>(u[l], v & pmask);
+      u[l] = w_sub<WB>(u[l], v & ~pmask);
+    }
+    if (MARG) u0 += pmask ? A0[M + j] : -A0[M + j];
+  }
+
+  int32_t best = INT32_MIN;
+  uint32_t first = 0xFFFFFFFFu;
+  const uint32_t nblk = 1u << qh;
+  for (uint32_t k = 0; k < nblk; ++k) {   // workgroup-uniform
+    if (k > 0) {
+      const int j = __builtin_ctz(k);
+      const uint32_t hb = (j + 1 < qh) ? ((k >> (j + 1)) & 1u) : pseg;
+      const cuint32_t *row2 = A2 + (size_t)(M + j) * NWT + wbase;
+      if (hb == 0) {
+#pragma unroll
+        for (int l = 0; l < NW; ++l) u[l] = w_add<WB>(u[l], row2[l]);
+        if (MARG) u0 += 2 * A0[M + j];
+      } else {
+#pragma unroll
+        for (int l = 0; l < NW; ++l) u[l] = w_sub<WB>(u[l], row2[l]);
+        if (MARG) u0 -= 2 * A0[M + j];
+      }
+    }
+    const cuint32_t *negk = as_const(tab_);
+    asm volatile("" : "+s"(negk));
+    if constexpr (MODE == MODE_MAX) {
+      wtb_partials<NW, NWT, M, MARG, WB>(u, u0, negk + wbase, pos0, wv == 0, mypart, lane,
+                                         std::make_integer_sequence<int, 1 << M>{});
+    } else {
+      // MODE_FIRST re-walks a single unit: the patterns' partial sums as a
+      // loop over the wave-uniform pattern index (half the unrolled code)
+#pragma unroll 1
+      for (int S = 0; S < (1 << M); ++S) {
+        const cuint32_t *row = negk + wbase + (size_t)S * NWT;
+        uint32_t acc = (MARG && wv == 0) ? (uint32_t)(u0 + pos0[S]) : w_zero<WB>();
+#pragma unroll
+        for (int w = 0; w < NW; ++w) acc = w_sad<WB>(u[w], row[w], acc);
+        mypart[S * 64 + lane] = acc;
+      }
+    }
+    __syncthreads();
+    if constexpr (MODE == MODE_MAX) {
+      uint32_t unused = 0;
+      WtbEval<W, M, MODE, false, 0>::run(wv, part, lane, best, unused, target);
+    } else {
+      uint32_t fb = 0xFFFFFFFFu;
+      const uint32_t hp = qh > 0 ? (k & 1u) : pseg;
+      if (hp) WtbEval<W, M, MODE, true, 0>::run(wv, part, lane, best, fb, target);
+      else WtbEval<W, M, MODE, false, 0>::run(wv, part, lane, best, fb, target);
+      const uint32_t c = fb != 0xFFFFFFFFu ? (k << M) + fb : 0xFFFFFFFFu;
+      first = c < first ? c : first;
+    }
+    __syncthreads();
+  }
+  const uint32_t nsteps = nblk << M;
+
+  if constexpr (MODE == MODE_MAX) fin[wv * 64 + lane] = (uint32_t)best;
+  else fin[wv * 64 + lane] = first;
+  __syncthreads();
+  if (wv == 0) {
+    if constexpr (MODE == MODE_MAX) {
+      int32_t m = (int32_t)fin[lane];
+#pragma unroll
+      for (int w2 = 1; w2 < W; ++w2) m = (int32_t)fin[w2 * 64 + lane] > m ? (int32_t)fin[w2 * 64 + lane] : m;
+      if (active) unit_best[local] = m;
+      const int32_t wm = wave_max_i32(active ? m : INT32_MIN);
+      if (wm > __builtin_amdgcn_readfirstlane(*(volatile int32_t *)gbest)) atomicMax(gbest, wm);
+    } else {
+      uint32_t f = fin[lane];
+#pragma unroll
+      for (int w2 = 1; w2 < W; ++w2) f = fin[w2 * 64 + lane] < f ? fin[w2 * 64 + lane] : f;
+      const uint64_t r = (active && f != 0xFFFFFFFFu) ? (seg * (uint64_t)nsteps + f) : ~0ull;
+      const uint64_t m = wave_min_u64(r);
+      if (m != ~0ull) atomicMin(gfirst, (unsigned long long)m);
+    }
+  }
+}
+
+// Launchers of both word formats; the width in columns is a.ny_pad (16-bit:
+// 2 columns per word).
+template <int W, int NW, bool MARG, int WB>
+static hipError_t lwtb(Mode mode, const PlainArgs &a, hipStream_t st) {
+  const uint64_t groups = 2 * ((a.nseg + 127) / 128);
+  const dim3 grid((unsigned)groups), blk(64 * W);
+  const uint32_t *Ap = (const uint32_t *)a.A;
+  const uint32_t *tab = (const uint32_t *)a.D;
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_wide_tb16<W, NW, MARG, MODE_MAX, WB>), grid, blk, 0, st, Ap, tab, a.R, a.q, a.seg0,
+                       a.nseg, a.target, a.unit_best, a.gbest, a.gfirst);
+  else
+    hipLaunchKernelGGL((k_wide_tb16<W, NW, MARG, MODE_FIRST, WB>), grid, blk, 0, st, Ap, tab, a.R, a.q, a.seg0,
+                       a.nseg, a.target, a.unit_best, a.gbest, a.gfirst);
+  return hipGetLastError();
+}
+
+// Shapes (W, NW) of the W-wave L1 / LM walk by total words: 256, 320, 384,
+// 512 or 1024 words (8, 10, 12 or 16 waves of 32 words, 16 of 64).
+template <bool MARG, int WB>
+static hipError_t lwtb_m(Mode mode, const PlainArgs &a, hipStream_t st) {
+  switch (WB == 16 ? a.ny_pad / 2 : a.ny_pad) {
+    case 256: return lwtb<8, 32, MARG, WB>(mode, a, st);
+    case 320: return lwtb<10, 32, MARG, WB>(mode, a, st);
+    case 384: return lwtb<12, 32, MARG, WB>(mode, a, st);
+    case 512: return lwtb<16, 32, MARG, WB>(mode, a, st);
+    case 1024: return lwtb<16, 64, MARG, WB>(mode, a, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+// the W-wave L1 / LM walk's total words for `need` words (0: none)
+inline int wtb_words_for(int need) {
+  for (int w : {256, 320, 384, 512, 1024})
+    if (need <= w) return w;
+  return 0;
+}
+
+template <int D, int W, int WB>
+static hipError_t lwt(Mode mode, const PlainArgs &a, hipStream_t st) {
+  constexpr int NW = wt16_nw(D), M = wt16_m(D, W);
+  const uint64_t groups = 2 * ((a.nseg + 127) / 128);
+  const dim3 grid((unsigned)groups), blk(64 * W);
+  const uint32_t *Ap = (const uint32_t *)a.A;
+  const uint32_t *neg = (const uint32_t *)a.D;
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_wide_t16<D, NW, W, M, MODE_MAX, WB>), grid, blk, 0, st, Ap, neg, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
+  else
+    hipLaunchKernelGGL((k_wide_t16<D, NW, W, M, MODE_FIRST, WB>), grid, blk, 0, st, Ap, neg, a.R, a.q, a.seg0,
+                       a.nseg, a.target, a.unit_best, a.gbest, a.gfirst);
+  return hipGetLastError();
+}
+
+template <int D, int WB>
+static hipError_t lwt_d(Mode mode, const PlainArgs &a, hipStream_t st) {
+  const int words = WB == 16 ? a.ny_pad / 2 : a.ny_pad;
+  if (words == 2 * wt16_nw(D)) return lwt<D, 2, WB>(mode, a, st);
+  if (words == 3 * wt16_nw(D)) return lwt<D, 3, WB>(mode, a, st);
+  if (words == 4 * wt16_nw(D)) return lwt<D, 4, WB>(mode, a, st);
+  if constexpr (wt16_has(D, 5))
+    if (words == 5 * wt16_nw(D)) return lwt<D, 5, WB>(mode, a, st);
+  if constexpr (wt16_has(D, 6))
+    if (words == 6 * wt16_nw(D)) return lwt<D, 6, WB>(mode, a, st);
+  if constexpr (wt16_has(D, 8))
+    if (words == 8 * wt16_nw(D)) return lwt<D, 8, WB>(mode, a, st);
+  return hipErrorInvalidValue;
+}
+
+// the W-wave d-ary walk's total words for `need` words (0: none)
+inline int wt_words_for(int d, int need) {
+  const int nw = (d >= 2 && d <= 10) ? wt16_nw(d) : 0;
+  for (int w : kWtWaves)
+    if (nw > 0 && wt16_has(d, w) && need <= nw * w) return nw * w;
+  return 0;
+}
+
+inline int wt_rows_for(int d, int words) {
+  const int nw = (d >= 2 && d <= 10) ? wt16_nw(d) : 0;
+  return nw > 0 ? wt16_m(d, words / nw) : 0;
+}
+
+template <int WB>
+static hipError_t lwt_any(Mode mode, const PlainArgs &a, hipStream_t st) {
+  switch (a.d) {
+    case 2: return lwt_d<2, WB>(mode, a, st);
+    case 3: return lwt_d<3, WB>(mode, a, st);
+    case 4: return lwt_d<4, WB>(mode, a, st);
+    case 5: return lwt_d<5, WB>(mode, a, st);
+    case 6: return lwt_d<6, WB>(mode, a, st);
+    case 7: return lwt_d<7, WB>(mode, a, st);
+    case 8: return lwt_d<8, WB>(mode, a, st);
+    case 9: return lwt_d<9, WB>(mode, a, st);
+    case 10: return lwt_d<10, WB>(mode, a, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+}  // namespace lnorm

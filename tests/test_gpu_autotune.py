@@ -13,7 +13,7 @@ def _rate(e):
     return e["labelings_per_s"]
 
 
-@pytest.mark.parametrize("case", ["blocked16", "blocked16_marg", "blocked32", "plain16_d3", "plain_L1"])
+@pytest.mark.parametrize("case", ["blocked16", "blocked16_marg", "blocked32", "plain16_d3", "plain_L1", "table32_L1"])
 def test_autotune_single(pkg, case):
     from lnorm_amd import gen
     print("autotune", case, flush=True)
@@ -23,7 +23,9 @@ def test_autotune_single(pkg, case):
     elif case == "blocked16_marg":
         A, n, m = gen.gen_matrix(34, 30, 100, 8), 1, 1
     elif case == "blocked32":
-        A, n, m = gen.gen_matrix(33, 40, 30000, 9), 1, 0
+        A, n, m, kw = gen.gen_matrix(33, 40, 30000, 9), 1, 0, {"kernel": "blocked"}
+    elif case == "table32_L1":
+        A, n, m = gen.gen_matrix(33, 40, 30000, 9), 1, 0      # int16 entries: AUTO takes table32
     elif case == "plain16_d3":
         A, n, m = gen.gen_matrix(18, 32, 50, 103), 3, 0
     else:

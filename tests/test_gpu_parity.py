@@ -49,18 +49,18 @@ def test_golden_vectors_all_kernels(pkg, golden):
         if p.R == 0:
             continue
         want_line = case["stdout"].strip().splitlines()[-1]
-        for kernel in ("auto", "plain", "general", "blocked", "plain16", "table16"):
+        for kernel in ("auto", "plain", "general", "blocked", "plain16", "table16", "table32"):
             try:
                 norm, strat, res = pkg.calc_lnorm(A, p.n, p.marginal, kernel=kernel)
             except pkg.LnormError as e:
-                assert e.code == -5 and kernel in ("plain", "blocked", "plain16", "table16"), (case["name"], kernel,
-                                                                                              str(e))
+                assert e.code == -5 and kernel in ("plain", "blocked", "plain16", "table16", "table32"), (
+                    case["name"], kernel, str(e))
                 continue
             line, fname, lines = ho.result_lines(p, norm, strat)
             assert line == want_line, (case["name"], kernel, res["kernel_name"])
             assert {fname: "".join(s + "\n" for s in lines)} == case["strategy"], (case["name"], kernel)
             n_checked += 1
-    assert n_checked > 150
+    assert n_checked > 180
 
 
 @pytest.mark.parametrize("n,marg", [(1, 0), (1, 1), (2, 0), (3, 0), (4, 0), (5, 0), (6, 0), (7, 0),
@@ -118,13 +118,18 @@ def test_blocked32_vs_oracle(pkg, marg, P):
         assert res["kernel_name"].startswith("blocked32"), res["kernel_name"]
 
 
-def test_blocked32_auto_for_int16_entries(pkg):
+def test_int16_entries_auto_and_blocked32(pkg):
     from lnorm_amd import gen
-    # int16 entries: column sums beyond 2^15, AUTO takes the 32-bit blocked kernel
+    # int16 entries: column sums beyond 2^15; AUTO takes the 32-bit tabulated
+    # walk, and the blocked kernel runs them in 32-bit words
     A = gen.gen_matrix(128, 20, 30000, 105).T.copy()       # 20 x 128, 2^19 sign vectors
     res = _agree(pkg, A, 1, 0)
+    assert res["kernel_name"].startswith("table32<L1,words=128"), res["kernel_name"]
+    res = _agree(pkg, A, 1, 0, kernel="blocked")
     assert res["kernel_name"].startswith("blocked32<nc=32,P=2,W=4"), res["kernel_name"]
     res = _agree(pkg, A[:, :50].copy(), 1, 1)
+    assert res["kernel_name"].startswith("table32<LM"), res["kernel_name"]
+    res = _agree(pkg, A[:, :50].copy(), 1, 1, kernel="blocked")
     assert res["kernel_name"].startswith("blocked32"), res["kernel_name"]
 
 
@@ -468,18 +473,25 @@ def test_table16_binary_equals_blocked_large(pkg):
 
 
 # (d, rows, column counts) for the W-wave tabulated walk: beyond one wave's
-# tables, at 2, 4 and 8 waves (4 at most for d >= 7)
+# tables, at every wave count it reaches (2, 3, 4, 5, 6 and 8 waves for
+# d <= 6, not 5 at d = 6; up to 4 beyond)
 WIDE_T16 = [
-    (2, 12, [193, 200, 256, 257, 400, 512]),
-    (3, 10, [129, 130, 256, 300, 512]),
-    (4, 9, [97, 128, 200, 256, 512]),
-    (5, 8, [81, 128, 129, 256, 512]),
-    (6, 7, [65, 128, 200, 512]),
+    (2, 12, [193, 200, 256, 257, 350, 400, 512]),
+    (3, 10, [129, 130, 256, 300, 350, 512]),
+    (4, 9, [97, 128, 150, 200, 256, 300, 350, 512]),
+    (5, 8, [81, 128, 129, 256, 300, 350, 512]),
+    (6, 7, [65, 128, 150, 200, 350, 512]),
     (7, 7, [65, 128, 129, 256]),
     (8, 6, [49, 96, 97, 192]),
-    (9, 6, [49, 96, 192]),
+    (9, 6, [49, 96, 120, 192]),
     (10, 6, [33, 64, 65, 128]),
 ]
+# words per wave of the W-wave tabulated walk, its wave counts, and the
+# single-wave walk's widest word count (beyond which the W waves take over)
+WT16_NW = {2: 32, 3: 32, 4: 32, 5: 32, 6: 32, 7: 32, 8: 24, 9: 24, 10: 16}
+WT16_WAVES = {d: [w for w in (2, 3, 4, 5, 6, 8) if w <= (8 if d <= 6 else 4) and not (d == 6 and w == 5)]
+              for d in range(2, 11)}
+BLK16_MAX_WORDS = {2: 96, 3: 64, 4: 48, 5: 40, 6: 32, 7: 32, 8: 24, 9: 24, 10: 16}
 
 
 @pytest.mark.parametrize("d,R,cols", WIDE_T16)
@@ -489,11 +501,16 @@ def test_table16w_walk_vs_oracle(pkg, d, R, cols):
     strategy and rank at every wave count, at forced segment lengths and
     with the fewest enumerated rows it accepts."""
     rng = np.random.default_rng(990 + d)
+    seen = set()
     for C in cols:
         A = _rand(rng, R, C, 50)
         print("table16w", d, A.shape, flush=True)
         res = _agree(pkg, A, d, 0)
         assert res["kernel_name"].startswith("table16w<"), res["kernel_name"]
+        seen.add(int(res["kernel_name"].split("words=")[1].split(",")[0]))
+    # every wave count the walk reaches is covered
+    reach = {WT16_NW[d] * w for w in WT16_WAVES[d] if WT16_NW[d] * w > BLK16_MAX_WORDS[d]}
+    assert seen == reach, (d, sorted(seen), sorted(reach))
     A = _rand(rng, R, cols[1], 50)
     M = int(pkg.calc_lnorm(A, d, 0)[2]["kernel_name"].split("M=")[1].split(",")[0])
     for q in (M, M + 1, M + 2):
@@ -549,11 +566,13 @@ def test_wide16_equals_general_larger(pkg):
 
 def test_wide16_needs_16bit_sums(pkg):
     """Column sums above 32767 do not fit the packed halves: AUTO takes the
-    general kernel and a forced W-wave walk is refused."""
+    32-bit tabulated W-wave walk (not the general kernel), and a forced
+    packed W-wave walk is refused."""
     rng = np.random.default_rng(5)
     A = _rand(rng, 6, 120, 30000)
     res = _agree(pkg, A, 3, 0)
-    assert res["kernel_name"].startswith("general<"), res["kernel_name"]
+    assert res["kernel_name"].startswith("table32w<"), res["kernel_name"]
+    assert _agree(pkg, A, 3, 0, kernel="general")["kernel_name"].startswith("general<")
     with pytest.raises(pkg.LnormError) as e:
         pkg.calc_lnorm(A, 3, 0, kernel="wide")
     assert e.value.code == -5
