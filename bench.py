@@ -73,30 +73,41 @@ def valu_ops_per_unit(kernel: str, ny: int, marginal: int, P: int = 2) -> float:
         nwt = 256 if words <= 256 else 512 if words <= 512 else 1024
         w, m = (8, 5) if nwt == 256 else (16, 4)
         return nwt + nwt / 2 ** m + w / 2 + 1 + marginal
+    if kernel == "table32":          # 32-bit words: one v_sad_u32 per padded |.| column, half a v_max3,
+        words = next(w for w in TAB32_WORDS if w >= ny - marginal)   # the high step per 2^M patterns
+        return words + 0.5 + marginal + words / 2 ** (5 if words <= 128 else 4)
+    if kernel == "table32w":         # over W waves of 32 (64 at 1024) words, 2^M patterns per block
+        words = next(w for w in (256, 320, 384, 512, 1024) if w >= ny - marginal)
+        w, m = {256: (8, 5), 320: (10, 4), 384: (12, 4), 512: (16, 4), 1024: (16, 4)}[words]
+        return words + words / 2 ** m + w / 2 + 1 + marginal
     return 2 * ny + 1
+
+
+# instantiated widths (words = |.| columns) of the 32-bit tabulated L1 walk in one wave
+TAB32_WORDS = (8, 16, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192)
 
 
 AUTOTUNE = True    # bench.py --no-autotune clears it
 
 
-def run_steps(pkg, A, kernel, steps, warmup, P=0):
+def run_steps(pkg, A, kernel, steps, warmup, P=0, n=1):
     """`steps` timed calls (norm + strategy) after `warmup` untimed ones.
     With autotuning, the launch shape is tuned once before any of them
     (lnorm_hip_autotune, cached) and every call uses the cached shape."""
     res = None
     tuned = None
     if AUTOTUNE:
-        best, _ = pkg.autotune(A, 1, 0, kernel=kernel)
+        best, _ = pkg.autotune(A, n, 0, kernel=kernel)
         tuned = {k: best[k] for k in ("patterns_per_lane", "wave_log2", "block")}
     kw = {"kernel": kernel, "patterns_per_lane": P, "autotune": 1 if AUTOTUNE else 0}
     for _ in range(warmup):
-        pkg.calc_lnorm(A, 1, 0, **kw)
+        pkg.calc_lnorm(A, n, 0, **kw)
     times, kms = [], []
     out = None
     t0 = time.perf_counter()
     for _ in range(steps):
         t = time.perf_counter()
-        norm, strat, res = pkg.calc_lnorm(A, 1, 0, **kw)
+        norm, strat, res = pkg.calc_lnorm(A, n, 0, **kw)
         times.append(time.perf_counter() - t)
         kms.append(res["ms_kernel"])
         if out is None:
@@ -436,10 +447,48 @@ def wide_lines(pkg, gen) -> dict:
     return out
 
 
+# int16-range workloads: column sums beyond 16 bits, so the 32-bit words
+#   int16:      L1 of the 128 x 40 int16 matrix (entries in [-30000, 30000],
+#               seed 105), laid out 40 x 128: 2^39 sign vectors
+#   wide_int16: L1 of a 34 x 1000 matrix, entries in [-30000, 30000] (seed
+#               110): 2^33 sign vectors over W waves
+#   l3_int16:   L3 of a 20 x 200 matrix, entries in [-20000, 20000] (seed
+#               111): 3^19 labelings over W waves
+INT16_CASES = {
+    "int16": lambda gen: (gen.gen_matrix(128, 40, 30000, 105).T.copy(), 1, 2 ** 39),
+    "wide_int16": lambda gen: (gen.gen_matrix(34, 1000, 30000, 110), 1, 2 ** 33),
+    "l3_int16": lambda gen: (gen.gen_matrix(20, 200, 20000, 111), 3, 3 ** 19),
+}
+
+
+def int16_line(pkg, gen, case: str, kernel: str, steps: int, warmup: int) -> dict:
+    """One int16-range workload on one kernel: rates, the kernel that ran
+    and, for the L1 walks, the VALU fraction by the kernel's op model."""
+    A, n, total = INT16_CASES[case](gen)
+    r = run_steps(pkg, A, kernel, steps, warmup, n=n)
+    name = r["res"]["kernel_name"]
+    model = next((k for k in ("table32w", "table32", "blocked32", "plain") if name.startswith(k)), None)
+    if n == 1 and model:
+        s = summarize(r, total, A.shape[1], model)
+    else:
+        steps_ = len(r["step_s"])
+        kms = sum(r["kernel_ms"]) / steps_
+        s = {"value": total * steps_ / r["wall_s"], "ms_per_step": 1e3 * r["wall_s"] / steps_, "kernel_ms": kms,
+             "kernel_labelings_per_s": total / (kms / 1e3), "kernel_name": name, "norm": r["norm"],
+             "best_rank": r["best_rank"]}
+    return {"workload": case, "units": total, **s}
+
+
 def run_case(pkg, gen, case: str, steps: int, warmup: int) -> int:
     """One workload alone: the headline matrix on the blocked, plain or
-    tabulated kernel, the L3 workload, the wide L1 matrices or the 4096-item
-    batch."""
+    tabulated kernel, the L3 workload, the wide L1 matrices, the 4096-item
+    batch, or an int16-range workload (int16, wide_int16, l3_int16; with
+    the suffix _blocked, _plain, _wide or _general on another kernel)."""
+    base = case.rsplit("_", 1)[0] if case.endswith(("_blocked", "_plain", "_wide", "_general")) else case
+    if base in INT16_CASES:
+        kernel = case[len(base) + 1:] if case != base else "auto"
+        print(json.dumps({"case": case, **int16_line(pkg, gen, base, kernel, steps, warmup)}), flush=True)
+        return 0
     if case == "batch":
         print(json.dumps({"case": case, **batch_line(pkg, gen, steps)}), flush=True)
         return 0
@@ -467,7 +516,9 @@ def main():
                     help="run the default launch shapes instead of the autotuner's (tuned before timing)")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the autotuner's candidates on the bench workloads and print them as JSON")
-    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "l3", "wide"],
+    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "l3", "wide", "int16", "int16_blocked",
+                                       "int16_plain", "wide_int16", "wide_int16_wide", "l3_int16",
+                                       "l3_int16_general"],
                     help="run only this workload (for profiler passes) and print its figures")
     ap.add_argument("--copy-calibration", type=int, metavar="MIB",
                     help="run one device-to-device copy of MIB MiB (torch) and exit: the reference "

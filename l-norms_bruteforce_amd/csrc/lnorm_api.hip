@@ -93,7 +93,6 @@ struct DevCtx {
   bool noalloc = false;   // every buffer is a fixed carve-out of `scratch`
 };
 
-std::mutex g_ctx_mu;
 DevCtx g_ctx[64];
 
 int ensure(Buf &b, size_t bytes) {

@@ -153,12 +153,14 @@ hipError_t launch_tab16_bin(Kind kind, Mode mode, const PlainArgs &a, hipStream_
 // The tabulated walks in 32-bit words (one column per word, biased by 2^31,
 // v_sad_u32): int32 inputs whose column sums exceed the 16-bit walks' bound.
 // Their widths are in words, one column each (PlainArgs::ny_pad = words).
-// L1 / LM table rows: 5 up to 128 words, 4 beyond.
+// L1 / LM table rows in one wave: 5 up to 128 words, 4 beyond.  The table
+// (2^M x NW words, read by scalar loads every block) stays within 16 KiB: 6
+// rows at 128 words (32 KiB) ran 1.4x slower than 5.
 constexpr int tab32_bin_rows(int nw) { return nw <= 128 ? 5 : 4; }
 int tab32_bin_words(int nabs);        // one wave, up to 192 |.| columns (0: none)
 int tab32_bin_blocks_per_cu(Kind kind, int words, int block);
 hipError_t launch_tab32_bin(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st, int block);
-int wtb32_words(int nabs);            // W waves, 256..1024 |.| columns (0: none)
+int wtb32_words(int nabs);            // W waves (the 16-bit walk's shapes): 193..1024 |.| columns (0: none)
 int wtb32_rows(int words);
 hipError_t launch_wide_tb32(Kind kind, Mode mode, const PlainArgs &a, hipStream_t st);
 int blk32_words(int d, int ny);       // d-ary, one wave: up to 96 columns at d = 2 ... 16 at d = 10 (0: none)

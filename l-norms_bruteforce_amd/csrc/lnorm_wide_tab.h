@@ -356,12 +356,13 @@ struct WtbEval {
   }
 };
 
-template <int W, int NW, bool MARG, int MODE, int WB>
+// M: the table rows (wtb_m(W) in both word formats).
+template <int W, int NW, bool MARG, int MODE, int WB, int M>
 __global__ __launch_bounds__(64 * W) void k_wide_tb16(
     const uint32_t *__restrict__ Ap_, const uint32_t *__restrict__ tab_, int R, int q, uint64_t seg0, uint64_t nseg,
     int32_t target, int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest,
     unsigned long long *__restrict__ gfirst) {
-  constexpr int NWT = NW * W, M = wtb_m(W);
+  constexpr int NWT = NW * W;
   static_assert(W * (1 << M) * 64 * 4 <= 65536, "partials exceed 64 KiB");
   __shared__ uint32_t part[W * (1 << M) * 64];
   uint32_t *fin = part;   // per-wave results, once the walk's last barrier has passed
@@ -474,31 +475,37 @@ __global__ __launch_bounds__(64 * W) void k_wide_tb16(
 
 // Launchers of both word formats; the width in columns is a.ny_pad (16-bit:
 // 2 columns per word).
-template <int W, int NW, bool MARG, int WB>
+template <int W, int NW, bool MARG, int WB, int M>
 static hipError_t lwtb(Mode mode, const PlainArgs &a, hipStream_t st) {
   const uint64_t groups = 2 * ((a.nseg + 127) / 128);
   const dim3 grid((unsigned)groups), blk(64 * W);
   const uint32_t *Ap = (const uint32_t *)a.A;
   const uint32_t *tab = (const uint32_t *)a.D;
   if (mode == MODE_MAX)
-    hipLaunchKernelGGL((k_wide_tb16<W, NW, MARG, MODE_MAX, WB>), grid, blk, 0, st, Ap, tab, a.R, a.q, a.seg0,
+    hipLaunchKernelGGL((k_wide_tb16<W, NW, MARG, MODE_MAX, WB, M>), grid, blk, 0, st, Ap, tab, a.R, a.q, a.seg0,
                        a.nseg, a.target, a.unit_best, a.gbest, a.gfirst);
   else
-    hipLaunchKernelGGL((k_wide_tb16<W, NW, MARG, MODE_FIRST, WB>), grid, blk, 0, st, Ap, tab, a.R, a.q, a.seg0,
+    hipLaunchKernelGGL((k_wide_tb16<W, NW, MARG, MODE_FIRST, WB, M>), grid, blk, 0, st, Ap, tab, a.R, a.q, a.seg0,
                        a.nseg, a.target, a.unit_best, a.gbest, a.gfirst);
   return hipGetLastError();
 }
 
-// Shapes (W, NW) of the W-wave L1 / LM walk by total words: 256, 320, 384,
-// 512 or 1024 words (8, 10, 12 or 16 waves of 32 words, 16 of 64).
+// Shapes (W, NW) of the W-wave L1 / LM walk by total words, in both word
+// formats: 256, 320, 384, 512 or 1024 words (8, 10, 12 or 16 waves of 32
+// words, 16 of 64) with wtb_m(W) table rows.  Each wave reads its slice of
+// every table row by scalar loads, so the scalar-cache working set of a CU
+// is the whole table: in 32-bit words the 1024-word shape misses in the
+// scalar cache on about half its loads (3 rows), where one wave reading the
+// whole table misses on 2%.  Fewer rows (3 at 1024 words: 0.92x) or 128-word
+// waves (2 to 8 waves, 5 rows: 0.77x) measured slower than these shapes.
 template <bool MARG, int WB>
 static hipError_t lwtb_m(Mode mode, const PlainArgs &a, hipStream_t st) {
   switch (WB == 16 ? a.ny_pad / 2 : a.ny_pad) {
-    case 256: return lwtb<8, 32, MARG, WB>(mode, a, st);
-    case 320: return lwtb<10, 32, MARG, WB>(mode, a, st);
-    case 384: return lwtb<12, 32, MARG, WB>(mode, a, st);
-    case 512: return lwtb<16, 32, MARG, WB>(mode, a, st);
-    case 1024: return lwtb<16, 64, MARG, WB>(mode, a, st);
+    case 256: return lwtb<8, 32, MARG, WB, wtb_m(8)>(mode, a, st);
+    case 320: return lwtb<10, 32, MARG, WB, wtb_m(10)>(mode, a, st);
+    case 384: return lwtb<12, 32, MARG, WB, wtb_m(12)>(mode, a, st);
+    case 512: return lwtb<16, 32, MARG, WB, wtb_m(16)>(mode, a, st);
+    case 1024: return lwtb<16, 64, MARG, WB, wtb_m(16)>(mode, a, st);
     default: return hipErrorInvalidValue;
   }
 }
