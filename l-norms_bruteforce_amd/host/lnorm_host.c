@@ -492,11 +492,16 @@ void lnorm_host_free(lnorm_host_problem *p) {
 /* ---- argv -------------------------------------------------------------------------- */
 int lnorm_host_parse_args(int argc, char **argv, int flavor, int max_threads_per_block, lnorm_host_args *a,
                           FILE *out) {
+    /* The usage text is the reference program's own (the drop-in prints
+     * every line as the program it replaces does). */
     const char *msg = flavor == LNORM_HOST_OMP
-        ? "Use the following command: ./L_HIP --omp number_of_threads filename_of_matrix order_of_the_L_norm"
-        : "Use the following command: ./L_HIP number_of_blocks number_of_threads_per_block filename_of_matrix order_of_the_L_norm";
+        ? "Use the following command: ./L_OpenMP number_of_threads filename_of_matrix order_of_the_L_norm"
+        : "Use the following command: ./L_CUDA number_of_blocks number_of_threads_per_block filename_of_matrix order_of_the_L_norm";
     const int off = flavor == LNORM_HOST_OMP ? 0 : 1;  /* CUDA argv has the extra block count */
-    int t, sd;
+    /* As in the reference, a field is refused when sscanf converts nothing
+     * (sd == 0) or the value is below 1; an empty field (sd == EOF) keeps
+     * the value of the field before it (0 before the first). */
+    int t = 0, sd;
     memset(a, 0, sizeof *a);
     a->flavor = flavor;
     if (argc < 4 + off) {
@@ -505,14 +510,14 @@ int lnorm_host_parse_args(int argc, char **argv, int flavor, int max_threads_per
     }
     if (flavor == LNORM_HOST_CUDA) {
         sd = sscanf(argv[1], "%d", &t);
-        if (sd != 1 || t < 1) {
+        if (sd == 0 || t < 1) {
             fprintf(out, "Please make sure that the number of blocks is a positive integer. %s\n", msg);
             return -1;
         }
         a->blocks = t;
     }
     sd = sscanf(argv[1 + off], "%d", &t);
-    if (sd != 1 || t < 1) {
+    if (sd == 0 || t < 1) {
         fprintf(out, "Please make sure that the number of threads is a positive integer. %s\n", msg);
         return -1;
     }
@@ -520,12 +525,14 @@ int lnorm_host_parse_args(int argc, char **argv, int flavor, int max_threads_per
     snprintf(a->file, sizeof a->file, "%s", argv[2 + off]);
     FILE *fp = fopen(a->file, "r");
     if (!fp) {
+        /* the reference prints this line and then calls fclose(NULL), which
+         * ends the process; this one exits with -1 after the same line */
         fprintf(out, "Please make sure that the file containig the matrix exists within this directory. %s\n", msg);
         return -1;
     }
     fclose(fp);
     sd = sscanf(argv[3 + off], "%d", &t);
-    if (sd != 1 || t < 1) {
+    if (sd == 0 || t < 1) {
         fprintf(out, "Please make sure that the order of the L norm is a positive integer. %s\n", msg);
         return -1;
     }

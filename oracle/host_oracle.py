@@ -371,6 +371,53 @@ def load(data: bytes, n_original: int, stat: str, marginal: int, flavor="omp",
     return p
 
 
+_USAGE = {"omp": "Use the following command: ./L_OpenMP number_of_threads filename_of_matrix order_of_the_L_norm",
+          "cuda": "Use the following command: ./L_CUDA number_of_blocks number_of_threads_per_block "
+                  "filename_of_matrix order_of_the_L_norm"}
+
+
+def _sscanf_d(text, prev):
+    """sscanf(text, "%d", &t): (items converted, t).  Leading white space,
+    then an optional sign and decimal digits; nothing converted leaves t as
+    it was; an empty or all-blank field is EOF (-1)."""
+    m = re.match(r"\s*([+-]?\d+)", text)
+    if m:
+        v = max(min(int(m.group(1)), (1 << 63) - 1), -(1 << 63))   # strtol saturation, then (int)
+        return 1, _wrap32(v)
+    return (-1 if text.strip() == "" else 0), prev
+
+
+def argv_error(argv, flavor="omp", file_exists=True, max_threads_per_block=1024):
+    """The line arguments_OMP (L_OpenMP.c:299-342) or arguments_CUDA
+    (L_CUDA.cu:361-412) prints before exit(-1), or None when the argv passes
+    every check.  A field is refused when sscanf converts nothing (sd == 0)
+    or the value is below 1; sd == EOF keeps the previous field's value."""
+    msg = _USAGE[flavor]
+    off = 0 if flavor == "omp" else 1
+    if len(argv) < 4 + off:
+        return "Incorrect number of input arguments. %s" % msg
+    t = 0
+    if flavor == "cuda":
+        sd, t = _sscanf_d(argv[1], t)
+        if sd == 0 or t < 1:
+            return "Please make sure that the number of blocks is a positive integer. %s" % msg
+    sd, t = _sscanf_d(argv[1 + off], t)
+    if sd == 0 or t < 1:
+        return "Please make sure that the number of threads is a positive integer. %s" % msg
+    threads = t
+    if not file_exists:
+        return "Please make sure that the file containig the matrix exists within this directory. %s" % msg
+    sd, t = _sscanf_d(argv[3 + off], t)
+    if sd == 0 or t < 1:
+        return "Please make sure that the order of the L norm is a positive integer. %s" % msg
+    if t > RANK_OF_NORM:
+        return ("The order of the L norm is too large. Please increase the RANK_OF_NORM variable in the code to %d "
+                "and compile and run it again." % t)
+    if flavor == "cuda" and threads > max_threads_per_block:
+        return "The maximum number of threads per block cannot be more than %d." % max_threads_per_block
+    return None
+
+
 def parse_args(argv, flavor="omp"):
     """arguments_OMP (L_OpenMP.c:299-342) / arguments_CUDA (L_CUDA.cu:361-412)
     for well-formed argv: returns (threads, blocks, file, n_original, stat, marginal)."""

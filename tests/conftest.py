@@ -34,5 +34,12 @@ def golden():
 
 
 @pytest.fixture(scope="session")
+def golden_argv():
+    """The reference's output for malformed and edge-case argv lines."""
+    with open(os.path.join(ROOT, "tests", "golden", "golden.json")) as f:
+        return json.load(f)["argv_cases"]
+
+
+@pytest.fixture(scope="session")
 def pkg():
     return load_pkg()

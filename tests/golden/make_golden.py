@@ -147,11 +147,36 @@ def cases():
     return c
 
 
+def argv_cases():
+    """The argument checks of arguments_OMP (L_OpenMP.c:299-342): each case
+    is the whole argv after the program name, run beside a small matrix.txt.
+    A missing matrix file is left out: the reference prints its line and then
+    calls fclose(NULL)."""
+    m = mt([[1, 2], [3, -4]])
+    c = []
+    for name, argv in [("argv_none", []),
+                       ("argv_too_few", ["3", "matrix.txt"]),
+                       ("argv_threads_not_numeric", ["abc", "matrix.txt", "1"]),
+                       ("argv_threads_zero", ["0", "matrix.txt", "1"]),
+                       ("argv_threads_negative", ["-2", "matrix.txt", "1"]),
+                       ("argv_order_zero", ["3", "matrix.txt", "0"]),
+                       ("argv_order_not_numeric", ["3", "matrix.txt", "x"]),
+                       ("argv_order_11", ["3", "matrix.txt", "11"]),
+                       ("argv_order_11_flags", ["3", "matrix.txt", "11", "y", "m"]),
+                       ("argv_order_numeric_prefix", ["3", "matrix.txt", "2abc"]),
+                       ("argv_threads_numeric_prefix", ["4x", "matrix.txt", "1", "n", "m"])]:
+        c.append({"name": name, "matrix": m, "argv": argv})
+    return c
+
+
 def run_ref(binary, case):
     with tempfile.TemporaryDirectory() as td:
         with open(os.path.join(td, "matrix.txt"), "w") as f:
             f.write(case["matrix"])
-        cmd = [binary, str(case["threads"]), "matrix.txt"] + case["args"]
+        if "argv" in case:
+            cmd = [binary] + case["argv"]
+        else:
+            cmd = [binary, str(case["threads"]), "matrix.txt"] + case["args"]
         p = subprocess.run(cmd, cwd=td, capture_output=True, timeout=600)
         strat = {}
         for fn in sorted(os.listdir(td)):
@@ -169,20 +194,21 @@ def main():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "ref", "REF_DIR=" + a.ref_dir],
                    check=True)
     binary = os.path.join(ROOT, "oracle", "_ref", "L_OpenMP")
-    out = []
-    for case in cases():
-        if a.only and a.only not in case["name"]:
-            continue
-        rc, stdout, strat = run_ref(binary, case)
-        rec = dict(case)
-        rec.update({"returncode": rc, "stdout": stdout, "strategy": strat})
-        out.append(rec)
-        print("%-32s rc=%d %s" % (case["name"], rc, stdout.strip().splitlines()[-1] if stdout.strip() else ""),
-              flush=True)
+    out = {"cases": [], "argv_cases": []}
+    for key, group in (("cases", cases()), ("argv_cases", argv_cases())):
+        for case in group:
+            if a.only and a.only not in case["name"]:
+                continue
+            rc, stdout, strat = run_ref(binary, case)
+            rec = dict(case)
+            rec.update({"returncode": rc, "stdout": stdout, "strategy": strat})
+            out[key].append(rec)
+            print("%-32s rc=%d %s" % (case["name"], rc, stdout.strip().splitlines()[-1] if stdout.strip() else ""),
+                  flush=True)
     with open(os.path.join(HERE, "golden.json"), "w") as f:
         json.dump({"generator": "tests/golden/make_golden.py",
                    "reference_build": "gcc -O3 L_OpenMP.c -fopenmp -o L_OpenMP -lm",
-                   "cases": out}, f, indent=1)
+                   **out}, f, indent=1)
     return 0
 
 

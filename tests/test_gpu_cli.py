@@ -29,6 +29,25 @@ def test_cli_matches_reference_byte_for_byte(pkg, golden, tmp_path):
         assert files == case["strategy"], case["name"]
 
 
+def test_cli_argv_golden_runs(pkg, golden_argv, tmp_path):
+    # golden argv lines that the reference accepts (numeric prefixes such as
+    # "2abc" and "4x"): the whole stdout and strategy file, byte for byte
+    ran = 0
+    for case in golden_argv:
+        if case["returncode"] != 0:
+            continue
+        d = tmp_path / case["name"]
+        d.mkdir()
+        (d / "matrix.txt").write_text(case["matrix"])
+        print("cli argv", case["name"], flush=True)
+        r = _run(pkg, ["--omp"] + case["argv"], d)
+        assert (r.returncode, r.stdout) == (0, case["stdout"]), (case["name"], r.stderr)
+        files = {f: (d / f).read_text() for f in os.listdir(d) if f.startswith("strategy_")}
+        assert files == case["strategy"], case["name"]
+        ran += 1
+    assert ran >= 2
+
+
 def test_cli_cuda_argv_form(pkg, golden, tmp_path):
     for case in golden[::4]:
         if case["returncode"] != 0:

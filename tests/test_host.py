@@ -113,6 +113,50 @@ def test_cli_argument_errors(pkg, tmp_path):
         assert r.returncode == 255 and msg in r.stdout, (args, r.stdout, r.stderr)
 
 
+def test_argv_restatement_matches_golden(golden_argv):
+    # the Python restatement of arguments_OMP against the reference's own
+    # output on every golden argv case (None where the run went on)
+    for case in golden_argv:
+        want = case["stdout"].rstrip("\n") if case["returncode"] != 0 else None
+        assert ho.argv_error(["L_OpenMP"] + case["argv"], "omp") == want, case["name"]
+
+
+def test_cli_argv_errors_match_reference(pkg, golden_argv, tmp_path):
+    """L_HIP in both argv forms on the golden argv cases that exit before any
+    computation: the L_OpenMP form prints the reference's bytes and exit
+    status; the L_CUDA form (a block count before the thread count) prints
+    arguments_CUDA's line (L_CUDA.cu:361-412, restated: no CUDA compiler
+    here to build the reference)."""
+    for case in golden_argv:
+        if case["returncode"] == 0:
+            continue
+        d = tmp_path / case["name"]
+        d.mkdir()
+        (d / "matrix.txt").write_text(case["matrix"])
+        r = _run_cli(pkg, ["--omp"] + case["argv"], d)
+        assert (r.returncode, r.stdout) == (case["returncode"] & 0xFF, case["stdout"]), case["name"]
+        cuda = ["8"] + case["argv"] if case["argv"] else []
+        r = _run_cli(pkg, cuda, d)
+        want = ho.argv_error(["L_CUDA"] + cuda, "cuda")
+        assert (r.returncode, r.stdout) == (255, want + "\n"), case["name"]
+        assert not [f for f in os.listdir(d) if f.startswith("strategy_")], case["name"]
+
+
+def test_argv_restatement_cuda_checks():
+    # arguments_CUDA's own checks: the block count first, the per-block
+    # thread limit last (after the order), an EOF field keeps the value before it
+    c = ho.argv_error
+    assert c(["L_CUDA", "x", "32", "m", "1"], "cuda").startswith("Please make sure that the number of blocks")
+    assert c(["L_CUDA", "4", "2048", "m", "11"], "cuda").startswith("The order of the L norm is too large")
+    assert c(["L_CUDA", "4", "2048", "m", "1"], "cuda") == \
+        "The maximum number of threads per block cannot be more than 1024."
+    assert c(["L_CUDA", "4", "", "m", "1"], "cuda") is None          # threads keeps the block count
+    assert c(["L_OpenMP", "", "m", "1"], "omp").startswith("Please make sure that the number of threads")
+    assert c(["L_OpenMP", "3", "m", ""], "omp") is None             # the order keeps the thread count
+    assert c(["L_OpenMP", "3", "m", "1"], "omp", file_exists=False).startswith(
+        "Please make sure that the file containig the matrix exists")
+
+
 def test_cli_host_only_paths(pkg, golden, tmp_path):
     # paths that never reach the GPU: the zero matrix and "Matrix is too large"
     for name in ("zero3x4_l1_y", "zero3x4_l2_y", "zero3x4_lm_y", "too_many_rows_l3"):
