@@ -29,18 +29,22 @@ PKG_DIR = os.path.join(ROOT, "l-norms_bruteforce_amd")
 
 METRIC = "exact L_1 sign-vectors/s on 1 MI355X (int8-MFMA vs plain Gray), bit-exact"
 VALU_PEAK = 256 * 4 * 16 * 2.4e9        # int32 lane-ops/s: 256 CUs x 4 SIMD16 x 2.4 GHz
+# packed 16-bit |.|-accumulates: one v_sad_u16 lane-op retires two row-updates
+# (two biased 16-bit halves), so the row-update roof of the 16-bit paths is twice the int32 one
+VALU16_PEAK = 2 * VALU_PEAK
 MFMA_I8_PEAK = 5.0e15                   # dense int8 ops/s
-# HBM bytes per launch of the headline instantiations, measured with
-# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE on `bench.py --case blocked`
-# (with and without --no-autotune), each scaled by the counter's reading for a
-# 1 GiB device-to-device copy (x2.0 for FETCH_SIZE, x1.0 for WRITE_SIZE):
-#   P=2, s=11 (autotuned): 130.1 KiB and 81920 KiB per launch of 2^21 waves
-#                          (one launch per call);
-#   P=4, s=12 (default):   207.6 KiB and 20480 KiB per launch of 2^19 waves.
-# Only the per-wave maxima reach HBM; the matrix and the pattern table stay
-# on chip.
-PROFILED_TRAFFIC = {"blocked16<np=32,P=2,marg=0,s=11>": 266445 + 83886080,
-                    "blocked16<np=32,P=4,marg=0,s=12>": 425216 + 20971520}
+# HBM bytes per launch of the headline instantiation, measured with
+# rocprofv3 --pmc FETCH_SIZE and --pmc WRITE_SIZE on `bench.py --case blocked`,
+# each scaled by the counter's reading for a 1 GiB device-to-device copy (x2.0
+# for FETCH_SIZE, x1.0 for WRITE_SIZE).  Each launch leaves one 64-bit key
+# (its maximum and first wave); the matrix and the pattern table stay on chip.
+#   P=2, s=11 (autotuned, one launch of 2^21 waves per call): FETCH_SIZE
+#   137.1 KiB (x2.0), WRITE_SIZE 1.3 KiB (x1.0)
+PROFILED_TRAFFIC = {"blocked16<np=32,P=2,marg=0,s=11>": 280832 + 1344}
+# VALU issue rate of the headline instantiation from the counters:
+# SQ_INSTS_VALU x 64 / dispatch time / 39.32e12 (rocprofv3 --pmc on
+# `bench.py --case blocked`: 2.7946e11 instructions in 466.0 ms)
+PROFILED_ISSUE = {"blocked16<np=32,P=2,marg=0,s=11>": 0.976}
 
 
 def load_pkg():
@@ -119,6 +123,19 @@ def run_steps(pkg, A, kernel, steps, warmup, P=0, n=1):
             "kernel_ms": kms, "res": res, "tuned": tuned}
 
 
+def alg_frac(kernel: str, row_updates_per_s: float) -> float:
+    """Row-update rate over its roof by the algorithmic rule: 1 op per
+    row-update on the blocked and tabulated paths (the update itself is
+    tabulated or on the scalar unit), 2 on the plain walk (add and |.|), each
+    against the VALU rate of its word size: 16-bit halves (v_sad_u16, two
+    row-updates per lane-op) or 32-bit words."""
+    if kernel in ("blocked", "table16", "table16w"):
+        return row_updates_per_s / VALU16_PEAK
+    if kernel == "plain":
+        return 2 * row_updates_per_s / VALU_PEAK
+    return row_updates_per_s / VALU_PEAK
+
+
 def summarize(r, total, ny, kernel, marginal=0):
     steps = len(r["step_s"])
     kms = sum(r["kernel_ms"]) / steps
@@ -126,6 +143,7 @@ def summarize(r, total, ny, kernel, marginal=0):
     ops = valu_ops_per_unit(kernel, ny, marginal, P) * total
     achieved = ops / (kms / 1e3)
     return {
+        "alg_frac": alg_frac(kernel, total * ny / (kms / 1e3)),
         "value": total * steps / r["wall_s"],
         "ms_per_step": 1e3 * r["wall_s"] / steps,
         "kernel_ms": kms,
@@ -169,8 +187,10 @@ def cpu_model() -> str:
 
 def cpu_reference(gen, M, d: int, units: int, extra=(), runs: int = 3) -> dict:
     """The reference L_OpenMP (built from its sources into oracle/_ref) on the
-    host cores: `runs` runs of the file M with argv tail `d [extra]`, the
-    median wall time (the time_order_1.sh method: whole process).  Falls back
+    host cores, threads bound one per core (OMP_PLACES=cores,
+    OMP_PROC_BIND=spread): `runs` runs of the file M with argv tail
+    `d [extra]`, the median wall time (the time_order_1.sh method: whole
+    process).  Falls back
     to the oracle's restatement when the reference binary is absent."""
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1
     ref = os.path.join(ROOT, "oracle", "_ref", "L_OpenMP")
@@ -180,8 +200,9 @@ def cpu_reference(gen, M, d: int, units: int, extra=(), runs: int = 3) -> dict:
             gen.write_matrix(os.path.join(td, "m.txt"), M)
             for _ in range(runs):
                 t = time.perf_counter()
+                env = dict(os.environ, OMP_PLACES="cores", OMP_PROC_BIND="spread")
                 p = subprocess.run([ref, str(threads), "m.txt", str(d), *extra], cwd=td, capture_output=True,
-                                   text=True, timeout=600)
+                                   text=True, timeout=600, env=env)
                 times.append(time.perf_counter() - t)
                 if p.returncode != 0:
                     raise RuntimeError("reference L_OpenMP failed: %s" % p.stderr)
@@ -395,6 +416,85 @@ def batch_line(pkg, gen, steps: int) -> dict:
             "graph_replayed": st["graph_replayed"], "autotuned": tuned}
 
 
+def batch_full_line(pkg, gen, chunk_labelings: float = 1.2e14) -> dict:
+    """The mixed batch workload as drawn, without the 20-row cap on d = 3
+    files (m up to 30 rows: 3^29 labelings for the largest item, about
+    3.5e15 in all).  Every item goes through lnorm_hip_calc_batch_ex: first
+    one call with the items the persistent launches take (up to 2^32
+    labelings each), then the larger items, which the library runs one by
+    one on AUTO's single-problem kernels, in calls of at most
+    `chunk_labelings` labelings each, with a progress line after each call.
+    Every result is checked on the host: the objective of the returned
+    strategy (oracle.objective, one evaluation) equals the returned norm."""
+    sys.path.insert(0, ROOT)
+    import oracle
+    from lnorm_amd import host
+    work = gen.batch_workload(4096, d3_max_rows=30)
+    probs, totals = [], []
+    for w in work:
+        p, _, exited = host.load(w["text"], w["d"], w["stat"], w["marginal"])
+        if exited:
+            raise RuntimeError("batch item %d rejected by the host pipeline" % w["seed"])
+        probs.append((p.matrix_laid_out(), p.n, p.marginal))
+        totals.append(p.total)
+        host.free(p)
+    small = [i for i, t in enumerate(totals) if t <= 2 ** 32]
+    large = sorted((i for i, t in enumerate(totals) if t > 2 ** 32), key=lambda i: totals[i])
+    calls = [small]
+    cur, acc = [], 0
+    for i in large:
+        if cur and acc + totals[i] > chunk_labelings:
+            calls.append(cur)
+            cur, acc = [], 0
+        cur.append(i)
+        acc += totals[i]
+    if cur:
+        calls.append(cur)
+    results = [None] * len(probs)
+    by_d = {}
+    wall_small = wall_large = 0.0
+    st_small = None
+    done = 0
+    for k, idx in enumerate(calls):
+        t = time.perf_counter()
+        # the persistent call uses the batch autotuner; the larger items run
+        # on AUTO's round-aware default launch shapes
+        out, st = pkg.calc_batch([probs[i] for i in idx], stats=True, autotune=1 if (AUTOTUNE and k == 0) else 0)
+        dt = time.perf_counter() - t
+        if k == 0:
+            wall_small, st_small = dt, st
+        else:
+            wall_large += dt
+        for i, r in zip(idx, out):
+            if r[2] != 0:
+                raise RuntimeError("batch item %d failed (status %d)" % (i, r[2]))
+            results[i] = r
+            A, n, m = probs[i]
+            if A.shape[0] > 1 and oracle.objective(A, n, m, r[1]) != r[0]:
+                raise RuntimeError("item %d: the strategy's objective differs from the norm" % i)
+            by_d.setdefault(n, [0, 0])
+            by_d[n][0] += totals[i]
+            by_d[n][1] += 1
+        done += sum(totals[i] for i in idx)
+        print("batch_full: call %d/%d, %d items, %.3g labelings in %.1f s (%.3g of %.3g done)"
+              % (k + 1, len(calls), len(idx), sum(totals[i] for i in idx), dt, done, sum(totals)),
+              file=sys.stderr, flush=True)
+    lab_small = sum(totals[i] for i in small)
+    lab_large = sum(totals[i] for i in large)
+    wall = wall_small + wall_large
+    return {"items": len(probs), "labelings": sum(totals),
+            "labelings_by_d": {str(k): v[0] for k, v in sorted(by_d.items())},
+            "items_by_d": {str(k): v[1] for k, v in sorted(by_d.items())},
+            "value": sum(totals) / wall, "unit": "labelings/s", "wall_s": wall,
+            "persistent_call": {"items": len(small), "labelings": lab_small, "wall_s": wall_small,
+                                "ms_device": st_small["ms_device"], "groups": st_small["groups"],
+                                "labelings_per_s": lab_small / wall_small},
+            "large_items": {"items": len(large), "labelings": lab_large, "calls": len(calls) - 1,
+                            "wall_s": wall_large, "labelings_per_s": lab_large / max(wall_large, 1e-9),
+                            "rows": sorted({int(probs[i][0].shape[0]) for i in large})},
+            "checked": "objective(strategy) == norm for every item"}
+
+
 def tune_sweep(pkg, gen) -> dict:
     """Every autotuner candidate on the bench workloads: labelings/s of each
     probe and the shape the tuner keeps (lnorm_hip_autotune and
@@ -492,11 +592,19 @@ def run_case(pkg, gen, case: str, steps: int, warmup: int) -> int:
     if case == "batch":
         print(json.dumps({"case": case, **batch_line(pkg, gen, steps)}), flush=True)
         return 0
+    if case == "batch_full":
+        print(json.dumps({"case": case, **batch_full_line(pkg, gen)}), flush=True)
+        return 0
     if case == "l3":
         print(json.dumps({"case": case, **l3_line(pkg, gen, steps, warmup, False)}), flush=True)
         return 0
     if case == "wide":
         print(json.dumps({"case": case, **wide_lines(pkg, gen)}), flush=True)
+        return 0
+    if case == "int64":
+        # L1 of a 34 x 40 matrix scaled by 2^28 + 3 (beyond int32): the 64-bit walk
+        A64 = gen.gen_matrix(34, 40, 100, 108).astype("int64") * (2 ** 28 + 3)
+        print(json.dumps({"case": case, **single_line(pkg, A64, 1, 2 ** 33, None, int64=True)}), flush=True)
         return 0
     A = gen.gen_matrix(64, 40, 127, 104).T.copy()
     k, ny = A.shape
@@ -516,7 +624,7 @@ def main():
                     help="run the default launch shapes instead of the autotuner's (tuned before timing)")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the autotuner's candidates on the bench workloads and print them as JSON")
-    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "l3", "wide", "int16", "int16_blocked",
+    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "batch_full", "l3", "int64", "wide", "int16", "int16_blocked",
                                        "int16_plain", "wide_int16", "wide_int16_wide", "l3_int16",
                                        "l3_int16_general"],
                     help="run only this workload (for profiler passes) and print its figures")
@@ -578,13 +686,20 @@ def main():
         },
         "roofline": {
             "bound": "valu",
-            "achieved": sb["valu_TOPs"],
-            "peak": VALU_PEAK / 1e12,
+            "achieved": sb["row_updates_per_s"] / 1e12,
+            "peak": VALU16_PEAK / 1e12,
             "unit": "TOP/s",
-            "frac": sb["valu_frac"],
+            "frac": sb["row_updates_per_s"] / VALU16_PEAK,
+            "work": "algorithmic: 1 op per row-update (2^39 sign vectors x 64 partial sums per launch: the "
+                    "MFMA path's VALU rule) over the kernel's HIP-event time; peak: the packed 16-bit VALU rate, "
+                    "2 row-updates per lane-op (v_sad_u16 on two biased halves), 2 x 39.32e12",
+            "issue_frac": sb["valu_frac"],
+            "issue_frac_basis": "the kernel's VALU lane-ops by its op model (%.1f per sign vector, equal to "
+                                "SQ_INSTS_VALU x 64 per sign vector in the counter profile) over the int32 "
+                                "lane-op peak 39.32e12" % sb["valu_ops_per_sign_vector"],
+            "issue_frac_counters": PROFILED_ISSUE.get(sb["kernel_name"]),
             "traffic": PROFILED_TRAFFIC.get(sb["kernel_name"]),
             "traffic_per": "launch (2^22 waves at most per launch; one per call at the tuned shape)",
-            "ops_per_sign_vector": sb["valu_ops_per_sign_vector"],
             "kernel_ms": sb["kernel_ms"],
         },
         "row_updates_per_s": sb["row_updates_per_s"],
@@ -598,7 +713,7 @@ def main():
             raise RuntimeError("plain and blocked kernels disagree")
         sp = summarize(pl, total, ny, "plain")
         line["plain_gray"] = {k2: sp[k2] for k2 in ("value", "ms_per_step", "kernel_ms", "row_updates_per_s",
-                                                    "valu_TOPs", "valu_frac", "kernel_name")}
+                                                    "valu_TOPs", "valu_frac", "alg_frac", "kernel_name")}
         line["speedup_vs_plain"] = sb["value"] / sp["value"]
         line["kernel_speedup_vs_plain"] = sp["kernel_ms"] / sb["kernel_ms"]
         tb = run_steps(pkg, A, "table16", a.steps, a.warmup)
@@ -606,7 +721,7 @@ def main():
             raise RuntimeError("tabulated and blocked kernels disagree")
         st16 = summarize(tb, total, ny, "table16")
         line["table16_same_matrix"] = {k2: st16[k2] for k2 in ("value", "ms_per_step", "kernel_ms", "valu_TOPs",
-                                                              "valu_frac", "kernel_name")}
+                                                              "valu_frac", "alg_frac", "kernel_name")}
         B = gen.gen_matrix(40, 40, 100, 106)
         nb = run_steps(pkg, B, "blocked", a.steps, a.warmup)
         npl = run_steps(pkg, B, "plain", max(1, a.steps // 2), a.warmup)
@@ -626,13 +741,18 @@ def main():
         # column sums beyond 2^15, so the blocked path uses 32-bit words
         # (hi/lo int8 MFMA planes for the table, one v_sad_u32 per column)
         C16 = gen.gen_matrix(128, 40, 30000, 105).T.copy()
+        a16 = run_steps(pkg, C16, "auto", 2, 1)
         b16 = run_steps(pkg, C16, "blocked", 1, 0)
         p16 = run_steps(pkg, C16, "plain", 1, 0)
-        if (b16["norm"], b16["best_rank"]) != (p16["norm"], p16["best_rank"]):
-            raise RuntimeError("plain and blocked kernels disagree on the int16 matrix")
+        if not ((a16["norm"], a16["best_rank"]) == (b16["norm"], b16["best_rank"]) == (p16["norm"], p16["best_rank"])):
+            raise RuntimeError("the kernels disagree on the int16 matrix")
+        s6 = summarize(a16, 2 ** 39, 128, "table32")
         s3 = summarize(b16, 2 ** 39, 128, "blocked32")
         s4 = summarize(p16, 2 ** 39, 128, "plain")
-        line["int16_128x40"] = {"blocked": s3["value"], "plain": s4["value"], "speedup": s3["value"] / s4["value"],
+        line["int16_128x40"] = {"auto": s6["value"], "auto_kernel": s6["kernel_name"],
+                                "auto_speedup": s6["value"] / s4["value"],
+                                "auto_valu_frac": s6["valu_frac"], "auto_alg_frac": s6["alg_frac"],
+                                "blocked": s3["value"], "plain": s4["value"], "speedup": s3["value"] / s4["value"],
                                 "blocked_valu_frac": s3["valu_frac"], "plain_valu_frac": s4["valu_frac"],
                                 "kernel": s3["kernel_name"], "norm": s3["norm"]}
         line["cli_end_to_end"] = cli_line(pkg, gen)

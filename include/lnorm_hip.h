@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define LNORM_HIP_ABI_VERSION 5
+#define LNORM_HIP_ABI_VERSION 6
 
 /* error codes */
 #define LNORM_OK 0
@@ -36,7 +36,7 @@ extern "C" {
 /* kernel selection */
 #define LNORM_KERNEL_AUTO 0     /* n = 1: blocked from 2^33 sign vectors at up to 40 |.| columns, else
                                    table16; n >= 2: table16; then table32, blocked (32-bit words), packed
-                                   plain16, plain, wide, general */
+                                   plain16, plain, wide, columns, general */
 #define LNORM_KERNEL_PLAIN 1    /* register-resident Gray walk, lane per segment */
 #define LNORM_KERNEL_BLOCKED 2  /* L1/LM: MFMA-built low-row table + scalar high walk (16- or 32-bit words) */
 #define LNORM_KERNEL_GENERAL 3  /* any width: wave per segment, columns over lanes */
@@ -53,6 +53,11 @@ extern "C" {
                                    n = 2..10 up to 96 (d = 2) .. 16 (d = 10) columns in one wave, 256
                                    (d <= 6) .. 64 (d = 10) over W waves.  AUTO takes it where the 16-bit
                                    walks do not apply (column sums beyond 32767) */
+#define LNORM_KERNEL_COLUMNS 8  /* the column walk: one wave per segment, the columns over the lanes, label sums
+                                   in registers, the rows in LDS; any int32 input up to 4096 columns for n <= 2,
+                                   2048 at n = 3, 4 ... 768 at n = 9, 10, while the rows fit one workgroup's LDS
+                                   (R x 64 x ceil(C / 64) x 4 B <= 160 KiB).  AUTO takes it where no other walk
+                                   but the general kernel applies */
 
 typedef struct lnorm_hip_opts {
     int32_t kernel;             /* LNORM_KERNEL_* */
@@ -69,12 +74,22 @@ typedef struct lnorm_hip_opts {
     int32_t autotune;           /* nonzero: use launch shapes chosen by lnorm_hip_autotune (cached) */
     int32_t batch_streams;      /* batches: streams the kernel groups spread over, 1..4; 0 = auto */
     int32_t batch_cap_log;      /* batches: at most ~2^cap ranks per lane segment, 8..20; 0 = auto */
+    /* Single problems: enumerate only the Gray ranks [rank_begin, rank_end): one shard of the rank space,
+       as calc_jmin_jmax hands a worker its words (L_OpenMP.c:6-13).  The result is that range's maximum
+       and its smallest maximising rank.  rank_end = 0 means total_num_to_calc, the reference's count
+       (L_OpenMP.c:352-367): 2^(k-1) for n <= 2, (unsigned long long) pow(n, k-1) beyond, which is inexact
+       from 2^53 on; the engine enumerates [0, min(that count, n^(k-1))), which gives the reference's
+       result either way (ranks from n^(k-1) on repeat earlier labelings and never win its strict
+       comparison).  Batches refuse nonzero values. */
+    uint64_t rank_begin;
+    uint64_t rank_end;
 } lnorm_hip_opts;
 
 typedef struct lnorm_hip_result {
     int32_t Lnorm;              /* second->Lnorm */
     int32_t kernel;             /* LNORM_KERNEL_* actually used */
-    uint64_t total;             /* labelings enumerated: total_num_to_calc (L_OpenMP.c:352-367) */
+    uint64_t total;             /* ranks enumerated: rank_end - rank_begin (by default total_num_to_calc,
+                                   L_OpenMP.c:352-367) */
     uint64_t best_rank;         /* smallest Gray rank attaining Lnorm (if the strategy was recovered) */
     double ms_enumerate;        /* HIP-event time of the enumeration phase (table, walk, reduction) */
     double ms_kernel;           /* HIP-event time of the enumeration kernel launches alone */

@@ -23,8 +23,9 @@ HOST_LIB = os.path.join(LIB_DIR, "liblnorm_host.so")
 CLI = os.path.join(PKG_DIR, "bin", "L_HIP")
 
 (KERNEL_AUTO, KERNEL_PLAIN, KERNEL_BLOCKED, KERNEL_GENERAL, KERNEL_PLAIN16, KERNEL_WIDE, KERNEL_TABLE16,
- KERNEL_TABLE32) = range(8)
-KERNELS = {"auto": 0, "plain": 1, "blocked": 2, "general": 3, "plain16": 4, "wide": 5, "table16": 6, "table32": 7}
+ KERNEL_TABLE32, KERNEL_COLUMNS) = range(9)
+KERNELS = {"auto": 0, "plain": 1, "blocked": 2, "general": 3, "plain16": 4, "wide": 5, "table16": 6, "table32": 7,
+           "columns": 8}
 ERRORS = {0: "OK", -1: "E_ARG", -2: "E_BOUND", -3: "E_SIZE", -4: "E_HIP", -5: "E_UNSUPPORTED"}
 
 
@@ -34,7 +35,8 @@ class Opts(ctypes.Structure):
                 ("wave_log2", ctypes.c_int32), ("block", ctypes.c_int32),
                 ("device", ctypes.c_int32), ("word_bits", ctypes.c_int32),
                 ("stream", ctypes.c_void_p), ("batch_graph", ctypes.c_int32), ("autotune", ctypes.c_int32),
-                ("batch_streams", ctypes.c_int32), ("batch_cap_log", ctypes.c_int32)]
+                ("batch_streams", ctypes.c_int32), ("batch_cap_log", ctypes.c_int32),
+                ("rank_begin", ctypes.c_uint64), ("rank_end", ctypes.c_uint64)]
 
 
 class Result(ctypes.Structure):

@@ -30,7 +30,7 @@ HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
 HIP_SOURCES = ["lnorm_plain.hip", "lnorm_blocked.hip", "lnorm_general.hip", "lnorm_batch.hip", "lnorm_batch_tab.hip",
                "lnorm_batch_tab_hi.hip", "lnorm_int64.hip", "lnorm_wide.hip", "lnorm_dary.hip", "lnorm_dary_hi.hip",
                "lnorm_tab_bin.hip", "lnorm_wide_tab.hip", "lnorm_dary32.hip", "lnorm_dary32_hi.hip",
-               "lnorm_tab32_bin.hip", "lnorm_wide_tab32.hip", "lnorm_wide_tb32.hip", "lnorm_api.hip"]
+               "lnorm_tab32_bin.hip", "lnorm_wide_tab32.hip", "lnorm_wide_tb32.hip", "lnorm_colwalk.hip", "lnorm_api.hip"]
 # -simplifycfg-sink-common=false: the d-ary walks branch (uniformly) into one
 # move per (from, to) label pair; sinking those near-identical bodies into a
 # shared tail would index the label-vector array at run time and push it into
@@ -56,12 +56,25 @@ def _run(cmd):
     return p
 
 
+def _includes(path: str, seen: set) -> set:
+    """path and every file it includes with #include "...", transitively."""
+    path = os.path.normpath(path)
+    if path in seen or not os.path.exists(path):
+        return seen
+    seen.add(path)
+    with open(path) as f:
+        for line in f:
+            s = line.strip()
+            if s.startswith("#include") and '"' in s:
+                _includes(os.path.join(os.path.dirname(path), s.split('"')[1]), seen)
+    return seen
+
+
 def _stale(obj: str, src: str) -> bool:
-    """True when obj is missing or older than src or any engine header."""
+    """True when obj is missing or older than src or a header it includes."""
     if not os.path.exists(obj):
         return True
-    deps = [src] + [os.path.join(CSRC, h) for h in os.listdir(CSRC) if h.endswith(".h")]
-    deps.append(os.path.join(INC, "lnorm_hip.h"))
+    deps = _includes(src, set())
     return os.path.getmtime(obj) < max(os.path.getmtime(d) for d in deps)
 
 

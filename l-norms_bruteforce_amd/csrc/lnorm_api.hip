@@ -32,6 +32,7 @@
 #include <vector>
 
 #include "../../include/lnorm_hip.h"
+#include "lnorm_colwalk.h"
 #include "lnorm_internal.h"
 
 using namespace lnorm;
@@ -144,7 +145,8 @@ int ctx_init(DevCtx &c) {
 //   unit maxima     4 MiB   2^20 units per launch (launches are chunked)
 //   results         4 KiB   the maximum and the first rank/unit
 //   matrix          1 MiB   R x C int32 (general kernel: 63 x 4096 at most)
-//   deltas/tables 512 KiB   [2][R-1][1024] deltas of the widest binary walk
+//   deltas/tables   1 MiB   [2][R-1][1024] deltas of the widest binary walk, or
+//                   + 4 KiB the column walk's 2 A rows (63 x 4096) and column 0
 //   blocked       468 KiB   int8 planes, low-pattern table, seed, high deltas
 //   general state   rest    per-wave label vectors: as many concurrent waves
 //                           as whole slices fit (at most kGeneralSlices)
@@ -172,7 +174,7 @@ int noalloc_init(DevCtx &c) {
   carve(c.units, kNoallocUnits * 4);
   carve(c.scal, 4096);
   carve(c.A, (size_t)1 << 20);
-  carve(c.D, (size_t)512 << 10);
+  carve(c.D, ((size_t)1 << 20) + (4u << 10));
   carve(c.lo, (size_t)16 << 10);
   carve(c.table, (size_t)256 << 10);
   carve(c.seed, (size_t)4 << 10);
@@ -191,7 +193,8 @@ struct Problem {
   int R, C, n, marginal;
   Kind kind;
   int base;          // 2 for d <= 2, else d
-  uint64_t total;    // total_num_to_calc
+  uint64_t total;    // ranks the reference enumerates: min(total_num_to_calc, full)
+  uint64_t full = 0; // base^(R-1): every labeling
 };
 
 uint64_t ipow(uint64_t b, int e) {
@@ -203,6 +206,31 @@ uint64_t ipow(uint64_t b, int e) {
 int max_rows(int n) {  // calc_Parameters, L_OpenMP.c:354-366
   if (n <= 2) return 63;
   return (int)(std::floor(64.0 / std::log2((double)n)) + 1);
+}
+
+// The ranks the reference enumerates.  total_num_to_calc (L_OpenMP.c:352-367)
+// is 2^(k-1) for n <= 2 and (unsigned long long) pow(n, k-1) beyond, which is
+// inexact from 2^53 on: a count below n^(k-1) leaves the highest ranks out;
+// one above it walks on into ranks whose digits below k-1 repeat labelings
+// already seen (the changed digit is searched below k-1 only), which never
+// pass the strict comparison.  Either way the reference's result is that of
+// the ranks [0, min(count, n^(k-1))).  n^(k-1) itself must fit 64 bits: only
+// n = 4 at 33 rows reaches 2^64, where the reference's count wraps to 0.
+int rank_count(Problem &p) {
+  if (p.n <= 2) {
+    p.total = p.full = 1ull << (p.R - 1);
+    return 0;
+  }
+  unsigned __int128 f = 1;
+  for (int i = 0; i < p.R - 1; ++i) f *= (unsigned)p.n;
+  if (f > (unsigned __int128)UINT64_MAX)
+    return fail(LNORM_E_SIZE, "%d^%d labelings do not fit 64 bits (the reference's count wraps to 0)", p.n, p.R - 1);
+  p.full = (uint64_t)f;
+  const double dt = std::pow((double)p.n, (double)(p.R - 1));
+  const uint64_t count = dt < 18446744073709551616.0 ? (uint64_t)dt : 0;
+  if (count == 0) return fail(LNORM_E_SIZE, "the reference's count of %d^%d labelings is 0", p.n, p.R - 1);
+  p.total = std::min(count, p.full);
+  return 0;
 }
 
 int validate(Problem &p) {
@@ -221,17 +249,49 @@ int validate(Problem &p) {
                 (long long)s);
   p.kind = p.n == 1 ? (p.marginal ? KIND_LM : KIND_L1) : (p.n == 2 ? KIND_L2 : KIND_LD);
   p.base = p.n <= 2 ? 2 : p.n;
-  if (p.n <= 2) {
-    p.total = 1ull << (p.R - 1);
-  } else {
-    // the reference counts with pow() (L_OpenMP.c:361,365); exact below 2^53
-    const double dt = std::pow((double)p.n, (double)(p.R - 1));
-    if (dt >= 9007199254740992.0)
-      return fail(LNORM_E_SIZE, "%d^%d labelings exceed 2^53, where the reference's pow() count is inexact",
-                  p.n, p.R - 1);
-    p.total = ipow((uint64_t)p.n, p.R - 1);
-  }
+  return rank_count(p);
+}
+
+// The ranks [rb, re) a single-problem call enumerates (opts->rank_begin,
+// rank_end; rank_end = 0: the reference's count).
+int rank_range(const Problem &p, const lnorm_hip_opts &o, uint64_t &rb, uint64_t &re) {
+  rb = o.rank_begin;
+  re = o.rank_end ? o.rank_end : p.total;
+  if (rb >= re || re > p.full)
+    return fail(LNORM_E_ARG, "rank range [%llu, %llu) is empty or beyond the %llu ranks", (unsigned long long)rb,
+                (unsigned long long)re, (unsigned long long)p.full);
   return 0;
+}
+
+// A piece of a rank range: the units [u0, u0 + n) of base^e ranks each.
+struct RankRun {
+  int e;
+  uint64_t u0, n;
+};
+
+// Splits [rb, re) into runs of aligned units in rank order: at each rank the
+// largest unit base^j (j <= e_main) that starts there and ends within the
+// range, consecutive units of one size merged into one run.  The whole
+// units of base^e_main form one run; below them at most base - 1 units of
+// each smaller size sit on either side of it.
+std::vector<RankRun> split_ranks(int base, int e_main, uint64_t rb, uint64_t re) {
+  std::vector<RankRun> runs;
+  uint64_t a = rb;
+  while (a < re) {
+    int j = 0;
+    uint64_t L = 1;
+    while (j < e_main && (a / L) % (uint64_t)base == 0 && L * (uint64_t)base <= re - a) {
+      L *= (uint64_t)base;
+      ++j;
+    }
+    const uint64_t n = j == e_main ? (re - a) / L : 1;
+    if (!runs.empty() && runs.back().e == j && (runs.back().u0 + runs.back().n) * L == a)
+      runs.back().n += n;
+    else
+      runs.push_back(RankRun{j, a / L, n});
+    a += n * L;
+  }
+  return runs;
 }
 
 void labels_of_rank(uint64_t N, int R, int n, int32_t *out) {
@@ -256,7 +316,7 @@ struct Geometry {
 
 // The lane- or wave-per-segment walks.
 enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2, WALK_WIDE = 3, WALK_WIDE16 = 4, WALK_TAB16 = 5,
-            WALK_TAB16W = 6, WALK_TAB32 = 7, WALK_TAB32W = 8 };
+            WALK_TAB16W = 6, WALK_TAB32 = 7, WALK_TAB32W = 8, WALK_COLS = 9 };
 
 // The tabulated walks in 32-bit words: one column per word (ny_pad = words).
 bool is_tab32(Walk w) { return w == WALK_TAB32 || w == WALK_TAB32W; }
@@ -269,8 +329,9 @@ Geometry geometry_for(int base, int e, int target_log, int forced, Walk walk, in
   const double lb = std::log2((double)base);
   int qmax;
   if (walk == WALK_GENERAL) qmax = (int)(62.0 / lb);
-  else if ((walk == WALK_PLAIN || walk == WALK_WIDE) && base == 2) qmax = 30;
+  else if ((walk == WALK_PLAIN || walk == WALK_WIDE || walk == WALK_COLS) && base == 2) qmax = 30;
   else qmax = std::min(15, (int)(31.0 / lb));
+  if (walk == WALK_COLS) target_log = std::min(target_log, 16);   // one wave (not lane) per segment
   int q = forced > 0 ? forced : std::max(0, e - (int)std::floor(target_log / lb));
   q = std::min(q, std::min(qmax, e));
   if (walk == WALK_WIDE) q = std::max(q, std::min(3, e));   // steps combine in groups of 8
@@ -445,7 +506,23 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
     }
     return;
   }
-  const int W = (walk == WALK_PLAIN || walk == WALK_WIDE) ? ny_pad : p.C;
+  if (walk == WALK_COLS && p.n == 1) {
+    // the column walk, L1 / LM: A = R x Cp rows of the |.| columns, D = R x Cp
+    // rows of 2 A, then the R entries of column 0 (LM's marginal)
+    const int marg = p.kind == KIND_LM ? 1 : 0;
+    pp.A.assign((size_t)p.R * ny_pad, 0);
+    pp.D.assign((size_t)p.R * ny_pad + p.R, 0);
+    for (int i = 0; i < p.R; ++i) {
+      for (int l = marg; l < p.C; ++l) {
+        const int32_t a = p.A[(size_t)i * p.C + l];
+        pp.A[(size_t)i * ny_pad + l - marg] = a;
+        pp.D[(size_t)i * ny_pad + l - marg] = (int32_t)(2u * (uint32_t)a);
+      }
+      pp.D[(size_t)p.R * ny_pad + i] = marg ? p.A[(size_t)i * p.C] : 0;
+    }
+    return;
+  }
+  const int W = (walk == WALK_PLAIN || walk == WALK_WIDE || walk == WALK_COLS) ? ny_pad : p.C;
   pp.A.assign((size_t)p.R * W, 0);
   for (int i = 0; i < p.R; ++i)
     for (int l = 0; l < p.C; ++l) pp.A[(size_t)i * W + l] = p.A[(size_t)i * p.C + l];
@@ -462,6 +539,14 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
   }
 }
 
+// The shortest segment exponent a walk runs: its table rows, or the W-wave
+// walks' shortest segments (base^3 ranks binary, base^2 packed).
+int walk_qmin(const Prepared &pp) {
+  if (pp.walk == WALK_WIDE) return 3;
+  if (pp.walk == WALK_WIDE16) return 2;
+  return pp.qmin;
+}
+
 hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const PlainArgs &a, hipStream_t st,
                        int block) {
   switch (pp.walk) {
@@ -473,6 +558,7 @@ hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const Pl
     case WALK_TAB16W: return p.n == 1 ? launch_wide_tb16(p.kind, mode, a, st) : launch_wide_t16(mode, a, st);
     case WALK_TAB32: return p.n == 1 ? launch_tab32_bin(p.kind, mode, a, st, block) : launch_blk32(mode, a, st, block);
     case WALK_TAB32W: return p.n == 1 ? launch_wide_tb32(p.kind, mode, a, st) : launch_wide_t32(mode, a, st);
+    case WALK_COLS: return launch_colwalk(p.kind, mode, a, st);
     default: return launch_general(p.kind, mode, a, p.C, st);
   }
 }
@@ -487,9 +573,12 @@ struct Phase1Out {
 };
 
 // max_units > 0 limits the walk to the first max_units units (the
-// autotuner's probes); the result is then a partial maximum.
+// autotuner's probes); the result is then a partial maximum.  unit0: the
+// first unit's index (the walk covers units [unit0, unit0 + g.nseg)); the
+// first unit returned is absolute.
 int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometry &g, hipStream_t st,
-                 int block, Phase1Out &out, uint64_t max_units = 0, uint64_t chunk_units = 0) {
+                 int block, Phase1Out &out, uint64_t max_units = 0, uint64_t chunk_units = 0,
+                 uint64_t unit0 = 0) {
   if (int r = ensure_result_bufs(cx)) return r;
   const uint64_t chunk = chunk_units ? std::min(chunk_units, cx.unit_chunk) : cx.unit_chunk;
   const uint64_t nunits = max_units ? std::min(max_units, g.nseg) : g.nseg;
@@ -510,7 +599,7 @@ int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometr
     a.ny_pad = pp.ny_pad;
     a.d = p.n;
     a.q = g.q;
-    a.seg0 = c0;
+    a.seg0 = unit0 + c0;
     a.nseg = n;
     a.unit_best = ub;
     a.gbest = gbest;
@@ -532,7 +621,7 @@ int run_walk_max(DevCtx &cx, const Problem &p, const Prepared &pp, const Geometr
     out.ms_kernel += kms;
     if (hb > out.best || out.first_unit == ~0ull) {
       out.best = hb;
-      out.first_unit = c0 + hf;
+      out.first_unit = unit0 + c0 + hf;
     }
   }
   return 0;
@@ -683,8 +772,9 @@ bool blocked_plan(const Problem &p, const lnorm_hip_opts &o, BlockedPlan &bp, st
   return true;
 }
 
+// The blocked walk over waves [wave_begin, wave_begin + min(max_units, all)).
 int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t st, Phase1Out &out,
-                uint64_t max_units = 0) {
+                uint64_t max_units = 0, uint64_t wave_begin = 0) {
   if (int r = ensure_result_bufs(cx)) return r;
   const int marg = p.kind == KIND_LM ? 1 : 0;
   const int NW = bp.NW, kl = bp.kl, nh = bp.nh;
@@ -751,17 +841,15 @@ int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t
   ta.wide = bp.wide ? 1 : 0;
   HIPCHK(launch_table(ta, st));
 
-  int32_t *ub = (int32_t *)cx.units.p;
-  int32_t *gbest = (int32_t *)cx.scal.p;
-  unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
+  // one 64-bit key per launch: its maximum and first wave (blocked_key)
+  unsigned long long *gkey = (unsigned long long *)((char *)cx.scal.p + 128);
   const uint64_t nunits = max_units ? std::min(max_units, bp.nunits) : bp.nunits;
   out.units = nunits;
   out.best = INT32_MIN;
   out.first_unit = ~0ull;
   for (uint64_t c0 = 0; c0 < nunits; c0 += cx.unit_chunk) {
     const uint64_t n = std::min(cx.unit_chunk, nunits - c0);
-    HIPCHK(launch_fill_i32(gbest, INT32_MIN, 1, st));
-    HIPCHK(launch_fill_u64(gfirst, ~0ull, 1, st));
+    HIPCHK(launch_fill_u64(gkey, 0ull, 1, st));
     BlockedArgs a{};
     a.table = (const uint32_t *)cx.table.p;
     a.dpk = (const uint32_t *)cx.dpk.p;
@@ -769,27 +857,26 @@ int run_blocked(DevCtx &cx, const Problem &p, const BlockedPlan &bp, hipStream_t
     a.dfull = (const uint32_t *)cx.dfull.p;
     a.nh = nh;
     a.s = bp.s;
-    a.wave0 = c0;
+    a.wave0 = wave_begin + c0;
     a.nwaves = n;
-    a.unit_best = ub;
-    a.gbest = gbest;
+    a.gkey = gkey;
     HIPCHK(hipEventRecord(cx.ev[4], st));
     hipError_t e = bp.wide ? launch_blocked32(bp.NC, bp.P, bp.W, marg, a, st)
                            : launch_blocked(bp.np, bp.P, marg, a, st);
     HIPCHK(hipEventRecord(cx.ev[5], st));
     if (e != hipSuccess) return fail(LNORM_E_HIP, "blocked launch failed: %s", hipGetErrorString(e));
-    HIPCHK(launch_first_unit(ub, n, gbest, gfirst, st));
-    int32_t hb;
-    unsigned long long hf;
-    HIPCHK(hipMemcpyAsync(&hb, gbest, sizeof hb, hipMemcpyDeviceToHost, st));
-    HIPCHK(hipMemcpyAsync(&hf, gfirst, sizeof hf, hipMemcpyDeviceToHost, st));
+    unsigned long long hk = 0;
+    HIPCHK(hipMemcpyAsync(&hk, gkey, sizeof hk, hipMemcpyDeviceToHost, st));
     HIPCHK(hipStreamSynchronize(st));
+    if (hk == 0) return fail(LNORM_E_HIP, "blocked launch left no result");
+    const int32_t hb = blocked_key_value(hk);
+    const uint64_t hf = blocked_key_wave(hk);
     float kms = 0;
     HIPCHK(hipEventElapsedTime(&kms, cx.ev[4], cx.ev[5]));
     out.ms_kernel += kms;
     if (hb > out.best || out.first_unit == ~0ull) {
       out.best = hb;
-      out.first_unit = c0 + hf;
+      out.first_unit = wave_begin + c0 + hf;
     }
   }
   return 0;
@@ -816,6 +903,7 @@ struct Choice {
   BlockedPlan bp;
   int ny_plain = 0, w16 = 0, ny_wide = 0, w16_wide = 0, wtab = 0, wtabw = 0;
   int wtab32 = 0, wtab32w = 0;   // the 32-bit tabulated walks: one wave or W waves (words = columns)
+  int cols = 0;                  // the column walk: 64 x columns per lane (0: none)
 };
 
 int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
@@ -869,8 +957,18 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
       }
     }
   }
+  // the column walk: any int32 input whose width has an instantiation and
+  // whose rows fit the LDS of one workgroup
+  {
+    const int ncols = p.C - (p.kind == KIND_LM ? 1 : 0);
+    const int cpl = ncols > 0 ? colwalk_cpl(p.n, ncols) : 0;
+    if (cpl && colwalk_lds_bytes(p.R, 64 * cpl) <= kColwalkLdsMax) c.cols = 64 * cpl;
+  }
   if (kernel == LNORM_KERNEL_BLOCKED && !can_block)
     return fail(LNORM_E_UNSUPPORTED, "blocked kernel not applicable: %s", why.c_str());
+  if (kernel == LNORM_KERNEL_COLUMNS && !c.cols)
+    return fail(LNORM_E_UNSUPPORTED, "column walk needs an instantiated width (%d columns at n = %d) whose rows "
+                "fit one workgroup's LDS", p.C, p.n);
   if (kernel == LNORM_KERNEL_TABLE32 && !c.wtab32 && !c.wtab32w)
     return fail(LNORM_E_UNSUPPORTED, "32-bit tabulated walk needs an instantiated width and more enumerated rows "
                 "than table rows (%d x %d at n = %d)", p.R, p.C, p.n);
@@ -905,10 +1003,12 @@ int choose_kernel(const Problem &p, const lnorm_hip_opts &o, Choice &c) {
       kernel = LNORM_KERNEL_PLAIN16;
     else if (c.ny_plain)
       kernel = LNORM_KERNEL_PLAIN;
+    else if (c.ny_wide || c.w16_wide)
+      kernel = LNORM_KERNEL_WIDE;
     else
-      kernel = (c.ny_wide || c.w16_wide) ? LNORM_KERNEL_WIDE : LNORM_KERNEL_GENERAL;
+      kernel = c.cols ? LNORM_KERNEL_COLUMNS : LNORM_KERNEL_GENERAL;
   }
-  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_TABLE32)
+  if (kernel < LNORM_KERNEL_PLAIN || kernel > LNORM_KERNEL_COLUMNS)
     return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
   c.kernel = kernel;
   return 0;
@@ -920,6 +1020,7 @@ void prepare_for(const Problem &p, const Choice &c, Prepared &pp) {
   else if (c.kernel == LNORM_KERNEL_TABLE16) prepare_walk(p, WALK_TAB16W, 2 * c.wtabw, pp);
   else if (c.kernel == LNORM_KERNEL_TABLE32 && c.wtab32) prepare_walk(p, WALK_TAB32, c.wtab32, pp);
   else if (c.kernel == LNORM_KERNEL_TABLE32) prepare_walk(p, WALK_TAB32W, c.wtab32w, pp);
+  else if (c.kernel == LNORM_KERNEL_COLUMNS) prepare_walk(p, WALK_COLS, c.cols, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE && c.w16_wide) prepare_walk(p, WALK_WIDE16, 2 * c.w16_wide, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
   else if (c.kernel == LNORM_KERNEL_BLOCKED && !c.ny_plain && c.ny_wide) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
@@ -1123,6 +1224,8 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
   g_err.clear();
   Problem p{A, R, C, n, marginal, KIND_L1, 2, 0};
   if (int r = validate(p)) return r;
+  uint64_t rb = 0, re = 0;
+  if (int r = rank_range(p, o, rb, re)) return r;
   if (o.want_strategy && R > 1 && !strategy) return fail(LNORM_E_ARG, "strategy output is NULL");
   int dev = 0;
   if (int r = select_device(o, dev)) return r;
@@ -1133,7 +1236,6 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
 
   lnorm_hip_result rr;
   std::memset(&rr, 0, sizeof rr);
-  rr.total = p.total;
 
   Choice ch;
   if (int r = choose_kernel(p, o, ch)) return r;
@@ -1172,6 +1274,7 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
   }
   const int kernel = ch.kernel;
   const BlockedPlan &bp = ch.bp;
+  rr.total = re - rb;
 
   // the walk that enumerates (non-blocked kernels) and recovers the strategy (all)
   Prepared pp;
@@ -1179,12 +1282,60 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
   const bool general = pp.walk == WALK_GENERAL;
   if (int r = upload_walk(cx, p, pp, st)) return r;
 
-  Phase1Out p1;
+  // The ranks in runs of aligned units (one run of whole units for the
+  // whole range): whole units on the chosen kernel, shorter units at the
+  // ends of a partial range on the walk `pp` where its segments may be that
+  // short, else on the plain (or general) walk `pp2`.
   int unit_e;   // log_base of the ranks per unit
-  HIPCHK(hipEventRecord(cx.ev[0], st));
+  uint64_t chunk = 0;
+  Geometry gmain{};
   if (kernel == LNORM_KERNEL_BLOCKED) {
-    if (int r = run_blocked(cx, p, bp, st, p1)) return r;
     unit_e = bp.s + bp.kl;
+  } else {
+    gmain = single_geometry(cx, p, pp, o, chunk);
+    unit_e = gmain.q;
+  }
+  const std::vector<RankRun> runs = split_ranks(p.base, unit_e, rb, re);
+  const int qlow = walk_qmin(pp);
+  std::vector<char> on2(runs.size(), 0);
+  bool need2 = false;
+  for (size_t i = 0; i < runs.size(); ++i)
+    if (runs[i].e < unit_e && runs[i].e < qlow) on2[i] = need2 = true;
+  Prepared pp2;
+  if (need2) prepare_walk(p, ch.ny_plain ? WALK_PLAIN : WALK_GENERAL, ch.ny_plain, pp2);
+
+  std::vector<Phase1Out> outs(runs.size());
+  Phase1Out p1;
+  HIPCHK(hipEventRecord(cx.ev[0], st));
+  for (int pass = 0; pass < (need2 ? 2 : 1); ++pass) {
+    if (pass == 1)
+      if (int r = upload_walk(cx, p, pp2, st)) return r;
+    for (size_t i = 0; i < runs.size(); ++i) {
+      const RankRun &run = runs[i];
+      if (on2[i] != pass) continue;
+      int r;
+      if (run.e == unit_e && kernel == LNORM_KERNEL_BLOCKED) {
+        r = run_blocked(cx, p, bp, st, outs[i], run.n, run.u0);
+      } else if (run.e == unit_e) {
+        Geometry g = gmain;
+        g.nseg = run.n;
+        r = run_walk_max(cx, p, pp, g, st, o.block, outs[i], 0, chunk, run.u0);
+      } else {
+        const Geometry g{run.e, ipow((uint64_t)p.base, run.e), run.n};
+        r = run_walk_max(cx, p, pass ? pp2 : pp, g, st, o.block, outs[i], 0, 0, run.u0);
+      }
+      if (r) return r;
+      if (run.e == unit_e) p1.units += run.n;
+      p1.ms_kernel += outs[i].ms_kernel;
+    }
+  }
+  // the maximum, and the first run attaining it (runs are in rank order)
+  size_t win = 0;
+  for (size_t i = 1; i < runs.size(); ++i)
+    if (outs[i].best > outs[win].best) win = i;
+  p1.best = outs[win].best;
+  p1.first_unit = outs[win].first_unit;
+  if (kernel == LNORM_KERNEL_BLOCKED) {
     rr.patterns_per_lane = bp.P;
     if (bp.wide)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "blocked32<nc=%d,P=%d,W=%d,marg=%d,s=%d>", bp.NC, bp.P, bp.W,
@@ -1193,39 +1344,39 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "blocked16<np=%d,P=%d,marg=%d,s=%d>", bp.np, bp.P,
                p.kind == KIND_LM, bp.s);
   } else {
-    uint64_t chunk = 0;
-    const Geometry g = single_geometry(cx, p, pp, o, chunk);
-    if (int r = run_walk_max(cx, p, pp, g, st, o.block, p1, 0, chunk)) return r;
-    unit_e = g.q;
+    const int q = gmain.q;
     if (general)
-      snprintf(rr.kernel_name, sizeof rr.kernel_name, "general<kind=%d,d=%d,q=%d>", (int)p.kind, p.n, g.q);
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "general<kind=%d,d=%d,q=%d>", (int)p.kind, p.n, q);
     else if (pp.walk == WALK_PLAIN16)
-      snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain16<d=%d,words=%d,q=%d>", p.n, pp.ny_pad / 2, g.q);
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain16<d=%d,words=%d,q=%d>", p.n, pp.ny_pad / 2, q);
     else if (pp.walk == WALK_WIDE)
-      snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide<kind=%d,ny=%d,q=%d>", (int)p.kind, pp.ny_pad, g.q);
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide<kind=%d,ny=%d,q=%d>", (int)p.kind, pp.ny_pad, q);
     else if (pp.walk == WALK_WIDE16)
-      snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide16<d=%d,words=%d,q=%d>", p.n, pp.ny_pad / 2, g.q);
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "wide16<d=%d,words=%d,q=%d>", p.n, pp.ny_pad / 2, q);
     else if (pp.walk == WALK_TAB16 && p.n == 1)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16<%s,words=%d,M=%d,q=%d>",
-               p.kind == KIND_LM ? "LM" : "L1", pp.ny_pad / 2, pp.qmin, g.q);
+               p.kind == KIND_LM ? "LM" : "L1", pp.ny_pad / 2, pp.qmin, q);
     else if (pp.walk == WALK_TAB16)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16<d=%d,words=%d,M=%d,q=%d>", p.n, pp.ny_pad / 2,
-               pp.qmin, g.q);
+               pp.qmin, q);
     else if (pp.walk == WALK_TAB16W && p.n == 1)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16w<%s,words=%d,M=%d,q=%d>",
-               p.kind == KIND_LM ? "LM" : "L1", pp.ny_pad / 2, pp.qmin, g.q);
+               p.kind == KIND_LM ? "LM" : "L1", pp.ny_pad / 2, pp.qmin, q);
     else if (pp.walk == WALK_TAB16W)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table16w<d=%d,words=%d,M=%d,q=%d>", p.n, pp.ny_pad / 2,
-               pp.qmin, g.q);
+               pp.qmin, q);
     else if (is_tab32(pp.walk) && p.n == 1)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table32%s<%s,words=%d,M=%d,q=%d>",
-               pp.walk == WALK_TAB32W ? "w" : "", p.kind == KIND_LM ? "LM" : "L1", pp.ny_pad, pp.qmin, g.q);
+               pp.walk == WALK_TAB32W ? "w" : "", p.kind == KIND_LM ? "LM" : "L1", pp.ny_pad, pp.qmin, q);
     else if (is_tab32(pp.walk))
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table32%s<d=%d,words=%d,M=%d,q=%d>",
-               pp.walk == WALK_TAB32W ? "w" : "", p.n, pp.ny_pad, pp.qmin, g.q);
+               pp.walk == WALK_TAB32W ? "w" : "", p.n, pp.ny_pad, pp.qmin, q);
+    else if (pp.walk == WALK_COLS)
+      snprintf(rr.kernel_name, sizeof rr.kernel_name, "cols<kind=%d,d=%d,cpl=%d,q=%d>", (int)p.kind, p.n,
+               pp.ny_pad / 64, q);
     else
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "plain<kind=%d,d=%d,ny=%d,q=%d>", (int)p.kind, p.n,
-               pp.ny_pad, g.q);
+               pp.ny_pad, q);
   }
   HIPCHK(hipEventRecord(cx.ev[1], st));
   rr.units = p1.units;
@@ -1234,9 +1385,14 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
   rr.Lnorm = p1.best;
 
   if (o.want_strategy && R > 1) {
+    // the smallest maximising rank: MODE_FIRST over the winning unit, on the
+    // walk of its run (the walk on the device is pp2 after a second pass)
+    if (need2 && !on2[win])
+      if (int r = upload_walk(cx, p, pp, st)) return r;
     HIPCHK(hipEventRecord(cx.ev[2], st));
     uint64_t rank = 0;
-    if (int r = run_walk_first(cx, p, pp, p1.first_unit, unit_e, p1.best, st, o.block, rank)) return r;
+    if (int r = run_walk_first(cx, p, on2[win] ? pp2 : pp, p1.first_unit, runs[win].e, p1.best, st, o.block, rank))
+      return r;
     HIPCHK(hipEventRecord(cx.ev[3], st));
     HIPCHK(hipEventSynchronize(cx.ev[3]));
     float ms = 0;
@@ -1443,6 +1599,8 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
   int first_err = 0;
   std::vector<BItem> bis;
   std::vector<int> fallback;
+  if (o.rank_begin || o.rank_end)
+    return fail(LNORM_E_ARG, "batches enumerate whole problems: rank_begin and rank_end must be 0");
   if (o.batch_streams < 0 || o.batch_streams > kSideStreams + 1)
     return fail(LNORM_E_ARG, "batch_streams must be 0..%d", kSideStreams + 1);
   if (o.batch_cap_log != 0 && (o.batch_cap_log < 8 || o.batch_cap_log > 20))
@@ -1816,15 +1974,9 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
   if (p.C > kMaxCols) return fail(LNORM_E_SIZE, "%d columns exceed length %d", p.C, kMaxCols);
   p.kind = p.n == 1 ? (p.marginal ? KIND_LM : KIND_L1) : (p.n == 2 ? KIND_L2 : KIND_LD);
   p.base = p.n <= 2 ? 2 : p.n;
-  if (p.n <= 2) {
-    p.total = 1ull << (p.R - 1);
-  } else {
-    const double dt = std::pow((double)p.n, (double)(p.R - 1));
-    if (dt >= 9007199254740992.0)
-      return fail(LNORM_E_SIZE, "%d^%d labelings exceed 2^53, where the reference's pow() count is inexact", p.n,
-                  p.R - 1);
-    p.total = ipow((uint64_t)p.n, p.R - 1);
-  }
+  if (int r = rank_count(p)) return r;
+  uint64_t rb = 0, re = 0;
+  if (int r = rank_range(p, o, rb, re)) return r;
   if (o.want_strategy && R > 1 && !strategy) return fail(LNORM_E_ARG, "strategy output is NULL");
   const int ny_plain = plain64_ny_pad(p.kind, p.n, C);
   int kernel = o.kernel;
@@ -1871,29 +2023,38 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
 
   lnorm_hip_result rr;
   std::memset(&rr, 0, sizeof rr);
-  rr.total = p.total;
+  rr.total = re - rb;
   const Walk walk = general ? WALK_GENERAL : WALK_PLAIN;
   const Geometry g = geometry_for(p.base, R - 1, 20, o.seg_log, walk);
   double ms_kernel = 0;
   int64_t best = INT64_MIN;
   uint64_t first_unit = ~0ull;
+  int first_e = g.q;
+  // the ranks in runs of aligned units (one run of whole units for the whole
+  // range), in rank order; every run on the same walk
   HIPCHK(hipEventRecord(cx.ev[0], st));
-  for (uint64_t c0 = 0; c0 < g.nseg; c0 += kUnitsPerChunk) {
-    const uint64_t nch = std::min(kUnitsPerChunk, g.nseg - c0);
-    int64_t b;
-    uint64_t f;
-    if (int r = run64_phase(cx, p.kind, p.n, ny, general, MODE_MAX, g, c0, nch, R, 0, st, ms_kernel, b, f)) return r;
-    if (first_unit == ~0ull || b > best) {
-      best = b;
-      first_unit = c0 + f;
+  for (const RankRun &run : split_ranks(p.base, g.q, rb, re)) {
+    const Geometry gr{run.e, ipow((uint64_t)p.base, run.e), run.n};
+    if (run.e == g.q) rr.units += run.n;
+    for (uint64_t c0 = 0; c0 < run.n; c0 += kUnitsPerChunk) {
+      const uint64_t nch = std::min(kUnitsPerChunk, run.n - c0);
+      int64_t b;
+      uint64_t f;
+      if (int r = run64_phase(cx, p.kind, p.n, ny, general, MODE_MAX, gr, run.u0 + c0, nch, R, 0, st, ms_kernel, b,
+                              f))
+        return r;
+      if (first_unit == ~0ull || b > best) {
+        best = b;
+        first_unit = run.u0 + c0 + f;
+        first_e = run.e;
+      }
     }
   }
   HIPCHK(hipEventRecord(cx.ev[1], st));
-  rr.units = g.nseg;
   rr.unit_log = g.q;
   rr.kernel = kernel;
   if (o.want_strategy && R > 1) {
-    const Geometry g2 = geometry_for(p.base, g.q, 16, 0, walk);
+    const Geometry g2 = geometry_for(p.base, first_e, 16, 0, walk);
     double ms2 = 0;
     int64_t b;
     uint64_t rank;

@@ -21,8 +21,10 @@
 // Every wave evaluates ranks N = N_high * 2^kl + N_low for all N_low, i.e.
 // the contiguous range [w 2^(s+kl), (w+1) 2^(s+kl)): the BRGC of N restricted
 // to the high rows is the BRGC of N_high.  The wave's maximum goes to
-// unit_best[w]; the smallest maximising rank is recovered afterwards by the
-// plain kernel in MODE_FIRST on the first wave that attains the maximum.
+// one 64-bit atomic maximum per launch (blocked_key: the value above the
+// complement of the wave index), which leaves the launch's maximum and the
+// first wave attaining it; the smallest maximising rank is recovered
+// afterwards by the plain kernel in MODE_FIRST on that wave's range.
 #include "lnorm_device.h"
 
 namespace lnorm {
@@ -31,7 +33,7 @@ template <int NP, int P, int MARG>
 __global__ __launch_bounds__(256) void k_blocked16(
     const uint32_t *__restrict__ table, const uint32_t *__restrict__ dpk,
     const uint32_t *__restrict__ seed, const uint32_t *__restrict__ dfull, int nh, int s,
-    uint64_t wave0, uint64_t nwaves, int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest) {
+    uint64_t wave0, uint64_t nwaves, unsigned long long *__restrict__ gkey) {
   constexpr int NW = NP + MARG;
   const int lane = wave_lane();
   const uint64_t wl = global_wave();
@@ -103,10 +105,10 @@ __global__ __launch_bounds__(256) void k_blocked16(
   eval();                        // state of rank nsteps-1
 
   const int32_t m = wave_max_i32(best);
-  if (lane == 0) {
-    unit_best[wl] = m;
-    atomicMax(gbest, m);
-  }
+  // the key only grows, so one that is not above the current key cannot
+  // change it: most waves skip the atomic (each one would write to memory)
+  const unsigned long long key = blocked_key(m, wl);
+  if (lane == 0 && key > *(volatile unsigned long long *)gkey) atomicMax(gkey, key);
 }
 
 // ---------------------------------------------------------------------------
@@ -211,8 +213,7 @@ template <int NC, int P, int W, int MARG>
 __global__ __launch_bounds__(64 * W, 3) void k_blocked32(
     const uint32_t *__restrict__ table, const uint32_t *__restrict__ dpk,
     const uint32_t *__restrict__ seed, const uint32_t *__restrict__ dfull, int nh, int s,
-    uint64_t unit0, uint64_t nunits, int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest,
-    uint32_t B32) {
+    uint64_t unit0, uint64_t nunits, unsigned long long *__restrict__ gkey, uint32_t B32) {
   constexpr int WNC = W * NC;            // |.| words of the whole group
   constexpr int NWT = WNC + MARG;        // words per pattern (table) and per state
   constexpr int B = 8;                   // steps per LDS combine
@@ -306,12 +307,12 @@ __global__ __launch_bounds__(64 * W, 3) void k_blocked32(
     if (wv == 0 && lane == 0) {
       int32_t m = wbest[0];
       for (int w2 = 1; w2 < W; ++w2) m = wbest[w2] > m ? wbest[w2] : m;
-      unit_best[gl] = m;
-      atomicMax(gbest, m);
+      const unsigned long long key = blocked_key(m, gl);
+      if (key > *(volatile unsigned long long *)gkey) atomicMax(gkey, key);
     }
   } else if (lane == 0) {
-    unit_best[gl] = bm;
-    atomicMax(gbest, bm);
+    const unsigned long long key = blocked_key(bm, gl);
+    if (key > *(volatile unsigned long long *)gkey) atomicMax(gkey, key);
   }
 }
 
@@ -325,7 +326,7 @@ bool blocked32_supported(int nc, int P, int W, int marginal) {
 template <int NC, int P, int W, int M>
 static hipError_t launch_b32(const BlockedArgs &a, hipStream_t st) {
   hipLaunchKernelGGL((k_blocked32<NC, P, W, M>), dim3((unsigned)a.nwaves), dim3(64 * W), 0, st, a.table, a.dpk,
-                     a.seed, a.dfull, a.nh, a.s, a.wave0, a.nwaves, a.unit_best, a.gbest, kBias32);
+                     a.seed, a.dfull, a.nh, a.s, a.wave0, a.nwaves, a.gkey, kBias32);
   return hipGetLastError();
 }
 
@@ -363,7 +364,7 @@ static hipError_t launch_b32n(int nc, int P, int W, const BlockedArgs &a, hipStr
 }
 
 hipError_t launch_blocked32(int nc, int P, int W, int marginal, const BlockedArgs &a, hipStream_t st) {
-  if (!a.table || !a.dpk || !a.seed || !a.dfull || !a.unit_best || !a.gbest) return hipErrorInvalidValue;
+  if (!a.table || !a.dpk || !a.seed || !a.dfull || !a.gkey || a.nwaves >= 0xFFFFFFFFull) return hipErrorInvalidValue;
   if (!blocked32_supported(nc, P, W, marginal)) return hipErrorInvalidValue;
   return marginal ? launch_b32n<1>(nc, P, W, a, st) : launch_b32n<0>(nc, P, W, a, st);
 }
@@ -391,7 +392,7 @@ static hipError_t launch_b(const BlockedArgs &a, hipStream_t st) {
   const int block = 256;
   const dim3 grid((unsigned)((a.nwaves + 3) / 4)), blk(block);
   hipLaunchKernelGGL((k_blocked16<NP, P, M>), grid, blk, 0, st, a.table, a.dpk, a.seed, a.dfull, a.nh,
-                     a.s, a.wave0, a.nwaves, a.unit_best, a.gbest);
+                     a.s, a.wave0, a.nwaves, a.gkey);
   return hipGetLastError();
 }
 
@@ -407,7 +408,7 @@ static hipError_t launch_bp(int np, const BlockedArgs &a, hipStream_t st) {
 }
 
 hipError_t launch_blocked(int np, int P, int marginal, const BlockedArgs &a, hipStream_t st) {
-  if (!a.table || !a.dpk || !a.seed || !a.dfull || !a.unit_best || !a.gbest) return hipErrorInvalidValue;
+  if (!a.table || !a.dpk || !a.seed || !a.dfull || !a.gkey || a.nwaves >= 0xFFFFFFFFull) return hipErrorInvalidValue;
   if (!blocked_supported(np, P, marginal)) return hipErrorInvalidValue;
   if (marginal) {
     switch (P) {

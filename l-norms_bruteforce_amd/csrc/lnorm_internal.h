@@ -64,10 +64,23 @@ struct BlockedArgs {
   int nh;                  // high rows walked by the Gray code
   int s;                   // high ranks per wave = 2^s
   uint64_t wave0;          // first wave index
-  uint64_t nwaves;
-  int32_t *unit_best;      // best per wave
-  int32_t *gbest;
+  uint64_t nwaves;         // waves of this launch (< 2^32 - 1)
+  unsigned long long *gkey;   // atomic max over the launch's waves of blocked_key(value, wave - wave0)
 };
+
+// The blocked kernels' result key of one wave (or workgroup) w of a launch:
+// the value, made order-preserving as unsigned, above the complement of w, so
+// the largest key holds the launch's maximum and the first wave attaining it.
+// A zero key (the host's initial value) is below every wave's key.
+__host__ __device__ inline unsigned long long blocked_key(int32_t v, uint64_t w) {
+  return ((unsigned long long)((uint32_t)v ^ 0x80000000u) << 32) | (0xFFFFFFFFull - (uint32_t)w);
+}
+__host__ __device__ inline int32_t blocked_key_value(unsigned long long k) {
+  return (int32_t)((uint32_t)(k >> 32) ^ 0x80000000u);
+}
+__host__ __device__ inline uint64_t blocked_key_wave(unsigned long long k) {
+  return 0xFFFFFFFFull - (k & 0xFFFFFFFFull);
+}
 
 struct TableArgs {
   const int8_t *lo;        // low rows as int8 planes: [planes][kl][ny] (hi plane first)

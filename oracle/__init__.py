@@ -58,6 +58,9 @@ def lib():
         L.lo_objective.restype = ctypes.c_int32
         L.lo_objective.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, i32p]
         i64p = ctypes.POINTER(ctypes.c_int64)
+        L.lo_walk64.restype = ctypes.c_int
+        L.lo_walk64.argtypes = [i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                ctypes.c_uint64, ctypes.c_uint64, i64p, u64p, i32p]
         L.lo_calc_lnorm64.restype = ctypes.c_int
         L.lo_calc_lnorm64.argtypes = [i64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int, i64p, i32p, u64p]
@@ -166,6 +169,21 @@ def brute64(A, n: int, marginal: int = 0):
     if lib().lo_brute64(p, R, C, n, marginal, ctypes.byref(norm), ctypes.byref(rank)) != 0:
         raise RuntimeError("lo_brute64 failed")
     return int(norm.value), int(rank.value)
+
+
+def walk64(A, n: int, marginal: int, jmin: int, jmax: int):
+    """(best, first rank of best, strategy) over the inclusive rank range,
+    long long variant."""
+    a, p = _arr64(A)
+    R, C = a.shape
+    best = ctypes.c_int64()
+    rank = ctypes.c_uint64()
+    strat = np.zeros(max(R - 1, 1), dtype=np.int32)
+    rc = lib().lo_walk64(p, R, C, n, marginal, jmin, jmax, ctypes.byref(best), ctypes.byref(rank),
+                         strat.ctypes.data_as(ctypes.POINTER(ctypes.c_int32)))
+    if rc != 0:
+        raise RuntimeError("lo_walk64 failed")
+    return int(best.value), int(rank.value), [int(v) for v in strat[: R - 1]]
 
 
 def objective64(A, n: int, marginal: int, labels) -> int:

@@ -141,3 +141,27 @@ def test_batch_workload_4096(pkg):
             norm, strat, _ = pkg.calc_lnorm(A, n, m)
             assert got[:2] == (norm, strat), k
     assert checked_oracle > 100
+
+
+@pytest.mark.timeout(300)
+def test_batch_workload_uncapped_d3_items(pkg):
+    """The mixed batch workload without the 20-row cap on d = 3 files: the
+    d = 3 items with 22 or 23 rows after preprocessing (3^21 and 3^22
+    labelings, beyond the persistent launches' 2^32, so the batch runs them
+    on the single-problem kernels).  The batch's result equals the
+    tabulated walk's and the packed walk's (two different kernels: norm,
+    strategy and rank), and the strategy's objective is the norm."""
+    from lnorm_amd import gen
+    print("batch workload without the d = 3 row cap: items with 22-23 rows", flush=True)
+    work = [w for w in gen.batch_workload(4096, d3_max_rows=30) if w["d"] == 3]
+    probs = [(A, n, m) for A, n, m, total in _workload_problems(work) if A.shape[0] in (22, 23)]
+    assert len(probs) >= 40
+    out, st = pkg.calc_batch(probs, stats=True)
+    assert st["fallback"] == len(probs) and st["batched"] == 0, st
+    for k, ((A, n, m), got) in enumerate(zip(probs, out)):
+        assert got[2] == 0, k
+        t_norm, t_strat, t_res = pkg.calc_lnorm(A, n, m, kernel="table16")
+        p_norm, p_strat, p_res = pkg.calc_lnorm(A, n, m, kernel="plain16")
+        assert (t_norm, t_strat, t_res["best_rank"]) == (p_norm, p_strat, p_res["best_rank"]), (k, A.shape)
+        assert got[:2] == (t_norm, t_strat), (k, A.shape)
+        assert oracle.objective(A, n, m, t_strat) == t_norm, (k, A.shape)
