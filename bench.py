@@ -547,6 +547,36 @@ def wide_lines(pkg, gen) -> dict:
     return out
 
 
+# Wide int32 workloads beyond the tabulated and W-wave walks: the column walk
+# (one wave per segment, the columns over the lanes) beside the general
+# kernel it replaces, which the test suite checks it against
+COLUMN_CASES = [("L3_14x300_int32", 14, 300, 20000, 112, 3), ("L3_16x512_int32", 16, 512, 20000, 113, 3),
+                ("L6_10x300_int32", 10, 300, 20000, 114, 6), ("L1_24x1500_int16", 24, 1500, 20000, 115, 1),
+                ("L10_7x700_int32", 7, 700, 20000, 118, 10)]
+
+
+def columns_lines(pkg, gen) -> dict:
+    """Each COLUMN_CASES matrix on AUTO (the column walk) and on the general
+    kernel (the 16x512 one on AUTO only): labelings/s, and the same norm and
+    rank from both."""
+    out = {}
+    for name, R, C, r, seed, d in COLUMN_CASES:
+        A = gen.gen_matrix(R, C, r, seed)
+        total = 2 ** (R - 1) if d == 1 else d ** (R - 1)
+        line = {}
+        for k in ("auto", "general") if C != 512 else ("auto",):
+            pkg.calc_lnorm(A, d, 0, kernel=k)
+            norm, _, res = pkg.calc_lnorm(A, d, 0, kernel=k)
+            line[k] = {"kernel": res["kernel_name"], "kernel_ms": res["ms_kernel"],
+                       "labelings_per_s": total / (res["ms_kernel"] / 1e3), "norm": norm, "rank": res["best_rank"]}
+        if len({(v["norm"], v["rank"]) for v in line.values()}) != 1:
+            raise RuntimeError("the column walk and the general kernel disagree on " + name)
+        if "general" in line:
+            line["speedup_vs_general"] = line["auto"]["labelings_per_s"] / line["general"]["labelings_per_s"]
+        out[name] = line
+    return out
+
+
 # int16-range workloads: column sums beyond 16 bits, so the 32-bit words
 #   int16:      L1 of the 128 x 40 int16 matrix (entries in [-30000, 30000],
 #               seed 105), laid out 40 x 128: 2^39 sign vectors
@@ -601,6 +631,9 @@ def run_case(pkg, gen, case: str, steps: int, warmup: int) -> int:
     if case == "wide":
         print(json.dumps({"case": case, **wide_lines(pkg, gen)}), flush=True)
         return 0
+    if case == "columns":
+        print(json.dumps({"case": case, **columns_lines(pkg, gen)}), flush=True)
+        return 0
     if case == "int64":
         # L1 of a 34 x 40 matrix scaled by 2^28 + 3 (beyond int32): the 64-bit walk
         A64 = gen.gen_matrix(34, 40, 100, 108).astype("int64") * (2 ** 28 + 3)
@@ -624,7 +657,7 @@ def main():
                     help="run the default launch shapes instead of the autotuner's (tuned before timing)")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the autotuner's candidates on the bench workloads and print them as JSON")
-    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "batch_full", "l3", "int64", "wide", "int16", "int16_blocked",
+    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "batch_full", "l3", "int64", "columns", "wide", "int16", "int16_blocked",
                                        "int16_plain", "wide_int16", "wide_int16_wide", "l3_int16",
                                        "l3_int16_general"],
                     help="run only this workload (for profiler passes) and print its figures")

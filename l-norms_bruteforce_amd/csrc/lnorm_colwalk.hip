@@ -270,10 +270,14 @@ __global__ __launch_bounds__(64 * kColWaves) void k_colwalk_bin(
     const int j = __builtin_ctz(tt);
     const bool up = !(((N0 + tt) >> (j + 1)) & 1);   // Gray bit j becomes 1: s_j from -1 to +1
     const uint32_t *row = mine + (size_t)j * Cp;
+    // `up` is wave-uniform: a branch, not a per-column select (which cost
+    // a negate and a v_cndmask besides the add)
+    if (up) {
 #pragma unroll
-    for (int k = 0; k < CPL; ++k) {
-      const uint32_t v = row[k * 64];
-      y[k] += up ? v : 0u - v;
+      for (int k = 0; k < CPL; ++k) y[k] += row[k * 64];
+    } else {
+#pragma unroll
+      for (int k = 0; k < CPL; ++k) y[k] -= row[k * 64];
     }
     if (MARG) y0 += up ? 2 * marg[j] : -2 * marg[j];
     eval(tt);
