@@ -10,8 +10,10 @@
 // the lane-per-segment walks (L1/LM: the binary reflected code; d >= 2: the
 // reflected d-ary code in a nibble counter, d - 1 moves of row 0 unrolled
 // per block of d steps).  The rows sit in LDS (the workgroup copies the
-// matrix once): a step reads the changed row's CPL words per lane
-// (ds_read_b32, lane-consecutive, no bank conflicts), moves them between the
+// matrix once), the ones most steps move also in registers (row 0 for
+// d >= 2, which moves in (d - 1) / d of the steps; rows 0..H-1 for L1/LM):
+// a step takes the changed row's CPL words per lane (from registers, or
+// ds_read_b32, lane-consecutive, no bank conflicts), moves them between the
 // two labels that exchange the row (L1/LM: adds 2 A or -2 A), re-sums those
 // two labels' |.| over the lane's columns, and one butterfly over the wave
 // totals the lanes' partial objectives.
@@ -24,7 +26,11 @@ namespace lnorm {
 
 // columns per lane of the instantiated widths (64 * CPL columns)
 constexpr int kColCpl[] = {1, 2, 3, 4, 6, 8, 12, 16, 24, 32, 48, 64};
-constexpr int kColWaves = 8;   // waves (segments) per workgroup: one copy of the rows in LDS
+// waves (segments) per workgroup, which share one copy of the rows in LDS:
+// 8 for d >= 2 (up to 209 VGPRs: 2 waves per SIMD each), 16 for L1/LM (at
+// most 128 VGPRs), whose wide rows would otherwise leave 2 waves per SIMD
+// on a CU to hide the latency of each step's wave total
+constexpr int kColWavesDary = 8, kColWavesBin = 16;
 
 // the widest label state a lane holds: D x CPL words (binary: CPL)
 constexpr bool col_fits(int D, int CPL) { return (D <= 1 ? CPL : D * CPL) <= 128; }
@@ -45,14 +51,26 @@ __device__ __forceinline__ uint32_t wave_total(uint32_t v) {
   return (uint32_t)__builtin_amdgcn_readlane((int)v, 63);
 }
 
+// A row's words of the lane's columns: in LDS (LdsRow) or, for the rows most
+// steps move, held in registers (RegRow).
+struct LdsRow {
+  const uint32_t *p;   // the row's word of column lane, k * 64 apart
+  __device__ __forceinline__ uint32_t operator()(int k) const { return p[k * 64]; }
+};
+template <int CPL>
+struct RegRow {
+  const uint32_t (&r)[CPL];
+  __device__ __forceinline__ uint32_t operator()(int k) const { return r[k]; }
+};
+
 // ---- d >= 2: the label sums t[c][k] -----------------------------------------
-template <int F, int T, int D, int CPL>
-__device__ __forceinline__ void col_move(uint32_t (&t)[D][CPL], uint32_t (&S)[D], const uint32_t *row,
-                                         uint32_t B, uint32_t z) {
+template <int F, int T, int D, int CPL, class Row>
+__device__ __forceinline__ void col_move(uint32_t (&t)[D][CPL], uint32_t (&S)[D], const Row &row, uint32_t B,
+                                         uint32_t z) {
   uint32_t sf = z, st = z;
 #pragma unroll
   for (int k = 0; k < CPL; ++k) {
-    const uint32_t a = row[k * 64];
+    const uint32_t a = row(k);
     t[F][k] -= a;
     t[T][k] += a;
     sf = sad32(t[F][k], B, sf);
@@ -64,7 +82,7 @@ __device__ __forceinline__ void col_move(uint32_t (&t)[D][CPL], uint32_t (&S)[D]
 
 template <int C, int D, int CPL>
 struct ColMover {
-  __device__ __forceinline__ static void run(uint32_t (&t)[D][CPL], uint32_t (&S)[D], const uint32_t *row,
+  __device__ __forceinline__ static void run(uint32_t (&t)[D][CPL], uint32_t (&S)[D], const LdsRow &row,
                                              int from, int up, uint32_t B, uint32_t z) {
     if (from == C) {
       if (up) {
@@ -78,11 +96,13 @@ struct ColMover {
   }
 };
 
-// the d - 1 moves of row 0 inside one block of d steps
+// the d - 1 moves of row 0 inside one block of d steps (row 0 moves in
+// (d - 1) / d of all steps: its words are held in registers where the state
+// leaves room)
 template <int K, int D, int CPL, bool UP>
 struct ColDigit0Run {
-  template <class Eval>
-  __device__ __forceinline__ static void run(uint32_t (&t)[D][CPL], uint32_t (&S)[D], const uint32_t *row0,
+  template <class Row, class Eval>
+  __device__ __forceinline__ static void run(uint32_t (&t)[D][CPL], uint32_t (&S)[D], const Row &row0,
                                              uint32_t B, uint32_t z, uint32_t &tt, Eval &eval) {
     if constexpr (K < D) {
       constexpr int F = UP ? K - 1 : D - K;
@@ -114,7 +134,7 @@ __device__ __forceinline__ void col_stage(uint32_t *lds, const int32_t *__restri
 }
 
 template <int D, int CPL, int MODE>
-__global__ __launch_bounds__(64 * kColWaves) void k_colwalk_dary(
+__global__ __launch_bounds__(64 * kColWavesDary) void k_colwalk_dary(
     const int32_t *__restrict__ A, int R, int Cp, int q, uint64_t seg0, uint64_t nseg, int32_t target,
     int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest, unsigned long long *__restrict__ gfirst,
     uint32_t B) {
@@ -152,6 +172,16 @@ __global__ __launch_bounds__(64 * kColWaves) void k_colwalk_dary(
     }
     ColAdd<0, D, CPL>::run(t, mine + (size_t)i * Cp, lab);
   }
+  // row 0 in registers up to 96 state words (beyond, the extra CPL
+  // registers made the widest d = 8 .. 10 shapes spill)
+  constexpr bool kRow0Regs = D * CPL <= 96;
+  uint32_t r0[kRow0Regs ? CPL : 1];
+#pragma unroll
+  for (int k = 0; k < (kRow0Regs ? CPL : 1); ++k) r0[k] = mine[k * 64];
+  const auto row0 = [&]() {
+    if constexpr (kRow0Regs) return RegRow<CPL>{r0};
+    else return LdsRow{mine};
+  }();
   uint32_t S[D];
 #pragma unroll
   for (int c = 0; c < D; ++c) {
@@ -188,8 +218,8 @@ __global__ __launch_bounds__(64 * kColWaves) void k_colwalk_dary(
   eval(0);
   uint32_t tt = 0;
   if (q > 0) {
-    if (pm & 1u) ColDigit0Run<1, D, CPL, false>::run(t, S, mine, B, z, tt, eval);
-    else ColDigit0Run<1, D, CPL, true>::run(t, S, mine, B, z, tt, eval);
+    if (pm & 1u) ColDigit0Run<1, D, CPL, false>::run(t, S, row0, B, z, tt, eval);
+    else ColDigit0Run<1, D, CPL, true>::run(t, S, row0, B, z, tt, eval);
     for (uint32_t b = 1; b < nblocks; ++b) {
       const int jj = __builtin_ctzll(~cnt) >> 2;
       const uint64_t low = (1ull << (4 * jj)) - 1ull;
@@ -199,10 +229,10 @@ __global__ __launch_bounds__(64 * kColWaves) void k_colwalk_dary(
       const int p = (pm >> j) & 1;
       const int from = p ? D - v : v - 1;
       pm ^= (1u << j) - 1u;
-      ColMover<0, D, CPL>::run(t, S, mine + (size_t)j * Cp, from, p ? 0 : 1, B, z);
+      ColMover<0, D, CPL>::run(t, S, LdsRow{mine + (size_t)j * Cp}, from, p ? 0 : 1, B, z);
       eval(++tt);
-      if (pm & 1u) ColDigit0Run<1, D, CPL, false>::run(t, S, mine, B, z, tt, eval);
-      else ColDigit0Run<1, D, CPL, true>::run(t, S, mine, B, z, tt, eval);
+      if (pm & 1u) ColDigit0Run<1, D, CPL, false>::run(t, S, row0, B, z, tt, eval);
+      else ColDigit0Run<1, D, CPL, true>::run(t, S, row0, B, z, tt, eval);
     }
   }
   const uint64_t nsteps = q > 0 ? (uint64_t)nblocks * (uint64_t)D : 1ull;
@@ -218,7 +248,7 @@ __global__ __launch_bounds__(64 * kColWaves) void k_colwalk_dary(
 // rows: the 2 A rows of the |.| columns (LDS); A: the rows themselves (the
 // seed); marg: LM's column 0 (scalar loads), which enters without |.|.
 template <int CPL, int MARG, int MODE>
-__global__ __launch_bounds__(64 * kColWaves) void k_colwalk_bin(
+__global__ __launch_bounds__(64 * kColWavesBin) void k_colwalk_bin(
     const int32_t *__restrict__ A, const int32_t *__restrict__ A2, const int32_t *__restrict__ margp, int R,
     int Cp, int q, uint64_t seg0, uint64_t nseg, int32_t target, int32_t *__restrict__ unit_best,
     int32_t *__restrict__ gbest, unsigned long long *__restrict__ gfirst, uint32_t B) {
@@ -264,20 +294,45 @@ __global__ __launch_bounds__(64 * kColWaves) void k_colwalk_bin(
     }
   };
 
+  // rows 0 .. H-1 move in 1 - 2^-H of all steps: their 2 A words sit in
+  // registers, the others are read from LDS (one word per column and step
+  // would otherwise take the CU's whole LDS bandwidth at 2 VALU ops a word)
+  constexpr int H = CPL <= 16 ? 3 : CPL <= 32 ? 2 : CPL <= 48 ? 1 : 0;
+  uint32_t hr[H > 0 ? H : 1][CPL];
+#pragma unroll
+  for (int h = 0; h < H; ++h)
+#pragma unroll
+    for (int k = 0; k < CPL; ++k) hr[h][k] = h < R ? mine[(size_t)h * Cp + k * 64] : 0u;
+
   eval(0);
   const uint32_t nsteps = 1u << q;
   for (uint32_t tt = 1; tt < nsteps; ++tt) {
     const int j = __builtin_ctz(tt);
     const bool up = !(((N0 + tt) >> (j + 1)) & 1);   // Gray bit j becomes 1: s_j from -1 to +1
-    const uint32_t *row = mine + (size_t)j * Cp;
-    // `up` is wave-uniform: a branch, not a per-column select (which cost
-    // a negate and a v_cndmask besides the add)
-    if (up) {
+    // `up` and j are wave-uniform: branches, not per-column selects (which
+    // cost a negate and a v_cndmask besides the add)
+    if (j < H) {
 #pragma unroll
-      for (int k = 0; k < CPL; ++k) y[k] += row[k * 64];
+      for (int h = 0; h < H; ++h) {
+        if (j == h) {
+          if (up) {
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) y[k] += hr[h][k];
+          } else {
+#pragma unroll
+            for (int k = 0; k < CPL; ++k) y[k] -= hr[h][k];
+          }
+        }
+      }
     } else {
+      const uint32_t *row = mine + (size_t)j * Cp;
+      if (up) {
 #pragma unroll
-      for (int k = 0; k < CPL; ++k) y[k] -= row[k * 64];
+        for (int k = 0; k < CPL; ++k) y[k] += row[k * 64];
+      } else {
+#pragma unroll
+        for (int k = 0; k < CPL; ++k) y[k] -= row[k * 64];
+      }
     }
     if (MARG) y0 += up ? 2 * marg[j] : -2 * marg[j];
     eval(tt);
@@ -299,16 +354,16 @@ int colwalk_cpl(int d, int ncols) {
 
 size_t colwalk_lds_bytes(int R, int Cp) { return (size_t)R * Cp * 4; }
 
-// Launches one wave per segment, kColWaves per workgroup, with the rows'
-// LDS; above the default 64 KiB the kernel is allowed the larger size first.
-template <class K, class... Args>
+// Launches one wave per segment, W per workgroup, with the rows' LDS; above
+// the default 64 KiB the kernel is allowed the larger size first.
+template <int W, class K, class... Args>
 static hipError_t lcol_launch(K kernel, uint64_t nseg, size_t lds, hipStream_t st, Args... args) {
   if (lds > (64u << 10)) {
     const hipError_t e = hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize,
                                              (int)lds);
     if (e != hipSuccess) return e;
   }
-  const dim3 grid((unsigned)((nseg + kColWaves - 1) / kColWaves)), blk(64 * kColWaves);
+  const dim3 grid((unsigned)((nseg + W - 1) / W)), blk(64 * W);
   hipLaunchKernelGGL(kernel, grid, blk, lds, st, args...);
   return hipGetLastError();
 }
@@ -317,10 +372,10 @@ template <int D, int CPL>
 static hipError_t lcol_dary(Mode mode, const PlainArgs &a, hipStream_t st) {
   const size_t lds = colwalk_lds_bytes(a.R, a.ny_pad);
   if (mode == MODE_MAX)
-    return lcol_launch(k_colwalk_dary<D, CPL, MODE_MAX>, a.nseg, lds, st, a.A, a.R, a.ny_pad, a.q, a.seg0, a.nseg,
-                       a.target, a.unit_best, a.gbest, a.gfirst, kBias32);
-  return lcol_launch(k_colwalk_dary<D, CPL, MODE_FIRST>, a.nseg, lds, st, a.A, a.R, a.ny_pad, a.q, a.seg0, a.nseg,
-                     a.target, a.unit_best, a.gbest, a.gfirst, kBias32);
+    return lcol_launch<kColWavesDary>(k_colwalk_dary<D, CPL, MODE_MAX>, a.nseg, lds, st, a.A, a.R, a.ny_pad, a.q,
+                                      a.seg0, a.nseg, a.target, a.unit_best, a.gbest, a.gfirst, kBias32);
+  return lcol_launch<kColWavesDary>(k_colwalk_dary<D, CPL, MODE_FIRST>, a.nseg, lds, st, a.A, a.R, a.ny_pad, a.q,
+                                    a.seg0, a.nseg, a.target, a.unit_best, a.gbest, a.gfirst, kBias32);
 }
 
 template <int CPL, int MARG>
@@ -330,10 +385,10 @@ static hipError_t lcol_bin(Mode mode, const PlainArgs &a, hipStream_t st) {
   const int32_t *A2 = a.D;
   const int32_t *marg = a.D + (size_t)a.R * a.ny_pad;
   if (mode == MODE_MAX)
-    return lcol_launch(k_colwalk_bin<CPL, MARG, MODE_MAX>, a.nseg, lds, st, a.A, A2, marg, a.R, a.ny_pad, a.q,
-                       a.seg0, a.nseg, a.target, a.unit_best, a.gbest, a.gfirst, kBias32);
-  return lcol_launch(k_colwalk_bin<CPL, MARG, MODE_FIRST>, a.nseg, lds, st, a.A, A2, marg, a.R, a.ny_pad, a.q,
-                     a.seg0, a.nseg, a.target, a.unit_best, a.gbest, a.gfirst, kBias32);
+    return lcol_launch<kColWavesBin>(k_colwalk_bin<CPL, MARG, MODE_MAX>, a.nseg, lds, st, a.A, A2, marg, a.R,
+                                     a.ny_pad, a.q, a.seg0, a.nseg, a.target, a.unit_best, a.gbest, a.gfirst, kBias32);
+  return lcol_launch<kColWavesBin>(k_colwalk_bin<CPL, MARG, MODE_FIRST>, a.nseg, lds, st, a.A, A2, marg, a.R,
+                                   a.ny_pad, a.q, a.seg0, a.nseg, a.target, a.unit_best, a.gbest, a.gfirst, kBias32);
 }
 
 template <int D>
