@@ -91,6 +91,9 @@ KERNEL_CASES = [
     (21, 33, 1, 1, 100, "table16"),
     (18, 600, 1, 0, 20, "table16"),        # the W-wave tabulated walk
     (22, 40, 1, 0, 3000, "table32"),
+    (18, 600, 1, 1, 3000, "table32"),      # 32-bit words over W waves, LM
+    (11, 200, 3, 0, 5000, "table32"),      # d-ary, 32-bit words over W waves
+    (10, 300, 3, 0, 5000, "columns"),
     (22, 40, 1, 0, 100, "plain"),
     (20, 200, 1, 0, 50, "wide"),
     (16, 30, 2, 0, 50, "table16"),
