@@ -39,12 +39,12 @@ MFMA_I8_PEAK = 5.0e15                   # dense int8 ops/s
 # for FETCH_SIZE, x1.0 for WRITE_SIZE).  Each launch leaves one 64-bit key
 # (its maximum and first wave); the matrix and the pattern table stay on chip.
 #   P=2, s=11 (autotuned, one launch of 2^21 waves per call): FETCH_SIZE
-#   137.1 KiB (x2.0), WRITE_SIZE 1.3 KiB (x1.0)
-PROFILED_TRAFFIC = {"blocked16<np=32,P=2,marg=0,s=11>": 280832 + 1344}
+#   136.94 KiB (x2.0), WRITE_SIZE 0.97 KiB (x1.0)
+PROFILED_TRAFFIC = {"blocked16<np=32,P=2,marg=0,s=11>": 280448 + 992}
 # VALU issue rate of the headline instantiation from the counters:
 # SQ_INSTS_VALU x 64 / dispatch time / 39.32e12 (rocprofv3 --pmc on
-# `bench.py --case blocked`: 2.7946e11 instructions in 466.0 ms)
-PROFILED_ISSUE = {"blocked16<np=32,P=2,marg=0,s=11>": 0.976}
+# `bench.py --case blocked`: 2.79452e11 instructions in 464.14 ms)
+PROFILED_ISSUE = {"blocked16<np=32,P=2,marg=0,s=11>": 0.980}
 
 
 def load_pkg():
@@ -416,6 +416,54 @@ def batch_line(pkg, gen, steps: int) -> dict:
             "graph_replayed": st["graph_replayed"], "autotuned": tuned}
 
 
+def cfg2_dup_matrix(gen):
+    """The marginal workload's variant: the 36 x 36 matrix (entries in
+    [-100, 100], seed 102) with 4 rows and 3 columns duplicated (sources
+    among indices 1..35) and 2 further non-marginal rows zeroed: 40 x 39."""
+    import numpy as np
+    M = gen.gen_matrix(36, 36, 100, 102)
+    rng = np.random.default_rng(102)
+    dup_r = rng.choice(np.arange(1, 36), 4, replace=False)
+    dup_c = rng.choice(np.arange(1, 36), 3, replace=False)
+    V = np.vstack([M, M[dup_r]])
+    V = np.hstack([V, V[:, dup_c]])
+    zero = [r for r in range(1, 36) if r not in set(dup_r.tolist())][:2]
+    V[zero, :] = 0
+    return V
+
+
+def configs_line(pkg, gen) -> dict:
+    """The reference's configured workloads one by one, each file through
+    the host library (reader, preprocessing, layout, parameters) and one
+    timed call on AUTO's kernel after a warm-up call: L1 20 x 20 (2^19),
+    LM 36 x 36 and its duplicated-rows variant with 'y' and without, L2 and
+    L3 of the 18 x 32 file (2^17, 3^17).  The two marginal runs of the
+    variant must print the same norm."""
+    from lnorm_amd import host
+    cases = [("cfg1_L1_20x20", gen.gen_matrix(20, 20, 10, 101), 1, "n", 0),
+             ("cfg2_LM_36x36", gen.gen_matrix(36, 36, 100, 102), 1, "n", 1),
+             ("cfg2_LM_40x39_dups_y", cfg2_dup_matrix(gen), 1, "y", 1),
+             ("cfg2_LM_40x39_dups_n", cfg2_dup_matrix(gen), 1, "n", 1),
+             ("cfg3_L2_18x32", gen.gen_matrix(18, 32, 50, 103), 2, "n", 0),
+             ("cfg3_L3_18x32", gen.gen_matrix(18, 32, 50, 103), 3, "n", 0)]
+    out = {}
+    for name, M, d, stat, marg in cases:
+        p, _, exited = host.load(gen.matrix_text(M).encode(), d, stat, marg)
+        if exited:
+            raise RuntimeError(name + ": rejected by the host pipeline")
+        A, n, m, total = p.matrix_laid_out(), p.n, p.marginal, p.total
+        host.free(p)
+        pkg.calc_lnorm(A, n, m)
+        t = time.perf_counter()
+        norm, _, res = pkg.calc_lnorm(A, n, m)
+        wall = time.perf_counter() - t
+        out[name] = {"units": total, "laid_out": list(A.shape), "kernel": res["kernel_name"], "norm": norm,
+                     "call_ms": 1e3 * wall, "kernel_ms": res["ms_kernel"], "units_per_s": total / wall}
+    if out["cfg2_LM_40x39_dups_y"]["norm"] != out["cfg2_LM_40x39_dups_n"]["norm"]:
+        raise RuntimeError("the marginal workload's preprocessed and plain runs disagree")
+    return out
+
+
 def batch_full_line(pkg, gen, chunk_labelings: float = 1.2e14) -> dict:
     """The mixed batch workload as drawn, without the 20-row cap on d = 3
     files (m up to 30 rows: 3^29 labelings for the largest item, about
@@ -631,6 +679,9 @@ def run_case(pkg, gen, case: str, steps: int, warmup: int) -> int:
     if case == "wide":
         print(json.dumps({"case": case, **wide_lines(pkg, gen)}), flush=True)
         return 0
+    if case == "configs":
+        print(json.dumps({"case": case, **configs_line(pkg, gen)}), flush=True)
+        return 0
     if case == "columns":
         print(json.dumps({"case": case, **columns_lines(pkg, gen)}), flush=True)
         return 0
@@ -657,7 +708,8 @@ def main():
                     help="run the default launch shapes instead of the autotuner's (tuned before timing)")
     ap.add_argument("--tune", action="store_true",
                     help="sweep the autotuner's candidates on the bench workloads and print them as JSON")
-    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "batch_full", "l3", "int64", "columns", "wide", "int16", "int16_blocked",
+    ap.add_argument("--case", choices=["blocked", "plain", "table16", "batch", "batch_full", "l3", "int64", "columns",
+                                       "configs", "wide", "int16", "int16_blocked",
                                        "int16_plain", "wide_int16", "wide_int16_wide", "l3_int16",
                                        "l3_int16_general"],
                     help="run only this workload (for profiler passes) and print its figures")
@@ -793,6 +845,7 @@ def main():
         line["int64_34x40"] = single_line(pkg, gen.gen_matrix(34, 40, 100, 108).astype("int64") * (2 ** 28 + 3),
                                           1, 2 ** 33, None, int64=True)
         line["L3_24x32"] = l3_line(pkg, gen, a.steps, a.warmup, not a.no_cpu)
+        line["configs"] = configs_line(pkg, gen)
         line["batch_4096"] = batch_line(pkg, gen, max(1, a.steps // 2))
     if not a.no_cpu:
         cb = cpu_baseline(gen)
