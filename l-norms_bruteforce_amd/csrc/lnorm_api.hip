@@ -316,7 +316,10 @@ struct Geometry {
 
 // The lane- or wave-per-segment walks.
 enum Walk { WALK_PLAIN = 0, WALK_PLAIN16 = 1, WALK_GENERAL = 2, WALK_WIDE = 3, WALK_WIDE16 = 4, WALK_TAB16 = 5,
-            WALK_TAB16W = 6, WALK_TAB32 = 7, WALK_TAB32W = 8, WALK_COLS = 9 };
+            WALK_TAB16W = 6, WALK_TAB32 = 7, WALK_TAB32W = 8, WALK_COLS = 9, WALK_COLSD = 10 };
+// WALK_COLS: the column walk's L1/LM form (binary counter, segments up to
+// 2^30); WALK_COLSD: its d-ary form, L2 included (nibble counter, at most
+// 15 digits per segment)
 
 // The tabulated walks in 32-bit words: one column per word (ny_pad = words).
 bool is_tab32(Walk w) { return w == WALK_TAB32 || w == WALK_TAB32W; }
@@ -331,7 +334,7 @@ Geometry geometry_for(int base, int e, int target_log, int forced, Walk walk, in
   if (walk == WALK_GENERAL) qmax = (int)(62.0 / lb);
   else if ((walk == WALK_PLAIN || walk == WALK_WIDE || walk == WALK_COLS) && base == 2) qmax = 30;
   else qmax = std::min(15, (int)(31.0 / lb));
-  if (walk == WALK_COLS) target_log = std::min(target_log, 16);   // one wave (not lane) per segment
+  if (walk == WALK_COLS || walk == WALK_COLSD) target_log = std::min(target_log, 16);   // one wave per segment
   int q = forced > 0 ? forced : std::max(0, e - (int)std::floor(target_log / lb));
   q = std::min(q, std::min(qmax, e));
   if (walk == WALK_WIDE) q = std::max(q, std::min(3, e));   // steps combine in groups of 8
@@ -506,7 +509,7 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
     }
     return;
   }
-  if (walk == WALK_COLS && p.n == 1) {
+  if (walk == WALK_COLS) {
     // the column walk, L1 / LM: A = R x Cp rows of the |.| columns, D = R x Cp
     // rows of 2 A, then the R entries of column 0 (LM's marginal)
     const int marg = p.kind == KIND_LM ? 1 : 0;
@@ -522,7 +525,7 @@ void prepare_walk(const Problem &p, Walk walk, int ny_pad, Prepared &pp) {
     }
     return;
   }
-  const int W = (walk == WALK_PLAIN || walk == WALK_WIDE || walk == WALK_COLS) ? ny_pad : p.C;
+  const int W = (walk == WALK_PLAIN || walk == WALK_WIDE || walk == WALK_COLSD) ? ny_pad : p.C;
   pp.A.assign((size_t)p.R * W, 0);
   for (int i = 0; i < p.R; ++i)
     for (int l = 0; l < p.C; ++l) pp.A[(size_t)i * W + l] = p.A[(size_t)i * p.C + l];
@@ -558,7 +561,8 @@ hipError_t launch_walk(const Problem &p, const Prepared &pp, Mode mode, const Pl
     case WALK_TAB16W: return p.n == 1 ? launch_wide_tb16(p.kind, mode, a, st) : launch_wide_t16(mode, a, st);
     case WALK_TAB32: return p.n == 1 ? launch_tab32_bin(p.kind, mode, a, st, block) : launch_blk32(mode, a, st, block);
     case WALK_TAB32W: return p.n == 1 ? launch_wide_tb32(p.kind, mode, a, st) : launch_wide_t32(mode, a, st);
-    case WALK_COLS: return launch_colwalk(p.kind, mode, a, st);
+    case WALK_COLS:
+    case WALK_COLSD: return launch_colwalk(p.kind, mode, a, st);
     default: return launch_general(p.kind, mode, a, p.C, st);
   }
 }
@@ -1020,7 +1024,7 @@ void prepare_for(const Problem &p, const Choice &c, Prepared &pp) {
   else if (c.kernel == LNORM_KERNEL_TABLE16) prepare_walk(p, WALK_TAB16W, 2 * c.wtabw, pp);
   else if (c.kernel == LNORM_KERNEL_TABLE32 && c.wtab32) prepare_walk(p, WALK_TAB32, c.wtab32, pp);
   else if (c.kernel == LNORM_KERNEL_TABLE32) prepare_walk(p, WALK_TAB32W, c.wtab32w, pp);
-  else if (c.kernel == LNORM_KERNEL_COLUMNS) prepare_walk(p, WALK_COLS, c.cols, pp);
+  else if (c.kernel == LNORM_KERNEL_COLUMNS) prepare_walk(p, p.n == 1 ? WALK_COLS : WALK_COLSD, c.cols, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE && c.w16_wide) prepare_walk(p, WALK_WIDE16, 2 * c.w16_wide, pp);
   else if (c.kernel == LNORM_KERNEL_WIDE) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
   else if (c.kernel == LNORM_KERNEL_BLOCKED && !c.ny_plain && c.ny_wide) prepare_walk(p, WALK_WIDE, c.ny_wide, pp);
@@ -1371,7 +1375,7 @@ int calc_impl(const int32_t *A, int R, int C, int n, int marginal, const lnorm_h
     else if (is_tab32(pp.walk))
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "table32%s<d=%d,words=%d,M=%d,q=%d>",
                pp.walk == WALK_TAB32W ? "w" : "", p.n, pp.ny_pad, pp.qmin, q);
-    else if (pp.walk == WALK_COLS)
+    else if (pp.walk == WALK_COLS || pp.walk == WALK_COLSD)
       snprintf(rr.kernel_name, sizeof rr.kernel_name, "cols<kind=%d,d=%d,cpl=%d,q=%d>", (int)p.kind, p.n,
                pp.ny_pad / 64, q);
     else

@@ -13,12 +13,12 @@
 // matrix once), the ones most steps move also in registers (row 0 for
 // d >= 2, which moves in (d - 1) / d of the steps; rows 0..H-1 for L1/LM):
 // a step takes the changed row's CPL words per lane (from registers, or
-// ds_read_b32, lane-consecutive, no bank conflicts), moves them between the
-// two labels that exchange the row (L1/LM: adds 2 A or -2 A), re-sums those
-// two labels' |.| over the lane's columns, and one butterfly over the wave
-// totals the lanes' partial objectives.
+// ds_read_b32, lane-consecutive, no bank conflicts).  d >= 2: it moves them
+// between the two labels that exchange the row and re-sums those two labels'
+// |.| over the lane's columns; L1/LM: it adds 2 A or -2 A to the one sum and
+// re-sums its |.|.  Six DPP adds total the lanes' partial objectives.
 //   Ops per step and lane: CPL x (2 moves + 2 |.|) for d >= 2, CPL x 2 for
-//   L1/LM, plus the wave total (6 shuffled adds) and the running maximum.
+//   L1/LM, plus the wave total (6 DPP adds) and the running maximum.
 #include "lnorm_colwalk.h"
 #include "lnorm_walks.h"
 

@@ -93,3 +93,18 @@ def test_columns_auto_and_general(pkg):
         g = pkg.calc_lnorm(A, d, 0, kernel="general")
         assert a[:2] == g[:2] and a[2]["best_rank"] == g[2]["best_rank"], (R, C, d)
         assert oracle.objective(A, d, 0, a[1]) == a[0]
+
+
+def test_columns_long_segments(pkg):
+    """Segments at the walks' limits: 2^20 ranks for L1 (binary counter) and,
+    for L2, whose column walk is the d-ary form with 15 counter digits, a
+    requested 2^20 capped at 2^15; both against the general kernel."""
+    rng = np.random.default_rng(750)
+    for n, R, C in ((1, 22, 300), (2, 22, 700)):
+        A = rng.integers(-20000, 20001, size=(R, C)).astype(np.int32)
+        print("columns long segments", n, A.shape, flush=True)
+        c = pkg.calc_lnorm(A, n, 0, kernel="columns", seg_log=20)
+        assert c[2]["kernel_name"].endswith(",q=%d>" % (20 if n == 1 else 15)), c[2]["kernel_name"]
+        g = pkg.calc_lnorm(A, n, 0, kernel="general")
+        assert c[:2] == g[:2] and c[2]["best_rank"] == g[2]["best_rank"], (n, A.shape)
+
