@@ -80,10 +80,10 @@ def valu_ops_per_unit(kernel: str, ny: int, marginal: int, P: int = 2) -> float:
     if kernel == "table32":          # 32-bit words: one v_sad_u32 per padded |.| column, half a v_max3,
         words = next(w for w in TAB32_WORDS if w >= ny - marginal)   # the high step per 2^M patterns
         return words + 0.5 + marginal + words / 2 ** (5 if words <= 128 else 4)
-    if kernel == "table32w":         # over W waves of 32 (64 at 1024) words, 2^M patterns per block
-        words = next(w for w in (256, 320, 384, 512, 1024) if w >= ny - marginal)
-        w, m = {256: (8, 5), 320: (10, 4), 384: (12, 4), 512: (16, 4), 1024: (16, 4)}[words]
-        return words + words / 2 ** m + w / 2 + 1 + marginal
+    if kernel == "table32w":         # over W waves of 32 (64 at 1024) words, 2^M patterns per block;
+        words = next(w for w in (256, 320, 384, 512, 1024) if w >= ny - marginal)   # at 64 words per wave
+        w, m = {256: (8, 5), 320: (10, 4), 384: (12, 4), 512: (16, 4), 1024: (16, 4)}[words]   # two |.|
+        return words + words / 2 ** m + w / 2 + 1 + marginal + (1 if words == 1024 else 0)   # chains joined
     return 2 * ny + 1
 
 

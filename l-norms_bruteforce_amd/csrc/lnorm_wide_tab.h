@@ -305,18 +305,63 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
 // ---------------------------------------------------------------------------
 constexpr int wtb_m(int w) { return w > 8 ? 4 : 5; }   // partials: W * 2^M * 256 B <= 64 KiB
 
-template <int NW, int NWT, int M, bool MARG, int WB, int... Ss>
+// The 2^M patterns' partial sums over the wave's NW words.  The table words
+// arrive by scalar loads, which may return out of order, so a wait for one
+// load waits for all.  The wave reads its 2^M x NW table words in batches of
+// KB (in pattern order); each batch's loads are issued right after the first
+// operation of the batch before it (the one that waits for that batch), so a
+// batch has KB - 1 operations of this wave to arrive in, not the few the
+// compiler's own order leaves.  Scheduling barriers pin that order.
+// NC: interleaved |.| chains; two measured faster only for 32-bit words at
+// 64 words per wave (1024-word shape: 298 against 315 ms on 34 x 1000
+// int16-range entries), one elsewhere (1 to 3 % ahead, one register fewer).
+constexpr int wtb_batch(int nw) { return nw < 32 ? nw : 32; }
+constexpr int wtb_chains(int nw, int wb) { return (wb == 32 && nw >= 64) ? 2 : 1; }
+
+template <int NW, int NWT, int M, bool MARG, int WB, int KB = wtb_batch(NW), int NC = wtb_chains(NW, WB)>
 __device__ __forceinline__ void wtb_partials(const uint32_t (&u)[NW], int32_t u0, const cuint32_t *neg,
-                                             const cint32_t *pos0, bool wave0, uint32_t *p, int lane,
-                                             std::integer_sequence<int, Ss...>) {
-  auto one = [&](auto SS) {
-    constexpr int S = decltype(SS)::value;
-    uint32_t acc = (MARG && wave0) ? (uint32_t)(u0 + pos0[S]) : w_zero<WB>();
+                                             const cint32_t *pos0, bool wave0, uint32_t *p, int lane) {
+  constexpr int NT = (1 << M) * NW;   // table words of this wave
+  constexpr int NB = (NT + KB - 1) / KB;
+  static_assert(KB <= NW, "at most one pattern starts in a batch");
+  uint32_t buf[2][KB];
+  int32_t pn[2] = {0, 0};   // LM: pos0 of the pattern starting in the batch
 #pragma unroll
-    for (int w = 0; w < NW; ++w) acc = w_sad<WB>(u[w], neg[(size_t)S * NWT + w], acc);
-    p[S * 64 + lane] = acc;
-  };
-  (one(std::integral_constant<int, Ss>{}), ...);
+  for (int i = 0; i < KB; ++i) buf[0][i] = neg[(size_t)(i / NW) * NWT + i % NW];
+  if (MARG) pn[0] = pos0[0];
+  uint32_t acc[NC] = {};   // NC interleaved chains (words w = c mod NC)
+#pragma unroll
+  for (int b = 0; b < NB; ++b) {
+#pragma unroll
+    for (int i = 0; i < KB; ++i) {
+      const int f = b * KB + i;
+      if (f < NT) {
+        const int S = f / NW, w = f % NW, c = w % NC;
+        const uint32_t a0 = w >= NC ? acc[c] : (w == 0 && MARG && wave0) ? (uint32_t)(u0 + pn[b & 1]) : w_zero<WB>();
+        acc[c] = w_sad<WB>(u[w], buf[b & 1][i], a0);
+        if (w == NW - 1) {
+          uint32_t t = w_done<WB>(acc[0]);
+#pragma unroll
+          for (int c2 = 1; c2 < NC; ++c2) t += w_done<WB>(acc[c2]);
+          p[S * 64 + lane] = t;
+        }
+      }
+      if (i == 0) {
+        __builtin_amdgcn_sched_barrier(0);
+        if (b + 1 < NB) {
+#pragma unroll
+          for (int j = 0; j < KB; ++j) {
+            const int f1 = (b + 1) * KB + j;
+            if (f1 < NT) buf[(b + 1) & 1][j] = neg[(size_t)(f1 / NW) * NWT + f1 % NW];
+          }
+          const int S1 = ((b + 1) * KB + NW - 1) / NW;   // the pattern starting in batch b + 1
+          if (MARG && S1 < (1 << M) && S1 * NW < (b + 2) * KB) pn[(b + 1) & 1] = pos0[S1];
+        }
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
 }
 
 template <int W, int M, int MODE, bool ODD, int WV, int... Is>
@@ -356,9 +401,18 @@ struct WtbEval {
   }
 };
 
+// Waves per SIMD the compiler must leave room for (at most 512 / that many
+// VGPRs): the workgroups per CU that LDS admits (W * 2^M * 256 B each, 160 KiB
+// per CU), W waves each over 4 SIMDs, where the wave's registers can fit: 16
+// waves of 32 words (64 KiB, 2 per CU: 8 waves, 64 VGPRs; unbounded, the
+// compiler took 71 and one workgroup per CU ran 1.2x slower), 10 of 32
+// (40 KiB, 3 per CU: 8 waves); otherwise what the registers give.
+constexpr int wtb_waves_per_eu(int w, int nw) { return nw == 32 && (w == 16 || w == 10) ? 8 : 1; }
+
 // M: the table rows (wtb_m(W) in both word formats).
 template <int W, int NW, bool MARG, int MODE, int WB, int M>
-__global__ __launch_bounds__(64 * W) void k_wide_tb16(
+__global__ __launch_bounds__(64 * W) __attribute__((amdgpu_waves_per_eu(wtb_waves_per_eu(W, NW))))
+void k_wide_tb16(
     const uint32_t *__restrict__ Ap_, const uint32_t *__restrict__ tab_, int R, int q, uint64_t seg0, uint64_t nseg,
     int32_t target, int32_t *__restrict__ unit_best, int32_t *__restrict__ gbest,
     unsigned long long *__restrict__ gfirst) {
@@ -421,8 +475,7 @@ __global__ __launch_bounds__(64 * W) void k_wide_tb16(
     const cuint32_t *negk = as_const(tab_);
     asm volatile("" : "+s"(negk));
     if constexpr (MODE == MODE_MAX) {
-      wtb_partials<NW, NWT, M, MARG, WB>(u, u0, negk + wbase, pos0, wv == 0, mypart, lane,
-                                         std::make_integer_sequence<int, 1 << M>{});
+      wtb_partials<NW, NWT, M, MARG, WB>(u, u0, negk + wbase, pos0, wv == 0, mypart, lane);
     } else {
       // MODE_FIRST re-walks a single unit: the patterns' partial sums as a
       // loop over the wave-uniform pattern index (half the unrolled code)
@@ -495,9 +548,10 @@ static hipError_t lwtb(Mode mode, const PlainArgs &a, hipStream_t st) {
 // words, 16 of 64) with wtb_m(W) table rows.  Each wave reads its slice of
 // every table row by scalar loads, so the scalar-cache working set of a CU
 // is the whole table: in 32-bit words the 1024-word shape misses in the
-// scalar cache on about half its loads (3 rows), where one wave reading the
-// whole table misses on 2%.  Fewer rows (3 at 1024 words: 0.92x) or 128-word
-// waves (2 to 8 waves, 5 rows: 0.77x) measured slower than these shapes.
+// scalar cache on about half its loads, where one wave reading the whole
+// table misses on 2%; the batched loads of wtb_partials hide most of that
+// latency.  Fewer rows (3 at 1024 words: 0.93x with the batched loads) or
+// 128-word waves (2 to 8 waves, 5 rows: 0.77x) measured slower.
 template <bool MARG, int WB>
 static hipError_t lwtb_m(Mode mode, const PlainArgs &a, hipStream_t st) {
   switch (WB == 16 ? a.ny_pad / 2 : a.ny_pad) {
