@@ -94,6 +94,15 @@ TAB32_WORDS = (8, 16, 24, 32, 40, 48, 64, 80, 96, 128, 160, 192)
 AUTOTUNE = True    # bench.py --no-autotune clears it
 
 
+_T0 = time.perf_counter()
+
+
+def progress(what: str) -> None:
+    """One line on stderr as each workload of the default run ends (stdout
+    keeps the single JSON line)."""
+    print("[bench %6.1f s] %s" % (time.perf_counter() - _T0, what), file=sys.stderr, flush=True)
+
+
 def run_steps(pkg, A, kernel, steps, warmup, P=0, n=1):
     """`steps` timed calls (norm + strategy) after `warmup` untimed ones.
     With autotuning, the launch shape is tuned once before any of them
@@ -746,6 +755,7 @@ def main():
     total = 2 ** (k - 1)
     blk = run_steps(pkg, A, "blocked", a.steps, a.warmup)
     sb = summarize(blk, total, ny, "blocked")
+    progress("headline (blocked) done")
     line = {
         "metric": METRIC,
         "value": sb["value"],
@@ -807,6 +817,7 @@ def main():
         st16 = summarize(tb, total, ny, "table16")
         line["table16_same_matrix"] = {k2: st16[k2] for k2 in ("value", "ms_per_step", "kernel_ms", "valu_TOPs",
                                                               "valu_frac", "alg_frac", "kernel_name")}
+        progress("plain and table16 on the headline matrix done")
         B = gen.gen_matrix(40, 40, 100, 106)
         nb = run_steps(pkg, B, "blocked", a.steps, a.warmup)
         npl = run_steps(pkg, B, "plain", max(1, a.steps // 2), a.warmup)
@@ -822,6 +833,7 @@ def main():
                            "speedup": s1["value"] / s2["value"],
                            "blocked_valu_frac": s1["valu_frac"], "plain_valu_frac": s2["valu_frac"],
                            "table16_valu_frac": s5["valu_frac"], "norm": s1["norm"]}
+        progress("40 x 40 done")
         # 128 x 40 int16 (entries in [-30000, 30000], seed 105), laid out 40 x 128:
         # column sums beyond 2^15, so the blocked path uses 32-bit words
         # (hi/lo int8 MFMA planes for the table, one v_sad_u32 per column)
@@ -840,13 +852,17 @@ def main():
                                 "blocked": s3["value"], "plain": s4["value"], "speedup": s3["value"] / s4["value"],
                                 "blocked_valu_frac": s3["valu_frac"], "plain_valu_frac": s4["valu_frac"],
                                 "kernel": s3["kernel_name"], "norm": s3["norm"]}
+        progress("int16 128 x 40 done")
         line["cli_end_to_end"] = cli_line(pkg, gen)
         line.update(wide_lines(pkg, gen))
+        progress("CLI and wide matrices done")
         line["int64_34x40"] = single_line(pkg, gen.gen_matrix(34, 40, 100, 108).astype("int64") * (2 ** 28 + 3),
                                           1, 2 ** 33, None, int64=True)
         line["L3_24x32"] = l3_line(pkg, gen, a.steps, a.warmup, not a.no_cpu)
+        progress("int64 and L3 done")
         line["configs"] = configs_line(pkg, gen)
         line["batch_4096"] = batch_line(pkg, gen, max(1, a.steps // 2))
+        progress("configs and batch done")
     if not a.no_cpu:
         cb = cpu_baseline(gen)
         line["cpu_baseline"] = {k2: cb[k2] for k2 in ("value", "unit", "cores", "kind", "sample")}
@@ -855,6 +871,7 @@ def main():
             c40 = cpu_baseline(gen, rows=40, r=100, seed=106, budget_cols=34)
             line["n40_m40"]["cpu_baseline"] = {k2: c40[k2] for k2 in ("value", "cores", "kind", "sample")}
             line["n40_m40"]["speedup_vs_cpu"] = line["n40_m40"]["blocked"] / c40["value"]
+        progress("CPU baselines done")
     print(json.dumps(line), flush=True)
     return 0
 
