@@ -1,6 +1,7 @@
 """Concurrent calls into the C ABI from several host threads (ctypes drops
-the GIL during the call): single calls on every kernel family, 64-bit calls,
-a batch and calls that fail, interleaved in shuffled order over a thread
+the GIL during the call): single calls on every kernel family, calls that
+allocate nothing (one static scratch array per device), 64-bit calls, a
+batch and calls that fail, interleaved in shuffled order over a thread
 pool.  Every call must equal the same call made alone (norm, strategy, rank
 and kernel), and each failing call must get its own error text (the
 library keeps it per thread).  Calls on one device serialise on its
@@ -29,6 +30,8 @@ def _tasks():
     for R, C, n, m, kernel, r in SINGLE:
         A = rng.integers(-r, r + 1, size=(R, C)).astype(np.int32)
         tasks.append(("single", (A, n, m, kernel)))
+    for R, C, n in ((20, 40, 1), (12, 200, 2)):   # through lnorm_hip_calc_noalloc
+        tasks.append(("noalloc", (rng.integers(-100, 101, size=(R, C)).astype(np.int32), n, 0)))
     A64 = rng.integers(-100, 101, size=(16, 40)).astype(np.int64) * (2 ** 30 + 7)
     tasks.append(("int64", (A64, 1, 0)))
     probs = [(rng.integers(-50, 51, size=(int(rng.integers(4, 14)), int(rng.integers(2, 40)))).astype(np.int32),
@@ -46,6 +49,10 @@ def _run(pkg, task):
         if kind == "single":
             A, n, m, kernel = args
             norm, strat, res = pkg.calc_lnorm(A, n, m, kernel=kernel)
+            return ("ok", norm, tuple(strat), res["best_rank"], res["kernel_name"])
+        if kind == "noalloc":
+            A, n, m = args
+            norm, strat, res = pkg.calc_lnorm(A, n, m, noalloc=True)
             return ("ok", norm, tuple(strat), res["best_rank"], res["kernel_name"])
         if kind == "int64":
             A, n, m = args
