@@ -756,6 +756,16 @@ def main():
     blk = run_steps(pkg, A, "blocked", a.steps, a.warmup)
     sb = summarize(blk, total, ny, "blocked")
     progress("headline (blocked) done")
+    # the sustained rate of the headline's instruction (v_sad_u16) and of the
+    # 32-bit walks' (v_sad_u32) on this GPU, for the published peak beside it
+    r16, ms16 = pkg.valu_rate("sad_u16")
+    r32, _ = pkg.valu_rate("sad_u32")
+    measured = {"sad_u16_lane_ops_per_s": r16, "sad_u32_lane_ops_per_s": r32,
+                "sustained_clock_ghz": r16 / (256 * 64) / 1e9,
+                "row_update_peak_TOPs": 2 * r16 / 1e12,
+                "frac_of_measured": sb["row_updates_per_s"] / (2 * r16),
+                "probe": "lnorm_hip_valu_rate: 8 independent chains per lane, 8 waves per SIMD, SGPR operand, "
+                         "no memory traffic, one %.1f ms launch after a warm-up launch of half that" % ms16}
     line = {
         "metric": METRIC,
         "value": sb["value"],
@@ -796,6 +806,7 @@ def main():
             "traffic": PROFILED_TRAFFIC.get(sb["kernel_name"]),
             "traffic_per": "launch (2^22 waves at most per launch; one per call at the tuned shape)",
             "kernel_ms": sb["kernel_ms"],
+            "peak_measured": measured,
         },
         "row_updates_per_s": sb["row_updates_per_s"],
         "mfma": mfma_share(blk["res"], k, ny),

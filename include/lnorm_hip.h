@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define LNORM_HIP_ABI_VERSION 6
+#define LNORM_HIP_ABI_VERSION 7
 
 /* error codes */
 #define LNORM_OK 0
@@ -270,6 +270,19 @@ void lnorm_hip_release(void);
  * pseudo-random operands.  Returns the number of mismatches (0 = pass) or a
  * negative LNORM_E_* code. */
 int lnorm_hip_selftest(void);
+
+/* Measurement aid, with no counterpart in the reference (it has no device
+ * code to measure): the sustained rate of one VALU instruction the walks are
+ * built on, on the current device, for the roofline's denominator.
+ *   op 0: v_sad_u16, 1: v_sad_u32 (each with a wave-uniform SGPR operand, as
+ *   in the walks), 2: v_add_u32.  Eight independent chains per lane, 8 waves
+ *   per SIMD, no memory traffic in the loop; one launch of about 60 ms to
+ *   bring the clock to its sustained level, then one timed launch of about
+ *   120 ms.
+ *   *lane_ops_per_s: lane-operations per second (the published peak is
+ *   256 CUs x 64 lanes x 2.4 GHz = 3.93e13); *ms (may be NULL): the timed
+ *   launch.  Returns 0 or a negative LNORM_E_* code. */
+int lnorm_hip_valu_rate(int op, double *lane_ops_per_s, double *ms);
 
 /* Diagnostic: the low-pattern table the blocked kernel would use
  * (P*(words+marginal)*64 words) for the first kl = 6+log2(P) rows of A, in

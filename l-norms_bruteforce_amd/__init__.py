@@ -86,7 +86,7 @@ HIP_SYMBOLS = ["lnorm_hip_calc_Lnorm", "lnorm_hip_calc_ex", "lnorm_hip_calc_batc
                "lnorm_hip_device_count", "lnorm_hip_release", "lnorm_hip_selftest",
                "lnorm_hip_debug_table", "lnorm_hip_calc_batch_ex", "lnorm_hip_autotune",
                "lnorm_hip_autotune_batch", "lnorm_hip_calc_Lnorm64", "lnorm_hip_calc_ex64",
-               "lnorm_hip_calc_noalloc"]
+               "lnorm_hip_calc_noalloc", "lnorm_hip_valu_rate"]
 
 _hip = None
 
@@ -141,6 +141,9 @@ def hip_lib():
         L.lnorm_hip_device_count.restype = ctypes.c_int
         L.lnorm_hip_release.restype = None
         L.lnorm_hip_selftest.restype = ctypes.c_int
+        L.lnorm_hip_valu_rate.restype = ctypes.c_int
+        L.lnorm_hip_valu_rate.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                          ctypes.POINTER(ctypes.c_double)]
         L.lnorm_hip_debug_table.restype = ctypes.c_int
         L.lnorm_hip_debug_table.argtypes = [i32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                             ctypes.c_int, ctypes.POINTER(ctypes.c_uint32), ctypes.c_int]
@@ -323,6 +326,18 @@ def debug_table(A, marginal: int, P: int, word_bits: int = 0):
     n = _check(hip_lib().lnorm_hip_debug_table(a.ctypes.data_as(i32p), R, C, marginal, P, word_bits,
                                                 out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), words))
     return out[:n]
+
+
+VALU_RATE_OPS = {"sad_u16": 0, "sad_u32": 1, "add_u32": 2}
+
+
+def valu_rate(op: str = "sad_u16"):
+    """lnorm_hip_valu_rate: the measured sustained rate of one VALU
+    instruction on the current device.  Returns (lane-ops per second, ms of
+    the timed launch)."""
+    rate, ms = ctypes.c_double(), ctypes.c_double()
+    _check(hip_lib().lnorm_hip_valu_rate(VALU_RATE_OPS[op], ctypes.byref(rate), ctypes.byref(ms)))
+    return rate.value, ms.value
 
 
 def device_count() -> int:

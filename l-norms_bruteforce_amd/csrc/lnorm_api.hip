@@ -2302,6 +2302,38 @@ int lnorm_hip_selftest(void) {
   return bad;
 }
 
+int lnorm_hip_valu_rate(int op, double *lane_ops_per_s, double *ms) {
+  g_err.clear();
+  if (op < 0 || op > 2) return fail(LNORM_E_ARG, "op must be 0 (v_sad_u16), 1 (v_sad_u32) or 2 (v_add_u32)");
+  if (!lane_ops_per_s) return fail(LNORM_E_ARG, "lane_ops_per_s is NULL");
+  lnorm_hip_opts o;
+  lnorm_hip_default_opts(&o);
+  int dev = 0;
+  if (int r = select_device(o, dev)) return r;
+  DevCtx &cx = g_ctx[dev];
+  std::lock_guard<std::mutex> lock(cx.mu);
+  if (int r = ctx_init(cx)) return r;
+  // 8 waves per SIMD (2048 workgroups of 4 waves over 256 CUs), about 120 ms
+  // at the published rate, after a launch of half that length that brings the
+  // clock to its sustained level
+  const int blocks = 2048, iters = 32768;
+  Buf out;
+  if (int r = ensure(out, (size_t)blocks * 256 * 4)) return r;
+  float t = 0;
+  hipError_t e = launch_valu_rate(op, (uint32_t *)out.p, blocks, iters / 2, cx.stream);
+  if (e == hipSuccess) e = hipEventRecord(cx.ev[4], cx.stream);
+  if (e == hipSuccess) e = launch_valu_rate(op, (uint32_t *)out.p, blocks, iters, cx.stream);
+  if (e == hipSuccess) e = hipEventRecord(cx.ev[5], cx.stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(cx.stream);
+  if (e == hipSuccess) e = hipEventElapsedTime(&t, cx.ev[4], cx.ev[5]);
+  (void)hipFree(out.p);
+  if (e != hipSuccess) return fail(LNORM_E_HIP, "VALU rate probe: %s", hipGetErrorString(e));
+  if (t <= 0) return fail(LNORM_E_HIP, "VALU rate probe: no elapsed time");
+  *lane_ops_per_s = (double)valu_rate_lane_ops(blocks, iters) / (t / 1e3);
+  if (ms) *ms = t;
+  return 0;
+}
+
 int lnorm_hip_debug_table(const int32_t *A, int R, int C, int marginal, int P, int word_bits, uint32_t *out,
                           int out_words) {
   g_err.clear();

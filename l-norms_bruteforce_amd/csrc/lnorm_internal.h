@@ -213,5 +213,9 @@ hipError_t launch_first_unit(const int32_t *unit_best, uint64_t n, const int32_t
                              unsigned long long *first, hipStream_t st);
 hipError_t launch_sad_probe(const uint32_t *a, const uint32_t *b, const uint32_t *c, uint32_t *out, int n,
                             hipStream_t st);
+// lnorm_peak.hip: the VALU rate probe (op 0: v_sad_u16, 1: v_sad_u32, 2:
+// v_add_u32; 256-thread workgroups) and the lane-ops one launch issues
+hipError_t launch_valu_rate(int op, uint32_t *out, int blocks, int iters, hipStream_t st);
+uint64_t valu_rate_lane_ops(int blocks, int iters);
 
 }  // namespace lnorm

@@ -1,6 +1,8 @@
 """Parity of the HIP engine with the oracle and the reference's golden
 vectors (bit-exact norms, strategies and maximising Gray ranks).  GPU only:
 every call goes through the C ABI of lib/liblnorm_hip.so."""
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -38,6 +40,28 @@ def _agree(eng, A, n, marg, **kw):
 def test_selftest(pkg):
     print("selftest", flush=True)
     assert pkg.selftest() == 0, pkg.last_error()
+
+
+def test_valu_rate_probe(pkg):
+    """The measured rate of the walks' |.|-accumulates (v_sad_u16, v_sad_u32)
+    lies below the published int32 VALU peak (256 CUs x 64 lanes x 2.4 GHz)
+    and above half of it, the two at the same rate; v_add_u32 runs at least
+    at that rate (it measured faster: VOP2 adds are not bound by the 64 lanes
+    per clock of the published figure); a bad op is an error."""
+    peak = 256 * 64 * 2.4e9
+    rates = {}
+    for op in ("sad_u16", "sad_u32", "add_u32"):
+        print("valu rate", op, flush=True)
+        rates[op], ms = pkg.valu_rate(op)
+        print("  %.4e lane-ops/s, %.2f ms" % (rates[op], ms), flush=True)
+    for op in ("sad_u16", "sad_u32"):
+        assert 0.5 * peak < rates[op] < 1.01 * peak, rates
+    assert abs(rates["sad_u16"] / rates["sad_u32"] - 1) < 0.03, rates
+    assert rates["add_u32"] > 0.97 * rates["sad_u32"], rates
+    with pytest.raises(KeyError):
+        pkg.valu_rate("mul")
+    rc = pkg.hip_lib().lnorm_hip_valu_rate(3, ctypes.byref(ctypes.c_double()), None)
+    assert rc == -1, rc
 
 
 def test_golden_vectors_all_kernels(pkg, golden):
