@@ -27,11 +27,13 @@ INC = os.path.join(ROOT, "include")
 
 ARCH = os.environ.get("LNORM_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", shutil.which("hipcc") or "/opt/rocm/bin/hipcc")
-HIP_SOURCES = ["lnorm_plain.hip", "lnorm_blocked.hip", "lnorm_general.hip", "lnorm_batch.hip", "lnorm_batch_tab.hip",
-               "lnorm_batch_tab_hi.hip", "lnorm_int64.hip", "lnorm_wide.hip", "lnorm_dary.hip", "lnorm_dary_hi.hip",
-               "lnorm_tab_bin.hip", "lnorm_wide_tab.hip", "lnorm_dary32.hip", "lnorm_dary32_hi.hip",
-               "lnorm_tab32_bin.hip", "lnorm_wide_tab32.hip", "lnorm_wide_tb32.hip", "lnorm_colwalk.hip", "lnorm_peak.hip",
-               "lnorm_api.hip"]
+# the units in decreasing compile time (the pool starts them in this order, so
+# the longest ones do not wait for a free worker)
+HIP_SOURCES = ["lnorm_wide_tab32.hip", "lnorm_dary32.hip", "lnorm_dary_hi.hip", "lnorm_wide_tab32_hi.hip",
+               "lnorm_dary32_hi.hip", "lnorm_dary.hip", "lnorm_wide_tab.hip", "lnorm_plain.hip",
+               "lnorm_blocked.hip", "lnorm_general.hip", "lnorm_batch.hip", "lnorm_batch_tab.hip",
+               "lnorm_batch_tab_hi.hip", "lnorm_int64.hip", "lnorm_wide.hip", "lnorm_tab_bin.hip",
+               "lnorm_tab32_bin.hip", "lnorm_wide_tb32.hip", "lnorm_colwalk.hip", "lnorm_peak.hip", "lnorm_api.hip"]
 # -simplifycfg-sink-common=false: the d-ary walks branch (uniformly) into one
 # move per (from, to) label pair; sinking those near-identical bodies into a
 # shared tail would index the label-vector array at run time and push it into

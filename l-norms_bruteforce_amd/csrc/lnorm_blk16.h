@@ -94,8 +94,8 @@ hipError_t launch_blk16_hi(Mode mode, const PlainArgs &a, hipStream_t st, int bl
 int blocks_per_cu_blk16_lo(int d, int words, int block);
 int blocks_per_cu_blk16_hi(int d, int words, int block);
 // the 32-bit walk's share (lnorm_dary32.hip)
-hipError_t launch_blk32_lo(Mode mode, const PlainArgs &a, hipStream_t st, int block);   // d = 2..5
-hipError_t launch_blk32_hi(Mode mode, const PlainArgs &a, hipStream_t st, int block);   // d = 6..10
+hipError_t launch_blk32_lo(Mode mode, const PlainArgs &a, hipStream_t st, int block);   // d = 2..4
+hipError_t launch_blk32_hi(Mode mode, const PlainArgs &a, hipStream_t st, int block);   // d = 5..10
 int blocks_per_cu_blk32_lo(int d, int words, int block);
 int blocks_per_cu_blk32_hi(int d, int words, int block);
 

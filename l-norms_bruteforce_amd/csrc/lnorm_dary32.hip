@@ -2,7 +2,8 @@
 // with WB = 32): one column per word, biased by 2^31, v_sad_u32.  It serves
 // int32 inputs whose column sums exceed the 16-bit walks' bound (32767), at
 // the same table rows per word count as the 16-bit walk, so at half its
-// columns.  Widths, the launch entry and the kernels for d = 2..10.
+// columns.  Widths, the launch entry and the kernels for d = 2..4 (d = 5..10 in
+// lnorm_dary32_hi.hip, so the two halves compile in parallel).
 #include "lnorm_blk16.h"
 
 namespace lnorm {
@@ -22,7 +23,6 @@ hipError_t launch_blk32_lo(Mode mode, const PlainArgs &a, hipStream_t st, int bl
     case 2: return launch_blk<2, 32>(mode, a, st, block);
     case 3: return launch_blk<3, 32>(mode, a, st, block);
     case 4: return launch_blk<4, 32>(mode, a, st, block);
-    case 5: return launch_blk<5, 32>(mode, a, st, block);
     default: return hipErrorInvalidValue;
   }
 }
@@ -32,13 +32,12 @@ int blocks_per_cu_blk32_lo(int d, int words, int block) {
     case 2: return blocks_per_cu_blk<2, 32>(words, block);
     case 3: return blocks_per_cu_blk<3, 32>(words, block);
     case 4: return blocks_per_cu_blk<4, 32>(words, block);
-    case 5: return blocks_per_cu_blk<5, 32>(words, block);
     default: return 0;
   }
 }
 
 int blk32_blocks_per_cu(int d, int words, int block) {
-  return d <= 5 ? blocks_per_cu_blk32_lo(d, words, block) : blocks_per_cu_blk32_hi(d, words, block);
+  return d <= 4 ? blocks_per_cu_blk32_lo(d, words, block) : blocks_per_cu_blk32_hi(d, words, block);
 }
 
 hipError_t launch_blk32(Mode mode, const PlainArgs &a, hipStream_t st, int block) {
@@ -47,7 +46,7 @@ hipError_t launch_blk32(Mode mode, const PlainArgs &a, hipStream_t st, int block
   const int words = a.ny_pad;
   if (blk32_words(a.d, a.ny_pad) != words) return hipErrorInvalidValue;
   if (a.q < blk16_low_rows(a.d, words) || a.R - 1 < a.q) return hipErrorInvalidValue;
-  return a.d <= 5 ? launch_blk32_lo(mode, a, st, block) : launch_blk32_hi(mode, a, st, block);
+  return a.d <= 4 ? launch_blk32_lo(mode, a, st, block) : launch_blk32_hi(mode, a, st, block);
 }
 
 }  // namespace lnorm

@@ -615,6 +615,11 @@ inline int wt_rows_for(int d, int words) {
   return nw > 0 ? wt16_m(d, words / nw) : 0;
 }
 
+// lnorm_wide_tab32_hi.hip: the 32-bit W-wave d-ary walk for d = 5..10 (the
+// d = 2..4 kernels and the launch entry are in lnorm_wide_tab32.hip, so the
+// two halves compile in parallel)
+hipError_t launch_wide_t32_hi(Mode mode, const PlainArgs &a, hipStream_t st);
+
 template <int WB>
 static hipError_t lwt_any(Mode mode, const PlainArgs &a, hipStream_t st) {
   switch (a.d) {
