@@ -67,7 +67,8 @@ typedef struct lnorm_hip_opts {
     int32_t wave_log2;          /* blocked: 2^wave_log2 high ranks per wave; 0 = auto */
     int32_t block;              /* plain kernels: threads per workgroup, 64..256 in steps of 64; else 256 */
     int32_t device;             /* HIP device ordinal; -1 = current device */
-    int32_t word_bits;          /* blocked: 16 (two halves per word) or 32; 0 = 16 when column sums allow */
+    int32_t word_bits;          /* blocked: 16 (two halves per word) or 32; 0 = 16 when column sums allow;
+                                   the long long variant: 64 = integer walk, 0 = doubles while exact */
     void *stream;               /* hipStream_t to run on; NULL = the library's own stream */
     int32_t batch_graph;        /* batches: capture the launch sequence in a HIP graph and replay it
                                    while the batch's structure (groups, sizes) stays the same */
@@ -160,7 +161,11 @@ int lnorm_hip_calc_noalloc(const int32_t *mtx_as_vec, int iRows, int iCols, int 
  * norm in int64, every sum exact while sum |A| <= INT64_MAX (else
  * LNORM_E_BOUND).  Problems inside the int32 contract run on the int32
  * kernels (the same result); the rest on 64-bit walks (no blocked or packed
- * 16-bit counterpart: opts->kernel may be AUTO, PLAIN or GENERAL).
+ * 16-bit counterpart: opts->kernel may be AUTO, PLAIN or GENERAL).  The
+ * plain L1 / LM / L2 walk runs in doubles while sum |A| < 2^52, where every
+ * value is an integer below 2^53 and each add is exact (kernel name
+ * plainf64<...>); opts->word_bits = 64 keeps it on 64-bit integers
+ * (plain64<...>), 0 chooses; other values are LNORM_E_ARG.
  * res->Lnorm holds the norm only when it fits int32 (else 0); *Lnorm always.
  */
 int lnorm_hip_calc_Lnorm64(const int64_t *mtx_as_vec, int iRows, int iCols, int n, int marginal,

@@ -1899,8 +1899,8 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
 // ---- the int_type = long long variant (README.md:100-105) -------------------
 // Problems inside the int32 contract run on the int32 kernels (identical
 // results); the rest on the 64-bit walks of lnorm_int64.hip.
-int run64_phase(DevCtx &cx, Kind kind, int d, int ny, bool general, Mode mode, const Geometry &g, uint64_t seg0,
-                uint64_t nseg, int R, int64_t target, hipStream_t st, double &ms_kernel, int64_t &best,
+int run64_phase(DevCtx &cx, Kind kind, int d, int ny, bool general, bool fp64, Mode mode, const Geometry &g,
+                uint64_t seg0, uint64_t nseg, int R, int64_t target, hipStream_t st, double &ms_kernel, int64_t &best,
                 uint64_t &first) {
   long long *gbest = (long long *)cx.scal.p;
   unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
@@ -1920,6 +1920,7 @@ int run64_phase(DevCtx &cx, Kind kind, int d, int ny, bool general, Mode mode, c
   a.gbest = gbest;
   a.gfirst = gfirst;
   a.state = (uint64_t *)cx.state64.p;
+  a.fp64 = fp64 ? 1 : 0;
   HIPCHK(hipEventRecord(cx.ev[4], st));
   hipError_t e = general ? launch_general64(kind, mode, a, ny, st) : launch_plain64(kind, mode, a, st);
   HIPCHK(hipEventRecord(cx.ev[5], st));
@@ -1993,6 +1994,11 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
     return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
   const bool general = kernel == LNORM_KERNEL_GENERAL;
   const int ny = general ? C : ny_plain;
+  // the binary walk in doubles while sum |A| < 2^52 (every value an exact
+  // integer below 2^53); word_bits 64 keeps the 64-bit integer walk
+  if (o.word_bits != 0 && o.word_bits != 64)
+    return fail(LNORM_E_ARG, "word_bits must be 0 (auto) or 64 (integer walk) for the long long variant");
+  const bool fp64 = !general && p.n <= 2 && o.word_bits != 64 && s < ((unsigned __int128)1 << 52);
 
   // matrix padded to the kernel width and the binary delta table
   std::vector<int64_t> hA((size_t)R * ny, 0), hD;
@@ -2022,6 +2028,10 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
   if (int r = ensure(cx.scal, 256)) return r;
   if (general)
     if (int r = ensure(cx.state64, (size_t)kGeneralSlices * nvec * C * 8)) return r;
+  if (fp64) {   // the same layouts as doubles (exact: every value is below 2^53 in magnitude)
+    for (int64_t &v : hA) { const double x = (double)v; std::memcpy(&v, &x, 8); }
+    for (int64_t &v : hD) { const double x = (double)v; std::memcpy(&v, &x, 8); }
+  }
   HIPCHK(hipMemcpyAsync(cx.A64.p, hA.data(), hA.size() * 8, hipMemcpyHostToDevice, st));
   if (!hD.empty()) HIPCHK(hipMemcpyAsync(cx.D64.p, hD.data(), hD.size() * 8, hipMemcpyHostToDevice, st));
 
@@ -2044,8 +2054,8 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
       const uint64_t nch = std::min(kUnitsPerChunk, run.n - c0);
       int64_t b;
       uint64_t f;
-      if (int r = run64_phase(cx, p.kind, p.n, ny, general, MODE_MAX, gr, run.u0 + c0, nch, R, 0, st, ms_kernel, b,
-                              f))
+      if (int r = run64_phase(cx, p.kind, p.n, ny, general, fp64, MODE_MAX, gr, run.u0 + c0, nch, R, 0, st,
+                              ms_kernel, b, f))
         return r;
       if (first_unit == ~0ull || b > best) {
         best = b;
@@ -2062,8 +2072,8 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
     double ms2 = 0;
     int64_t b;
     uint64_t rank;
-    if (int r = run64_phase(cx, p.kind, p.n, ny, general, MODE_FIRST, g2, first_unit * g2.nseg, g2.nseg, R, best, st,
-                            ms2, b, rank))
+    if (int r = run64_phase(cx, p.kind, p.n, ny, general, fp64, MODE_FIRST, g2, first_unit * g2.nseg, g2.nseg, R,
+                            best, st, ms2, b, rank))
       return r;
     if (rank == ~0ull) return fail(LNORM_E_HIP, "no rank attains the maximum %lld", (long long)best);
     rr.ms_strategy = ms2;
@@ -2077,8 +2087,8 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
   rr.ms_enumerate = ms;
   rr.ms_kernel = ms_kernel;
   rr.Lnorm = (best >= INT32_MIN && best <= INT32_MAX) ? (int32_t)best : 0;
-  snprintf(rr.kernel_name, sizeof rr.kernel_name, "%s64<kind=%d,d=%d,ny=%d,q=%d>", general ? "general" : "plain",
-           (int)p.kind, p.n, ny, g.q);
+  snprintf(rr.kernel_name, sizeof rr.kernel_name, "%s64<kind=%d,d=%d,ny=%d,q=%d>",
+           general ? "general" : fp64 ? "plainf" : "plain", (int)p.kind, p.n, ny, g.q);
   rr.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   *Lnorm = best;
   if (res) *res = rr;

@@ -15,6 +15,7 @@
 // MFMA-blocked path has no 64-bit counterpart.
 //
 //   k_plain64_bin<KIND, NY>  L1 / LM / L2, NY up to 64 (L2: 32) columns
+//   k_plainf64_bin<KIND, NY> the same walk in doubles, for sum |A| < 2^52
 //   k_plain64_dary<D, NY>    L3..L10, d * NY <= 80 words of state
 //   k_general64              any width: wave per segment, columns over lanes
 #include "lnorm_device.h"
@@ -148,6 +149,105 @@ __global__ __launch_bounds__(256) void k_plain64_bin(const int64_t *__restrict__
     eval(t);
   }
   finish64<MODE>(active, local, best, (seg << q) + first, first != 0xFFFFFFFFu, unit_best, gbest, gfirst);
+}
+
+// ---------------------------------------------------------------------------
+// Binary family in double precision, for sum |A| < 2^52: every entry, delta
+// (f A, f <= 2), partial sum and objective is then an integer of magnitude
+// below 2^53, which a double holds exactly, and every add of two of them
+// whose result is such an integer is exact, so the walk computes the same
+// integers as k_plain64_bin.  A row update is one v_add_f64 and |y| + acc one
+// v_add_f64 with the |.| source modifier: 2 ops per row-update against about
+// 9 for the 64-bit integer walk.  A and D hold the doubles of the same
+// layout (the host converts them).
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(4))) const double cdouble_t;
+
+template <int KIND, int NY, int MODE>
+__global__ __launch_bounds__(256) void k_plainf64_bin(const double *__restrict__ A_, const double *__restrict__ D_,
+                                                      int R, int q, uint64_t seg0, uint64_t nseg, int64_t target,
+                                                      int64_t *__restrict__ unit_best, long long *__restrict__ gbest,
+                                                      unsigned long long *__restrict__ gfirst) {
+  const cdouble_t *A = (const cdouble_t *)(uintptr_t)A_, *Dt = (const cdouble_t *)(uintptr_t)D_;
+  const uint64_t w = global_wave();
+  const int lane = wave_lane();
+  if (((w >> 1) << 7) >= nseg) return;
+  const uint64_t local = lane_segment(w, lane);
+  const bool active = local < nseg;
+  const uint64_t seg = seg0 + local;
+  const uint32_t par = (uint32_t)__builtin_amdgcn_readfirstlane((int)((seg0 ^ w) & 1));
+
+  constexpr bool TWO = (KIND == KIND_L2);
+  double y[NY];
+  double z[TWO ? NY : 1];
+  const uint64_t N0 = seg << q;
+  const uint64_t g0 = N0 ^ (N0 >> 1);
+#pragma unroll
+  for (int l = 0; l < NY; ++l) y[l] = A[(size_t)(R - 1) * NY + l];
+  if constexpr (TWO) {
+#pragma unroll
+    for (int l = 0; l < NY; ++l) z[l] = 0.0;
+  }
+  for (int i = 0; i < R - 1; ++i) {
+    const bool g = (g0 >> i) & 1;
+    const cdouble_t *row = A + (size_t)i * NY;
+#pragma unroll
+    for (int l = 0; l < NY; ++l) {
+      const double a = row[l];
+      if constexpr (TWO) {
+        z[l] += g ? a : 0.0;
+        y[l] += g ? 0.0 : a;
+      } else {
+        y[l] += g ? a : -a;
+      }
+    }
+  }
+
+  const double dtarget = (double)target;
+  double best = -__builtin_inf();   // eval(0) below makes it finite
+  uint32_t first = 0xFFFFFFFFu;
+  auto eval = [&](uint32_t t) {
+    double acc;
+    if constexpr (KIND == KIND_LM) {
+      acc = y[0];
+#pragma unroll
+      for (int l = 1; l < NY; ++l) acc += __builtin_fabs(y[l]);
+    } else {
+      acc = 0.0;
+#pragma unroll
+      for (int l = 0; l < NY; ++l) {
+        acc += __builtin_fabs(y[l]);
+        if constexpr (TWO) acc += __builtin_fabs(z[l]);
+      }
+    }
+    if constexpr (MODE == MODE_MAX) {
+      best = __builtin_fmax(best, acc);
+    } else {
+      const uint32_t c = (acc == dtarget) ? t : 0xFFFFFFFFu;
+      first = c < first ? c : first;
+    }
+  };
+
+  const uint32_t nsteps = 1u << q;
+  const uint32_t nu = par << q;
+  eval(0);
+  for (uint32_t t = 1; t < nsteps; ++t) {
+    const int j = __builtin_ctz(t);
+    const uint32_t up = (((nu | t) >> (j + 1)) & 1u) ^ 1u;
+    const cdouble_t *dp = Dt + (size_t)(up * (uint32_t)(R - 1) + (uint32_t)j) * NY;
+#pragma unroll
+    for (int l = 0; l < NY; ++l) {
+      const double dl = dp[l];
+      if constexpr (TWO) {
+        z[l] += dl;
+        y[l] -= dl;
+      } else {
+        y[l] += dl;
+      }
+    }
+    eval(t);
+  }
+  finish64<MODE>(active, local, (int64_t)best, (seg << q) + first, first != 0xFFFFFFFFu, unit_best, gbest, gfirst);
 }
 
 // ---------------------------------------------------------------------------
@@ -460,6 +560,20 @@ static hipError_t l64_bin(Mode mode, const Plain64Args &a, hipStream_t st) {
   return hipGetLastError();
 }
 
+template <int KIND, int NY>
+static hipError_t lf64_bin(Mode mode, const Plain64Args &a, hipStream_t st) {
+  const uint64_t waves = 2 * ((a.nseg + 127) / 128);
+  const dim3 grid((unsigned)((waves + 3) / 4)), blk(256);
+  const double *A = (const double *)a.A, *D = (const double *)a.D;
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_plainf64_bin<KIND, NY, MODE_MAX>), grid, blk, 0, st, A, D, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
+  else
+    hipLaunchKernelGGL((k_plainf64_bin<KIND, NY, MODE_FIRST>), grid, blk, 0, st, A, D, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
+  return hipGetLastError();
+}
+
 template <int D, int NY>
 static hipError_t l64_dary(Mode mode, const Plain64Args &a, hipStream_t st) {
   const uint64_t waves = 2 * ((a.nseg + 127) / 128);
@@ -503,6 +617,33 @@ static hipError_t l64_dary_w(Mode mode, const Plain64Args &a, hipStream_t st) {
 hipError_t launch_plain64(Kind kind, Mode mode, const Plain64Args &a, hipStream_t st) {
   if (!a.A || (kind != KIND_LD && !a.D) || !a.unit_best || !a.gbest || !a.gfirst) return hipErrorInvalidValue;
   if (plain64_ny_pad(kind, a.d, a.ny_pad) != a.ny_pad) return hipErrorInvalidValue;
+  if (a.fp64) {   // the double-precision binary walk (A and D hold doubles)
+    switch (kind) {
+      case KIND_L1:
+        switch (a.ny_pad) {
+#define X(n) case n: return lf64_bin<KIND_L1, n>(mode, a, st);
+          X(8) X(16) X(24) X(32) X(40) X(48) X(64)
+#undef X
+        }
+        break;
+      case KIND_LM:
+        switch (a.ny_pad) {
+#define X(n) case n: return lf64_bin<KIND_LM, n>(mode, a, st);
+          X(8) X(16) X(24) X(32) X(40) X(48) X(64)
+#undef X
+        }
+        break;
+      case KIND_L2:
+        switch (a.ny_pad) {
+#define X(n) case n: return lf64_bin<KIND_L2, n>(mode, a, st);
+          X(8) X(16) X(24) X(32)
+#undef X
+        }
+        break;
+      default: break;
+    }
+    return hipErrorInvalidValue;
+  }
   switch (kind) {
     case KIND_L1:
       switch (a.ny_pad) {

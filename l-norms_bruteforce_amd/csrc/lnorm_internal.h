@@ -50,6 +50,7 @@ struct Plain64Args {
   long long *gbest;        // MAX mode: atomic max
   unsigned long long *gfirst;
   uint64_t *state;         // general kernel: per-wave scratch, kGeneralSlices * nvec * ny words
+  int fp64;                // binary walk: A and D hold doubles (sum |A| < 2^52), k_plainf64_bin
 };
 
 // The general kernel runs at most this many waves at once (its scratch size).

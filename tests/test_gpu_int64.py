@@ -1,7 +1,9 @@
 """The int_type = long long variant (lnorm_hip_calc_ex64): 64-bit matrices
 whose sums exceed int32, against the oracle's 64-bit walks (norm, strategy
-and maximising rank), the scaling identity on the GPU, delegation to the
-int32 kernels inside the int32 bound, and error codes.  GPU only."""
+and maximising rank) on both binary walks (doubles below sum |A| = 2^52,
+64-bit integers above it or on request), the bound between them, the
+scaling identity on the GPU, delegation to the int32 kernels inside the
+int32 bound, and error codes.  GPU only."""
 import numpy as np
 import pytest
 
@@ -39,6 +41,11 @@ def test_int64_vs_oracle(pkg, n, marg, R, cols):
             norm, strat, res = pkg.calc_lnorm64(A, n, marg)
             assert (norm, strat, res["best_rank"]) == want, (n, marg, A.shape, bits, res["kernel_name"])
             assert "64<" in res["kernel_name"]
+            if res["kernel_name"].startswith("plainf64"):
+                # the same problem on the 64-bit integer walk
+                norm, strat, res = pkg.calc_lnorm64(A, n, marg, word_bits=64)
+                assert (norm, strat, res["best_rank"]) == want, (n, marg, A.shape, bits, res["kernel_name"])
+                assert res["kernel_name"].startswith("plain64<"), res["kernel_name"]
 
 
 def test_int64_general_equals_plain(pkg):
@@ -48,7 +55,43 @@ def test_int64_general_equals_plain(pkg):
         p = pkg.calc_lnorm64(A, n, 0, kernel="plain")
         g = pkg.calc_lnorm64(A, n, 0, kernel="general")
         assert p[:2] == g[:2] and p[2]["best_rank"] == g[2]["best_rank"]
-        assert p[2]["kernel_name"].startswith("plain64") and g[2]["kernel_name"].startswith("general64")
+        assert p[2]["kernel_name"].startswith("plainf64" if n <= 2 else "plain64"), p[2]["kernel_name"]
+        assert g[2]["kernel_name"].startswith("general64")
+
+
+def _sum_exactly(rng, R, C, total):
+    """An R x C int64 matrix of random signs whose sum |A| is exactly total."""
+    A = np.full((R, C), total // (R * C), np.int64)
+    A[-1, -1] += total - int(A.sum())
+    A *= rng.choice(np.array([-1, 1], np.int64), size=(R, C))
+    assert int(np.abs(A).sum()) == total
+    return A
+
+
+@pytest.mark.parametrize("n,marg", [(1, 0), (1, 1), (2, 0)])
+def test_int64_double_walk_bound(pkg, n, marg):
+    """The binary walk runs in doubles up to sum |A| = 2^52 - 1 and on 64-bit
+    integers from 2^52 on; both agree with the oracle on either side, and
+    with each other, including random matrices whose entries differ in
+    magnitude."""
+    rng = np.random.default_rng(620 + 10 * n + marg)
+    for total, name in ((2 ** 52 - 1, "plainf64<"), (2 ** 52, "plain64<"), (2 ** 52 + 12345, "plain64<")):
+        A = _sum_exactly(rng, 11, 9, total)
+        A[rng.integers(0, 11), :] //= 3   # rows of different magnitude
+        A[-1, -1] += (1 if A[-1, -1] >= 0 else -1) * (total - int(np.abs(A).sum()))
+        assert int(np.abs(A).sum()) == total
+        print("int64 bound", n, marg, total, flush=True)
+        want = oracle.calc_lnorm64(A, n, marg, 8)
+        norm, strat, res = pkg.calc_lnorm64(A, n, marg)
+        assert (norm, strat, res["best_rank"]) == want, (total, res["kernel_name"])
+        assert res["kernel_name"].startswith(name), (total, res["kernel_name"])
+    A = rng.integers(-(2 ** 40), 2 ** 40, size=(26, 30)).astype(np.int64)   # sum |A| about 2^49
+    print("int64 double vs integer", n, marg, A.shape, flush=True)
+    f = pkg.calc_lnorm64(A, n, marg)
+    i = pkg.calc_lnorm64(A, n, marg, word_bits=64)
+    assert f[2]["kernel_name"].startswith("plainf64<") and i[2]["kernel_name"].startswith("plain64<")
+    assert f[:2] == i[:2] and f[2]["best_rank"] == i[2]["best_rank"]
+    assert oracle.objective64(A, n, marg, f[1]) == f[0]
 
 
 def test_int64_scaling_on_gpu(pkg):
@@ -81,4 +124,7 @@ def test_int64_errors(pkg):
     assert e.value.code == -5
     with pytest.raises(pkg.LnormError) as e:
         pkg.calc_lnorm64(np.full((12, 3), 2 ** 40, np.int64), 2, 1)
+    assert e.value.code == -1
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm64(np.full((12, 3), 2 ** 40, np.int64), 1, 0, word_bits=16)
     assert e.value.code == -1
