@@ -1994,11 +1994,11 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
     return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
   const bool general = kernel == LNORM_KERNEL_GENERAL;
   const int ny = general ? C : ny_plain;
-  // the binary walk in doubles while sum |A| < 2^52 (every value an exact
-  // integer below 2^53); word_bits 64 keeps the 64-bit integer walk
+  // the plain walks in doubles while sum |A| < 2^52 (every value an exact
+  // integer below 2^53); word_bits 64 keeps the 64-bit integer walks
   if (o.word_bits != 0 && o.word_bits != 64)
     return fail(LNORM_E_ARG, "word_bits must be 0 (auto) or 64 (integer walk) for the long long variant");
-  const bool fp64 = !general && p.n <= 2 && o.word_bits != 64 && s < ((unsigned __int128)1 << 52);
+  const bool fp64 = !general && o.word_bits != 64 && s < ((unsigned __int128)1 << 52);
 
   // matrix padded to the kernel width and the binary delta table
   std::vector<int64_t> hA((size_t)R * ny, 0), hD;

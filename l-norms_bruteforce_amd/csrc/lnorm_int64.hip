@@ -16,7 +16,8 @@
 //
 //   k_plain64_bin<KIND, NY>  L1 / LM / L2, NY up to 64 (L2: 32) columns
 //   k_plainf64_bin<KIND, NY> the same walk in doubles, for sum |A| < 2^52
-//   k_plain64_dary<D, NY>    L3..L10, d * NY <= 80 words of state
+//   k_plain64_dary<D, NY, V> L3..L10, d * NY <= 80 words of state (V = double
+//                            while sum |A| < 2^52, else uint64_t)
 //   k_general64              any width: wave per segment, columns over lanes
 #include "lnorm_device.h"
 
@@ -24,6 +25,7 @@ namespace lnorm {
 
 typedef __attribute__((address_space(4))) const int64_t cint64_t;
 __device__ __forceinline__ const cint64_t *as_const64(const int64_t *p) { return (const cint64_t *)(uintptr_t)p; }
+typedef __attribute__((address_space(4))) const double cdouble_t;
 
 __device__ __forceinline__ int64_t abs64(int64_t v) { return v < 0 ? -v : v; }
 
@@ -161,8 +163,6 @@ __global__ __launch_bounds__(256) void k_plain64_bin(const int64_t *__restrict__
 // 9 for the 64-bit integer walk.  A and D hold the doubles of the same
 // layout (the host converts them).
 // ---------------------------------------------------------------------------
-typedef __attribute__((address_space(4))) const double cdouble_t;
-
 template <int KIND, int NY, int MODE>
 __global__ __launch_bounds__(256) void k_plainf64_bin(const double *__restrict__ A_, const double *__restrict__ D_,
                                                       int R, int q, uint64_t seg0, uint64_t nseg, int64_t target,
@@ -252,60 +252,88 @@ __global__ __launch_bounds__(256) void k_plainf64_bin(const double *__restrict__
 
 // ---------------------------------------------------------------------------
 // d-ary family: reflected d-ary Gray code (L_OpenMP.c:126-239); the step
-// counter and direction logic of the int32 walk, with 64-bit label vectors.
+// counter and direction logic of the int32 walk, with 64-bit label vectors:
+// V = uint64_t (two's-complement adds, |.| of the signed value) or, while
+// sum |A| < 2^52, V = double (every value an exact integer below 2^53, as in
+// k_plainf64_bin; |.| a source modifier of the add).
 // ---------------------------------------------------------------------------
-template <int F, int T, int D, int NY>
-__device__ __forceinline__ void apply_move64(uint64_t (&t)[D][NY], int64_t (&S)[D], const cint64_t *row) {
-  int64_t sf = 0, st = 0;
+template <class V>
+struct Num64;
+template <>
+struct Num64<uint64_t> {
+  typedef int64_t Sum;
+  typedef cint64_t CV;
+  __device__ __forceinline__ static Sum mag(uint64_t x) { return abs64((int64_t)x); }
+  __device__ __forceinline__ static Sum max(Sum a, Sum b) { return a > b ? a : b; }
+  __device__ __forceinline__ static uint64_t masked(uint64_t v, uint64_t m) { return v & m; }
+};
+template <>
+struct Num64<double> {
+  typedef double Sum;
+  typedef cdouble_t CV;
+  __device__ __forceinline__ static Sum mag(double x) { return __builtin_fabs(x); }
+  __device__ __forceinline__ static Sum max(Sum a, Sum b) { return __builtin_fmax(a, b); }
+  // v or +0.0, by the bits (exact)
+  __device__ __forceinline__ static double masked(double v, uint64_t m) {
+    return __builtin_bit_cast(double, __builtin_bit_cast(uint64_t, v) & m);
+  }
+};
+
+template <int F, int T, int D, int NY, class V>
+__device__ __forceinline__ void apply_move64(V (&t)[D][NY], typename Num64<V>::Sum (&S)[D],
+                                             const typename Num64<V>::CV *row) {
+  typename Num64<V>::Sum sf = 0, st = 0;
 #pragma unroll
   for (int l = 0; l < NY; ++l) {
-    const uint64_t a = (uint64_t)row[l];
+    const V a = row[l];
     t[F][l] -= a;
     t[T][l] += a;
-    sf += abs64((int64_t)t[F][l]);
-    st += abs64((int64_t)t[T][l]);
+    sf += Num64<V>::mag(t[F][l]);
+    st += Num64<V>::mag(t[T][l]);
   }
   S[F] = sf;
   S[T] = st;
 }
 
-template <int C, int D, int NY>
+template <int C, int D, int NY, class V>
 struct Mover64 {
-  __device__ __forceinline__ static void run(uint64_t (&t)[D][NY], int64_t (&S)[D], const cint64_t *row, int from,
-                                             int up) {
+  __device__ __forceinline__ static void run(V (&t)[D][NY], typename Num64<V>::Sum (&S)[D],
+                                             const typename Num64<V>::CV *row, int from, int up) {
     if (from == C) {
       if (up) {
-        if constexpr (C + 1 < D) apply_move64<C, C + 1, D, NY>(t, S, row);
+        if constexpr (C + 1 < D) apply_move64<C, C + 1, D, NY, V>(t, S, row);
       } else {
-        if constexpr (C > 0) apply_move64<C, C - 1, D, NY>(t, S, row);
+        if constexpr (C > 0) apply_move64<C, C - 1, D, NY, V>(t, S, row);
       }
     } else {
-      if constexpr (C + 1 < D) Mover64<C + 1, D, NY>::run(t, S, row, from, up);
+      if constexpr (C + 1 < D) Mover64<C + 1, D, NY, V>::run(t, S, row, from, up);
     }
   }
 };
 
-template <int K, int D, int NY, bool UP>
+template <int K, int D, int NY, bool UP, class V>
 struct Digit0Run64 {
   template <class Eval>
-  __device__ __forceinline__ static void run(uint64_t (&t)[D][NY], int64_t (&S)[D], const cint64_t *row0,
-                                             uint32_t &tt, Eval &eval) {
+  __device__ __forceinline__ static void run(V (&t)[D][NY], typename Num64<V>::Sum (&S)[D],
+                                             const typename Num64<V>::CV *row0, uint32_t &tt, Eval &eval) {
     if constexpr (K < D) {
       constexpr int F = UP ? K - 1 : D - K;
       constexpr int T = UP ? K : D - K - 1;
-      apply_move64<F, T, D, NY>(t, S, row0);
+      apply_move64<F, T, D, NY, V>(t, S, row0);
       eval(++tt);
-      Digit0Run64<K + 1, D, NY, UP>::run(t, S, row0, tt, eval);
+      Digit0Run64<K + 1, D, NY, UP, V>::run(t, S, row0, tt, eval);
     }
   }
 };
 
-template <int D, int NY, int MODE>
-__global__ __launch_bounds__(256) void k_plain64_dary(const int64_t *__restrict__ A_, int R, int q, uint64_t seg0,
+template <int D, int NY, int MODE, class V>
+__global__ __launch_bounds__(256) void k_plain64_dary(const V *__restrict__ A_, int R, int q, uint64_t seg0,
                                                       uint64_t nseg, int64_t target, int64_t *__restrict__ unit_best,
                                                       long long *__restrict__ gbest,
                                                       unsigned long long *__restrict__ gfirst) {
-  const cint64_t *A = as_const64(A_);
+  typedef typename Num64<V>::Sum Sum;
+  typedef typename Num64<V>::CV CV;
+  const CV *A = (const CV *)(uintptr_t)A_;
   const uint64_t w = global_wave();
   const int lane = wave_lane();
   if (((w >> 1) << 7) >= nseg) return;
@@ -318,11 +346,11 @@ __global__ __launch_bounds__(256) void k_plain64_dary(const int64_t *__restrict_
   if (D & 1) par_mask = pseg ? ((1u << q) - 1u) : 0u;
   else par_mask = (q > 0 && pseg) ? (1u << (q - 1)) : 0u;
 
-  uint64_t t[D][NY];
+  V t[D][NY];
 #pragma unroll
   for (int c = 0; c < D; ++c)
 #pragma unroll
-    for (int l = 0; l < NY; ++l) t[c][l] = c == 0 ? (uint64_t)A[(size_t)(R - 1) * NY + l] : 0ull;
+    for (int l = 0; l < NY; ++l) t[c][l] = c == 0 ? (V)A[(size_t)(R - 1) * NY + l] : (V)0;
   uint64_t x = seg;
   for (int i = 0; i < R - 1; ++i) {
     int lab;
@@ -335,36 +363,37 @@ __global__ __launch_bounds__(256) void k_plain64_dary(const int64_t *__restrict_
     }
     // per-label lane masks, as seed_add32 (a lane-varying select per state
     // word doubles the registers)
-    const cint64_t *row = A + (size_t)i * NY;
+    const CV *row = A + (size_t)i * NY;
     uint64_t mask[D];
 #pragma unroll
     for (int c = 0; c < D; ++c) mask[c] = (lab == c) ? ~0ull : 0ull;
 #pragma unroll
     for (int l = 0; l < NY; ++l) {
-      const uint64_t v = (uint64_t)row[l];
+      const V v = row[l];
 #pragma unroll
-      for (int c = 0; c < D; ++c) t[c][l] += v & mask[c];
+      for (int c = 0; c < D; ++c) t[c][l] += Num64<V>::masked(v, mask[c]);
     }
   }
-  int64_t S[D];
+  Sum S[D];
 #pragma unroll
   for (int c = 0; c < D; ++c) {
-    int64_t s = 0;
+    Sum s = 0;
 #pragma unroll
-    for (int l = 0; l < NY; ++l) s += abs64((int64_t)t[c][l]);
+    for (int l = 0; l < NY; ++l) s += Num64<V>::mag(t[c][l]);
     S[c] = s;
   }
 
-  int64_t best = INT64_MIN;
+  const Sum starget = (Sum)target;
+  Sum best = 0;   // every objective is a sum of |.|
   uint32_t first = 0xFFFFFFFFu;
   auto eval = [&](uint32_t tt) {
-    int64_t acc = 0;
+    Sum acc = 0;
 #pragma unroll
     for (int c = 0; c < D; ++c) acc += S[c];
     if constexpr (MODE == MODE_MAX) {
-      best = acc > best ? acc : best;
+      best = Num64<V>::max(best, acc);
     } else {
-      const uint32_t cnd = (acc == target) ? tt : 0xFFFFFFFFu;
+      const uint32_t cnd = (acc == starget) ? tt : 0xFFFFFFFFu;
       first = cnd < first ? cnd : first;
     }
   };
@@ -381,8 +410,8 @@ __global__ __launch_bounds__(256) void k_plain64_dary(const int64_t *__restrict_
   eval(0);
   uint32_t tt = 0;
   if (q > 0) {
-    if (pm & 1u) Digit0Run64<1, D, NY, false>::run(t, S, A, tt, eval);
-    else Digit0Run64<1, D, NY, true>::run(t, S, A, tt, eval);
+    if (pm & 1u) Digit0Run64<1, D, NY, false, V>::run(t, S, A, tt, eval);
+    else Digit0Run64<1, D, NY, true, V>::run(t, S, A, tt, eval);
     for (uint32_t b = 1; b < nblocks; ++b) {
       const int jj = __builtin_ctzll(~cnt) >> 2;
       const uint64_t low = (1ull << (4 * jj)) - 1ull;
@@ -392,15 +421,15 @@ __global__ __launch_bounds__(256) void k_plain64_dary(const int64_t *__restrict_
       const int p = (pm >> j) & 1;
       const int from = p ? D - v : v - 1;
       pm ^= (1u << j) - 1u;
-      Mover64<0, D, NY>::run(t, S, A + (size_t)j * NY, from, p ? 0 : 1);
+      Mover64<0, D, NY, V>::run(t, S, A + (size_t)j * NY, from, p ? 0 : 1);
       eval(++tt);
-      if (pm & 1u) Digit0Run64<1, D, NY, false>::run(t, S, A, tt, eval);
-      else Digit0Run64<1, D, NY, true>::run(t, S, A, tt, eval);
+      if (pm & 1u) Digit0Run64<1, D, NY, false, V>::run(t, S, A, tt, eval);
+      else Digit0Run64<1, D, NY, true, V>::run(t, S, A, tt, eval);
     }
   }
   const uint32_t nsteps = q > 0 ? nblocks * (uint32_t)D : 1u;
-  finish64<MODE>(active, local, best, seg * (uint64_t)nsteps + first, first != 0xFFFFFFFFu, unit_best, gbest,
-                 gfirst);
+  finish64<MODE>(active, local, (int64_t)best, seg * (uint64_t)nsteps + first, first != 0xFFFFFFFFu, unit_best,
+                 gbest, gfirst);
 }
 
 // ---------------------------------------------------------------------------
@@ -574,17 +603,23 @@ static hipError_t lf64_bin(Mode mode, const Plain64Args &a, hipStream_t st) {
   return hipGetLastError();
 }
 
-template <int D, int NY>
-static hipError_t l64_dary(Mode mode, const Plain64Args &a, hipStream_t st) {
+template <int D, int NY, class V>
+static hipError_t l64_dary_v(Mode mode, const Plain64Args &a, hipStream_t st) {
   const uint64_t waves = 2 * ((a.nseg + 127) / 128);
   const dim3 grid((unsigned)((waves + 3) / 4)), blk(256);
+  const V *A = (const V *)a.A;
   if (mode == MODE_MAX)
-    hipLaunchKernelGGL((k_plain64_dary<D, NY, MODE_MAX>), grid, blk, 0, st, a.A, a.R, a.q, a.seg0, a.nseg, a.target,
-                       a.unit_best, a.gbest, a.gfirst);
+    hipLaunchKernelGGL((k_plain64_dary<D, NY, MODE_MAX, V>), grid, blk, 0, st, A, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
   else
-    hipLaunchKernelGGL((k_plain64_dary<D, NY, MODE_FIRST>), grid, blk, 0, st, a.A, a.R, a.q, a.seg0, a.nseg,
+    hipLaunchKernelGGL((k_plain64_dary<D, NY, MODE_FIRST, V>), grid, blk, 0, st, A, a.R, a.q, a.seg0, a.nseg,
                        a.target, a.unit_best, a.gbest, a.gfirst);
   return hipGetLastError();
+}
+
+template <int D, int NY>
+static hipError_t l64_dary(Mode mode, const Plain64Args &a, hipStream_t st) {
+  return a.fp64 ? l64_dary_v<D, NY, double>(mode, a, st) : l64_dary_v<D, NY, uint64_t>(mode, a, st);
 }
 
 int plain64_ny_pad(Kind kind, int d, int ny) {
@@ -640,9 +675,9 @@ hipError_t launch_plain64(Kind kind, Mode mode, const Plain64Args &a, hipStream_
 #undef X
         }
         break;
-      default: break;
+      default: break;   // d >= 3: the d-ary walk below, in doubles
     }
-    return hipErrorInvalidValue;
+    if (kind != KIND_LD) return hipErrorInvalidValue;
   }
   switch (kind) {
     case KIND_L1:
