@@ -161,11 +161,14 @@ int lnorm_hip_calc_noalloc(const int32_t *mtx_as_vec, int iRows, int iCols, int 
  * norm in int64, every sum exact while sum |A| <= INT64_MAX (else
  * LNORM_E_BOUND).  Problems inside the int32 contract run on the int32
  * kernels (the same result); the rest on 64-bit walks (no blocked or packed
- * 16-bit counterpart: opts->kernel may be AUTO, PLAIN or GENERAL).  The
- * plain walks run in doubles while sum |A| < 2^52, where every value is an
- * integer below 2^53 and each add is exact (kernel name plainf64<...>);
- * opts->word_bits = 64 keeps them on 64-bit integers (plain64<...>), 0
- * chooses; other values are LNORM_E_ARG.
+ * 16-bit counterpart: opts->kernel may be AUTO, PLAIN, WIDE or GENERAL).
+ * The plain walks run in doubles while sum |A| < 2^52, where every value is
+ * an integer below 2^53 and each add is exact (kernel name plainf64<...>),
+ * and there the binary walk also runs over W waves beyond one wave's width
+ * (WIDE, widef64<...>: L1 / LM up to 512 columns, L2 up to 256);
+ * opts->word_bits = 64 keeps the 64-bit integer walks (plain64<...>, and the
+ * general kernel beyond their widths), 0 chooses; other values are
+ * LNORM_E_ARG.
  * res->Lnorm holds the norm only when it fits int32 (else 0); *Lnorm always.
  */
 int lnorm_hip_calc_Lnorm64(const int64_t *mtx_as_vec, int iRows, int iCols, int n, int marginal,

@@ -1899,9 +1899,9 @@ int batch_impl(lnorm_hip_item *items, int count, const lnorm_hip_opts &o, lnorm_
 // ---- the int_type = long long variant (README.md:100-105) -------------------
 // Problems inside the int32 contract run on the int32 kernels (identical
 // results); the rest on the 64-bit walks of lnorm_int64.hip.
-int run64_phase(DevCtx &cx, Kind kind, int d, int ny, bool general, bool fp64, Mode mode, const Geometry &g,
-                uint64_t seg0, uint64_t nseg, int R, int64_t target, hipStream_t st, double &ms_kernel, int64_t &best,
-                uint64_t &first) {
+int run64_phase(DevCtx &cx, Kind kind, int d, int ny, bool general, bool wide, bool fp64, Mode mode,
+                const Geometry &g, uint64_t seg0, uint64_t nseg, int R, int64_t target, hipStream_t st,
+                double &ms_kernel, int64_t &best, uint64_t &first) {
   long long *gbest = (long long *)cx.scal.p;
   unsigned long long *gfirst = (unsigned long long *)((char *)cx.scal.p + 64);
   HIPCHK(launch_fill_i64(gbest, INT64_MIN, 1, st));
@@ -1922,7 +1922,9 @@ int run64_phase(DevCtx &cx, Kind kind, int d, int ny, bool general, bool fp64, M
   a.state = (uint64_t *)cx.state64.p;
   a.fp64 = fp64 ? 1 : 0;
   HIPCHK(hipEventRecord(cx.ev[4], st));
-  hipError_t e = general ? launch_general64(kind, mode, a, ny, st) : launch_plain64(kind, mode, a, st);
+  hipError_t e = general ? launch_general64(kind, mode, a, ny, st)
+                 : wide  ? launch_widef64(kind, mode, a, st)
+                         : launch_plain64(kind, mode, a, st);
   HIPCHK(hipEventRecord(cx.ev[5], st));
   if (e != hipSuccess) return fail(LNORM_E_HIP, "64-bit launch failed: %s", hipGetErrorString(e));
   if (mode == MODE_MAX) HIPCHK(launch_first_unit64(a.unit_best, nseg, gbest, gfirst, st));
@@ -1983,22 +1985,30 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
   uint64_t rb = 0, re = 0;
   if (int r = rank_range(p, o, rb, re)) return r;
   if (o.want_strategy && R > 1 && !strategy) return fail(LNORM_E_ARG, "strategy output is NULL");
+  // the plain walks in doubles while sum |A| < 2^52 (every value an exact
+  // integer below 2^53); word_bits 64 keeps the 64-bit integer walks
+  if (o.word_bits != 0 && o.word_bits != 64)
+    return fail(LNORM_E_ARG, "word_bits must be 0 (auto) or 64 (integer walk) for the long long variant");
+  const bool exact53 = o.word_bits != 64 && s < ((unsigned __int128)1 << 52);
   const int ny_plain = plain64_ny_pad(p.kind, p.n, C);
+  // beyond one wave's width, the binary walk in doubles over W waves
+  const int ny_wide = (exact53 && !ny_plain) ? widef64_ny_pad(p.kind, C) : 0;
   int kernel = o.kernel;
   if (kernel == LNORM_KERNEL_BLOCKED || kernel == LNORM_KERNEL_PLAIN16 || kernel == LNORM_KERNEL_TABLE16)
     return fail(LNORM_E_UNSUPPORTED, "the blocked and packed 16-bit kernels have no 64-bit counterpart");
   if (kernel == LNORM_KERNEL_PLAIN && !ny_plain)
     return fail(LNORM_E_UNSUPPORTED, "no 64-bit plain instantiation for %d columns at n = %d", C, n);
-  if (kernel == LNORM_KERNEL_AUTO) kernel = ny_plain ? LNORM_KERNEL_PLAIN : LNORM_KERNEL_GENERAL;
-  if (kernel != LNORM_KERNEL_PLAIN && kernel != LNORM_KERNEL_GENERAL)
+  if (kernel == LNORM_KERNEL_WIDE && !ny_wide)
+    return fail(LNORM_E_UNSUPPORTED, "the 64-bit W-wave walk runs in doubles: n <= 2, sum |A| < 2^52, beyond the "
+                "one-wave widths up to 512 columns (L2: 256); got %d columns at n = %d", C, n);
+  if (kernel == LNORM_KERNEL_AUTO)
+    kernel = ny_plain ? LNORM_KERNEL_PLAIN : ny_wide ? LNORM_KERNEL_WIDE : LNORM_KERNEL_GENERAL;
+  if (kernel != LNORM_KERNEL_PLAIN && kernel != LNORM_KERNEL_GENERAL && kernel != LNORM_KERNEL_WIDE)
     return fail(LNORM_E_ARG, "unknown kernel %d", o.kernel);
   const bool general = kernel == LNORM_KERNEL_GENERAL;
-  const int ny = general ? C : ny_plain;
-  // the plain walks in doubles while sum |A| < 2^52 (every value an exact
-  // integer below 2^53); word_bits 64 keeps the 64-bit integer walks
-  if (o.word_bits != 0 && o.word_bits != 64)
-    return fail(LNORM_E_ARG, "word_bits must be 0 (auto) or 64 (integer walk) for the long long variant");
-  const bool fp64 = !general && o.word_bits != 64 && s < ((unsigned __int128)1 << 52);
+  const bool wide = kernel == LNORM_KERNEL_WIDE;
+  const int ny = general ? C : wide ? ny_wide : ny_plain;
+  const bool fp64 = !general && exact53;
 
   // matrix padded to the kernel width and the binary delta table
   std::vector<int64_t> hA((size_t)R * ny, 0), hD;
@@ -2054,7 +2064,7 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
       const uint64_t nch = std::min(kUnitsPerChunk, run.n - c0);
       int64_t b;
       uint64_t f;
-      if (int r = run64_phase(cx, p.kind, p.n, ny, general, fp64, MODE_MAX, gr, run.u0 + c0, nch, R, 0, st,
+      if (int r = run64_phase(cx, p.kind, p.n, ny, general, wide, fp64, MODE_MAX, gr, run.u0 + c0, nch, R, 0, st,
                               ms_kernel, b, f))
         return r;
       if (first_unit == ~0ull || b > best) {
@@ -2072,8 +2082,8 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
     double ms2 = 0;
     int64_t b;
     uint64_t rank;
-    if (int r = run64_phase(cx, p.kind, p.n, ny, general, fp64, MODE_FIRST, g2, first_unit * g2.nseg, g2.nseg, R,
-                            best, st, ms2, b, rank))
+    if (int r = run64_phase(cx, p.kind, p.n, ny, general, wide, fp64, MODE_FIRST, g2, first_unit * g2.nseg,
+                            g2.nseg, R, best, st, ms2, b, rank))
       return r;
     if (rank == ~0ull) return fail(LNORM_E_HIP, "no rank attains the maximum %lld", (long long)best);
     rr.ms_strategy = ms2;
@@ -2088,7 +2098,7 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
   rr.ms_kernel = ms_kernel;
   rr.Lnorm = (best >= INT32_MIN && best <= INT32_MAX) ? (int32_t)best : 0;
   snprintf(rr.kernel_name, sizeof rr.kernel_name, "%s64<kind=%d,d=%d,ny=%d,q=%d>",
-           general ? "general" : fp64 ? "plainf" : "plain", (int)p.kind, p.n, ny, g.q);
+           general ? "general" : wide ? "widef" : fp64 ? "plainf" : "plain", (int)p.kind, p.n, ny, g.q);
   rr.ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_start).count();
   *Lnorm = best;
   if (res) *res = rr;

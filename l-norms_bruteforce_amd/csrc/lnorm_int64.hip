@@ -16,6 +16,8 @@
 //
 //   k_plain64_bin<KIND, NY>  L1 / LM / L2, NY up to 64 (L2: 32) columns
 //   k_plainf64_bin<KIND, NY> the same walk in doubles, for sum |A| < 2^52
+//   k_widef64_bin<KIND,NY,W> the walk in doubles over W waves: L1 / LM up to
+//                            512 columns, L2 up to 256
 //   k_plain64_dary<D, NY, V> L3..L10, d * NY <= 80 words of state (V = double
 //                            while sum |A| < 2^52, else uint64_t)
 //   k_general64              any width: wave per segment, columns over lanes
@@ -248,6 +250,142 @@ __global__ __launch_bounds__(256) void k_plainf64_bin(const double *__restrict__
     eval(t);
   }
   finish64<MODE>(active, local, (int64_t)best, (seg << q) + first, first != 0xFFFFFFFFu, unit_best, gbest, gfirst);
+}
+
+// ---------------------------------------------------------------------------
+// The binary walk in doubles over W waves, for widths beyond one wave's
+// registers (k_wide_bin's decomposition): wave wv keeps columns
+// [wv*NY, (wv+1)*NY) of the 64 lane segments' sums.  Each wave's partial |.|
+// sum of every step goes to LDS; after K steps (or the segment's end, so any
+// segment length works) every wave totals K/W steps over the W partials and
+// keeps their maximum (MODE_MAX) or first match (MODE_FIRST).  Exact while
+// sum |A| < 2^52, as k_plainf64_bin.
+// ---------------------------------------------------------------------------
+template <int KIND, int NY, int W, int MODE>
+__global__ __launch_bounds__(64 * W) void k_widef64_bin(const double *__restrict__ A_, const double *__restrict__ D_,
+                                                        int R, int q, uint64_t seg0, uint64_t nseg, int64_t target,
+                                                        int64_t *__restrict__ unit_best,
+                                                        long long *__restrict__ gbest,
+                                                        unsigned long long *__restrict__ gfirst) {
+  constexpr int NYT = NY * W;
+  constexpr int K = 8;   // steps per LDS combine
+  static_assert(K % W == 0, "steps per combine must split evenly over the waves");
+  constexpr bool TWO = (KIND == KIND_L2);
+  __shared__ double part[W * K * 64];
+  __shared__ double fin_best[W * 64];
+  __shared__ uint32_t fin_first[W * 64];
+  const cdouble_t *A = (const cdouble_t *)(uintptr_t)A_, *Dt = (const cdouble_t *)(uintptr_t)D_;
+  const int lane = wave_lane();
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t g = blockIdx.x;
+  if (((g >> 1) << 7) >= nseg) return;   // uniform over the workgroup
+  const uint64_t local = lane_segment(g, lane);
+  const bool active = local < nseg;
+  const uint64_t seg = seg0 + local;
+  const uint32_t par = (uint32_t)__builtin_amdgcn_readfirstlane((int)((seg0 ^ g) & 1));
+  const int wbase = wv * NY;
+  const bool mw = KIND == KIND_LM && wv == 0;   // the wave holding the marginal column 0
+
+  double y[NY];
+  double z[TWO ? NY : 1];
+  const uint64_t N0 = seg << q;
+  const uint64_t g0 = N0 ^ (N0 >> 1);
+#pragma unroll
+  for (int l = 0; l < NY; ++l) y[l] = A[(size_t)(R - 1) * NYT + wbase + l];
+  if constexpr (TWO) {
+#pragma unroll
+    for (int l = 0; l < NY; ++l) z[l] = 0.0;
+  }
+  for (int i = 0; i < R - 1; ++i) {
+    const bool gi = (g0 >> i) & 1;
+    const cdouble_t *row = A + (size_t)i * NYT + wbase;
+#pragma unroll
+    for (int l = 0; l < NY; ++l) {
+      const double a = row[l];
+      if constexpr (TWO) {
+        z[l] += gi ? a : 0.0;
+        y[l] += gi ? 0.0 : a;
+      } else {
+        y[l] += gi ? a : -a;
+      }
+    }
+  }
+
+  const double dtarget = (double)target;
+  double best = -__builtin_inf();
+  uint32_t first = 0xFFFFFFFFu;
+  const uint32_t nsteps = 1u << q;
+  const uint32_t nu = par << q;
+  for (uint32_t t0 = 0; t0 < nsteps; t0 += K) {
+    const uint32_t glen = nsteps - t0 < (uint32_t)K ? nsteps - t0 : (uint32_t)K;   // workgroup-uniform
+#pragma unroll 1
+    for (uint32_t b = 0; b < glen; ++b) {
+      const uint32_t t = t0 + b;
+      if (t > 0) {
+        const int j = __builtin_ctz(t);
+        const uint32_t up = (((nu | t) >> (j + 1)) & 1u) ^ 1u;
+        const cdouble_t *dp = Dt + (size_t)(up * (uint32_t)(R - 1) + (uint32_t)j) * NYT + wbase;
+#pragma unroll
+        for (int l = 0; l < NY; ++l) {
+          const double dl = dp[l];
+          if constexpr (TWO) {
+            z[l] += dl;
+            y[l] -= dl;
+          } else {
+            y[l] += dl;
+          }
+        }
+      }
+      double acc;
+      if constexpr (KIND == KIND_LM) {
+        // column 0 enters without |.| (L_OpenMP.c:40-41); it is wave 0's first
+        acc = mw ? y[0] : __builtin_fabs(y[0]);
+#pragma unroll
+        for (int l = 1; l < NY; ++l) acc += __builtin_fabs(y[l]);
+      } else {
+        acc = 0.0;
+#pragma unroll
+        for (int l = 0; l < NY; ++l) {
+          acc += __builtin_fabs(y[l]);
+          if constexpr (TWO) acc += __builtin_fabs(z[l]);
+        }
+      }
+      part[((size_t)wv * K + b) * 64 + lane] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int bb = 0; bb < K / W; ++bb) {
+      const uint32_t b = (uint32_t)(wv * (K / W) + bb);
+      if (b < glen) {
+        double tot = 0.0;
+#pragma unroll
+        for (int w2 = 0; w2 < W; ++w2) tot += part[((size_t)w2 * K + b) * 64 + lane];
+        if constexpr (MODE == MODE_MAX) {
+          best = __builtin_fmax(best, tot);
+        } else {
+          const uint32_t c = (tot == dtarget) ? t0 + b : 0xFFFFFFFFu;
+          first = c < first ? c : first;
+        }
+      }
+    }
+    __syncthreads();
+  }
+
+  // combine the waves' results per lane segment (wave 0 holds step 0's, so
+  // its maximum is finite)
+  fin_best[wv * 64 + lane] = best;
+  fin_first[wv * 64 + lane] = first;
+  __syncthreads();
+  if (wv == 0) {
+    double m = fin_best[lane];
+    uint32_t f = fin_first[lane];
+#pragma unroll
+    for (int w2 = 1; w2 < W; ++w2) {
+      m = __builtin_fmax(m, fin_best[w2 * 64 + lane]);
+      f = fin_first[w2 * 64 + lane] < f ? fin_first[w2 * 64 + lane] : f;
+    }
+    finish64<MODE>(active, local, (int64_t)m, (seg << q) + f, f != 0xFFFFFFFFu, unit_best, gbest, gfirst);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -715,6 +853,48 @@ hipError_t launch_plain64(Kind kind, Mode mode, const Plain64Args &a, hipStream_
       }
   }
   return hipErrorInvalidValue;
+}
+
+template <int KIND, int NY, int W>
+static hipError_t lwf64(Mode mode, const Plain64Args &a, hipStream_t st) {
+  const uint64_t groups = 2 * ((a.nseg + 127) / 128);
+  const dim3 grid((unsigned)groups), blk(64 * W);
+  const double *A = (const double *)a.A, *D = (const double *)a.D;
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_widef64_bin<KIND, NY, W, MODE_MAX>), grid, blk, 0, st, A, D, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
+  else
+    hipLaunchKernelGGL((k_widef64_bin<KIND, NY, W, MODE_FIRST>), grid, blk, 0, st, A, D, a.R, a.q, a.seg0,
+                       a.nseg, a.target, a.unit_best, a.gbest, a.gfirst);
+  return hipGetLastError();
+}
+
+// Widths of the W-wave walk in doubles: L1 / LM 128, 256 or 512 columns (4 or
+// 8 waves of 32 doubles, 8 of 64), L2 128 or 256 (4 or 8 waves of 32 pairs).
+int widef64_ny_pad(Kind kind, int ny) {
+  if (kind != KIND_L1 && kind != KIND_LM && kind != KIND_L2) return 0;
+  for (int w : {128, 256, 512})
+    if (w >= ny && (kind != KIND_L2 || w <= 256)) return w;
+  return 0;
+}
+
+hipError_t launch_widef64(Kind kind, Mode mode, const Plain64Args &a, hipStream_t st) {
+  if (!a.A || !a.D || !a.unit_best || !a.gbest || !a.gfirst || !a.fp64) return hipErrorInvalidValue;
+  if (widef64_ny_pad(kind, a.ny_pad) != a.ny_pad) return hipErrorInvalidValue;
+  switch (kind) {
+    case KIND_L1:
+      if (a.ny_pad == 128) return lwf64<KIND_L1, 32, 4>(mode, a, st);
+      if (a.ny_pad == 256) return lwf64<KIND_L1, 32, 8>(mode, a, st);
+      return lwf64<KIND_L1, 64, 8>(mode, a, st);
+    case KIND_LM:
+      if (a.ny_pad == 128) return lwf64<KIND_LM, 32, 4>(mode, a, st);
+      if (a.ny_pad == 256) return lwf64<KIND_LM, 32, 8>(mode, a, st);
+      return lwf64<KIND_LM, 64, 8>(mode, a, st);
+    case KIND_L2:
+      if (a.ny_pad == 128) return lwf64<KIND_L2, 32, 4>(mode, a, st);
+      return lwf64<KIND_L2, 32, 8>(mode, a, st);
+    default: return hipErrorInvalidValue;
+  }
 }
 
 hipError_t launch_general64(Kind kind, Mode mode, const Plain64Args &a, int ny, hipStream_t st) {

@@ -117,6 +117,41 @@ def test_int64_inside_int32_bound_uses_int32_kernels(pkg):
     assert b[2]["kernel_name"].startswith(("blocked", "table16")) and "64<" not in b[2]["kernel_name"]
 
 
+@pytest.mark.parametrize("n,marg", [(1, 0), (1, 1), (2, 0)])
+def test_int64_wide_double_walk(pkg, n, marg):
+    """Beyond the one-wave widths (L1 / LM 64 columns, L2 32) the walk in
+    doubles runs over W waves (128, 256 or 512 columns; L2 128 or 256):
+    against the oracle at every width, at segment lengths below and at one
+    8-step group (q = 0, 1, 2, 3) and beyond, over a rank range, and against
+    the general kernel on a larger problem."""
+    rng = np.random.default_rng(660 + 10 * n + marg)
+    widths = (65, 100, 128, 129, 256, 257, 400, 512) if n == 1 else (33, 100, 128, 129, 256)
+    for C in widths:
+        A = _rand64(rng, 9, C, 36)
+        print("int64 wide", n, marg, A.shape, flush=True)
+        want = oracle.calc_lnorm64(A, n, marg, 8)
+        norm, strat, res = pkg.calc_lnorm64(A, n, marg)
+        assert (norm, strat, res["best_rank"]) == want, (n, marg, A.shape, res["kernel_name"])
+        assert res["kernel_name"].startswith("widef64<"), res["kernel_name"]
+    A = _rand64(rng, 12, 200, 36)
+    want = oracle.calc_lnorm64(A, n, marg, 8)
+    for q in (0, 1, 2, 3, 5):
+        print("int64 wide seg_log", n, marg, q, flush=True)
+        norm, strat, res = pkg.calc_lnorm64(A, n, marg, seg_log=q)
+        assert (norm, strat, res["best_rank"]) == want, (q, res["kernel_name"])
+        assert f",q={q}>" in res["kernel_name"], res["kernel_name"]
+    rb, re = 37, 2 ** 11 - 5
+    print("int64 wide range", n, marg, rb, re, flush=True)
+    norm, strat, res = pkg.calc_lnorm64(A, n, marg, rank_begin=rb, rank_end=re)
+    assert (norm, res["best_rank"], strat) == oracle.walk64(A, n, marg, rb, re - 1)
+    B = _rand64(rng, 20, 300 if n == 1 else 200, 30)
+    print("int64 wide vs general", n, marg, B.shape, flush=True)
+    w = pkg.calc_lnorm64(B, n, marg)
+    g = pkg.calc_lnorm64(B, n, marg, kernel="general")
+    assert w[2]["kernel_name"].startswith("widef64<") and g[2]["kernel_name"].startswith("general64<")
+    assert w[:2] == g[:2] and w[2]["best_rank"] == g[2]["best_rank"]
+
+
 def test_int64_errors(pkg):
     with pytest.raises(pkg.LnormError) as e:
         pkg.calc_lnorm64(np.full((3, 2), 2 ** 62, np.int64), 1, 0)
@@ -130,3 +165,7 @@ def test_int64_errors(pkg):
     with pytest.raises(pkg.LnormError) as e:
         pkg.calc_lnorm64(np.full((12, 3), 2 ** 40, np.int64), 1, 0, word_bits=16)
     assert e.value.code == -1
+    # the W-wave walk runs in doubles only: refused with the integer walks
+    with pytest.raises(pkg.LnormError) as e:
+        pkg.calc_lnorm64(np.full((12, 100), 2 ** 33, np.int64), 1, 0, kernel="wide", word_bits=64)
+    assert e.value.code == -5
