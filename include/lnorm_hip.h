@@ -164,8 +164,9 @@ int lnorm_hip_calc_noalloc(const int32_t *mtx_as_vec, int iRows, int iCols, int 
  * 16-bit counterpart: opts->kernel may be AUTO, PLAIN, WIDE or GENERAL).
  * The plain walks run in doubles while sum |A| < 2^52, where every value is
  * an integer below 2^53 and each add is exact (kernel name plainf64<...>),
- * and there the binary walk also runs over W waves beyond one wave's width
- * (WIDE, widef64<...>: L1 / LM up to 512 columns, L2 up to 256);
+ * and there the walks also run over W waves beyond one wave's width (WIDE,
+ * widef64<...>: L1 / LM up to 512 columns, L2 up to 256, d = 3 up to 192,
+ * d = 4 up to 128, d = 5..10 up to 64);
  * opts->word_bits = 64 keeps the 64-bit integer walks (plain64<...>, and the
  * general kernel beyond their widths), 0 chooses; other values are
  * LNORM_E_ARG.

@@ -1992,15 +1992,15 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
   const bool exact53 = o.word_bits != 64 && s < ((unsigned __int128)1 << 52);
   const int ny_plain = plain64_ny_pad(p.kind, p.n, C);
   // beyond one wave's width, the binary walk in doubles over W waves
-  const int ny_wide = (exact53 && !ny_plain) ? widef64_ny_pad(p.kind, C) : 0;
+  const int ny_wide = (exact53 && !ny_plain) ? widef64_ny_pad(p.kind, p.n, C) : 0;
   int kernel = o.kernel;
   if (kernel == LNORM_KERNEL_BLOCKED || kernel == LNORM_KERNEL_PLAIN16 || kernel == LNORM_KERNEL_TABLE16)
     return fail(LNORM_E_UNSUPPORTED, "the blocked and packed 16-bit kernels have no 64-bit counterpart");
   if (kernel == LNORM_KERNEL_PLAIN && !ny_plain)
     return fail(LNORM_E_UNSUPPORTED, "no 64-bit plain instantiation for %d columns at n = %d", C, n);
   if (kernel == LNORM_KERNEL_WIDE && !ny_wide)
-    return fail(LNORM_E_UNSUPPORTED, "the 64-bit W-wave walk runs in doubles: n <= 2, sum |A| < 2^52, beyond the "
-                "one-wave widths up to 512 columns (L2: 256); got %d columns at n = %d", C, n);
+    return fail(LNORM_E_UNSUPPORTED, "the 64-bit W-wave walks run in doubles: sum |A| < 2^52, beyond the one-wave "
+                "widths up to 512 columns (L2: 256, d = 3: 192, 4: 128, 5..10: 64); got %d columns at n = %d", C, n);
   if (kernel == LNORM_KERNEL_AUTO)
     kernel = ny_plain ? LNORM_KERNEL_PLAIN : ny_wide ? LNORM_KERNEL_WIDE : LNORM_KERNEL_GENERAL;
   if (kernel != LNORM_KERNEL_PLAIN && kernel != LNORM_KERNEL_GENERAL && kernel != LNORM_KERNEL_WIDE)

@@ -18,6 +18,8 @@
 //   k_plainf64_bin<KIND, NY> the same walk in doubles, for sum |A| < 2^52
 //   k_widef64_bin<KIND,NY,W> the walk in doubles over W waves: L1 / LM up to
 //                            512 columns, L2 up to 256
+//   k_widef64_dary<D,NY,W>   the d-ary walk in doubles over 2, 4 or 8 waves:
+//                            up to 192 columns at d = 3, 128 at 4, 64 beyond
 //   k_plain64_dary<D, NY, V> L3..L10, d * NY <= 80 words of state (V = double
 //                            while sum |A| < 2^52, else uint64_t)
 //   k_general64              any width: wave per segment, columns over lanes
@@ -571,6 +573,157 @@ __global__ __launch_bounds__(256) void k_plain64_dary(const V *__restrict__ A_, 
 }
 
 // ---------------------------------------------------------------------------
+// The d-ary walk in doubles over W waves, beyond one wave's d * NY <= 80
+// doubles (sum |A| < 2^52): wave wv keeps columns [wv*NY, (wv+1)*NY) of every
+// label sum and its labels' partial |.| sums S.  The step counter and the
+// moves are workgroup-uniform, as in k_plain64_dary; each step's partial
+// objective (sum of S) goes to LDS, and after every K steps (or the
+// segment's end) each wave totals K/W of them over the W partials.
+// ---------------------------------------------------------------------------
+constexpr int kWideDaryK = 8;
+
+template <int D, int NY, int W, int MODE>
+__global__ __launch_bounds__(64 * W) void k_widef64_dary(const double *__restrict__ A_, int R, int q, uint64_t seg0,
+                                                         uint64_t nseg, int64_t target,
+                                                         int64_t *__restrict__ unit_best,
+                                                         long long *__restrict__ gbest,
+                                                         unsigned long long *__restrict__ gfirst) {
+  constexpr int NYT = NY * W;
+  constexpr int K = kWideDaryK;
+  static_assert(K % W == 0, "steps per combine must split evenly over the waves");
+  __shared__ double part[W * K * 64];
+  __shared__ double fin_best[W * 64];
+  __shared__ uint32_t fin_first[W * 64];
+  const cdouble_t *A = (const cdouble_t *)(uintptr_t)A_;
+  const int lane = wave_lane();
+  const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+  const uint64_t g = blockIdx.x;
+  if (((g >> 1) << 7) >= nseg) return;   // uniform over the workgroup
+  const uint64_t local = lane_segment(g, lane);
+  const bool active = local < nseg;
+  const uint64_t seg = seg0 + local;
+  const uint32_t pseg = (uint32_t)__builtin_amdgcn_readfirstlane((int)((seg0 ^ g) & 1));
+  const int wbase = wv * NY;
+  const cdouble_t *Aw = A + wbase;   // this wave's columns of row 0; row i at Aw + i * NYT
+
+  uint32_t par_mask;
+  if (D & 1) par_mask = pseg ? ((1u << q) - 1u) : 0u;
+  else par_mask = (q > 0 && pseg) ? (1u << (q - 1)) : 0u;
+
+  double t[D][NY];
+#pragma unroll
+  for (int c = 0; c < D; ++c)
+#pragma unroll
+    for (int l = 0; l < NY; ++l) t[c][l] = c == 0 ? Aw[(size_t)(R - 1) * NYT + l] : 0.0;
+  uint64_t x = seg;
+  for (int i = 0; i < R - 1; ++i) {
+    int lab;
+    if (i < q) {
+      lab = ((par_mask >> i) & 1) ? D - 1 : 0;
+    } else {
+      const uint32_t r = (uint32_t)(x % (uint64_t)(2 * D));
+      lab = r < (uint32_t)D ? (int)r : (int)(2 * D - 1 - r);
+      x /= (uint64_t)D;
+    }
+    const cdouble_t *row = Aw + (size_t)i * NYT;
+    uint64_t mask[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) mask[c] = (lab == c) ? ~0ull : 0ull;
+#pragma unroll
+    for (int l = 0; l < NY; ++l) {
+      const double v = row[l];
+#pragma unroll
+      for (int c = 0; c < D; ++c) t[c][l] += Num64<double>::masked(v, mask[c]);
+    }
+  }
+  double S[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    double sc = 0.0;
+#pragma unroll
+    for (int l = 0; l < NY; ++l) sc += __builtin_fabs(t[c][l]);
+    S[c] = sc;
+  }
+
+  uint32_t nblocks = 1;
+  for (int i = 1; i < q; ++i) nblocks *= (uint32_t)D;
+  const uint32_t nsteps = q > 0 ? nblocks * (uint32_t)D : 1u;
+  const double dtarget = (double)target;
+  double best = -__builtin_inf();
+  uint32_t first = 0xFFFFFFFFu;
+  // step tt's partial objective into LDS; after every K steps, or the last
+  // one, each wave totals its share (the step index is workgroup-uniform)
+  auto eval = [&](uint32_t tt) {
+    double acc = 0.0;
+#pragma unroll
+    for (int c = 0; c < D; ++c) acc += S[c];
+    const uint32_t b = tt % K;
+    part[((size_t)wv * K + b) * 64 + lane] = acc;
+    if (b == K - 1 || tt == nsteps - 1) {
+      const uint32_t t0 = tt - b, glen = b + 1;
+      __syncthreads();
+#pragma unroll
+      for (int bb = 0; bb < K / W; ++bb) {
+        const uint32_t e = (uint32_t)(wv * (K / W) + bb);
+        if (e < glen) {
+          double tot = 0.0;
+#pragma unroll
+          for (int w2 = 0; w2 < W; ++w2) tot += part[((size_t)w2 * K + e) * 64 + lane];
+          if constexpr (MODE == MODE_MAX) {
+            best = __builtin_fmax(best, tot);
+          } else {
+            const uint32_t c = (tot == dtarget) ? t0 + e : 0xFFFFFFFFu;
+            first = c < first ? c : first;
+          }
+        }
+      }
+      __syncthreads();
+    }
+  };
+
+  uint64_t biaspat = 0;
+  for (int i = 1; i < q; ++i) biaspat |= (uint64_t)(16 - D) << (4 * (i - 1));
+  uint64_t cnt = biaspat;
+  uint32_t pm = par_mask;
+
+  eval(0);
+  uint32_t tt = 0;
+  if (q > 0) {
+    if (pm & 1u) Digit0Run64<1, D, NY, false, double>::run(t, S, Aw, tt, eval);
+    else Digit0Run64<1, D, NY, true, double>::run(t, S, Aw, tt, eval);
+    for (uint32_t b = 1; b < nblocks; ++b) {
+      const int jj = __builtin_ctzll(~cnt) >> 2;
+      const uint64_t low = (1ull << (4 * jj)) - 1ull;
+      cnt = (cnt + 1ull) | (biaspat & low);
+      const int v = (int)((cnt >> (4 * jj)) & 15ull) - (16 - D);
+      const int j = jj + 1;
+      const int p = (pm >> j) & 1;
+      const int from = p ? D - v : v - 1;
+      pm ^= (1u << j) - 1u;
+      Mover64<0, D, NY, double>::run(t, S, Aw + (size_t)j * NYT, from, p ? 0 : 1);
+      eval(++tt);
+      if (pm & 1u) Digit0Run64<1, D, NY, false, double>::run(t, S, Aw, tt, eval);
+      else Digit0Run64<1, D, NY, true, double>::run(t, S, Aw, tt, eval);
+    }
+  }
+
+  fin_best[wv * 64 + lane] = best;
+  fin_first[wv * 64 + lane] = first;
+  __syncthreads();
+  if (wv == 0) {
+    double m = fin_best[lane];
+    uint32_t f = fin_first[lane];
+#pragma unroll
+    for (int w2 = 1; w2 < W; ++w2) {
+      m = __builtin_fmax(m, fin_best[w2 * 64 + lane]);
+      f = fin_first[w2 * 64 + lane] < f ? fin_first[w2 * 64 + lane] : f;
+    }
+    finish64<MODE>(active, local, (int64_t)m, seg * (uint64_t)nsteps + f, f != 0xFFFFFFFFu, unit_best, gbest,
+                   gfirst);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Any width: one wave per segment, lanes own columns, label vectors in a
 // per-wave global scratch slice (the 64-bit twin of k_general).
 // ---------------------------------------------------------------------------
@@ -869,19 +1022,64 @@ static hipError_t lwf64(Mode mode, const Plain64Args &a, hipStream_t st) {
   return hipGetLastError();
 }
 
-// Widths of the W-wave walk in doubles: L1 / LM 128, 256 or 512 columns (4 or
-// 8 waves of 32 doubles, 8 of 64), L2 128 or 256 (4 or 8 waves of 32 pairs).
-int widef64_ny_pad(Kind kind, int ny) {
-  if (kind != KIND_L1 && kind != KIND_LM && kind != KIND_L2) return 0;
+template <int D, int NY, int W>
+static hipError_t lwf64d(Mode mode, const Plain64Args &a, hipStream_t st) {
+  const uint64_t groups = 2 * ((a.nseg + 127) / 128);
+  const dim3 grid((unsigned)groups), blk(64 * W);
+  const double *A = (const double *)a.A;
+  if (mode == MODE_MAX)
+    hipLaunchKernelGGL((k_widef64_dary<D, NY, W, MODE_MAX>), grid, blk, 0, st, A, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
+  else
+    hipLaunchKernelGGL((k_widef64_dary<D, NY, W, MODE_FIRST>), grid, blk, 0, st, A, a.R, a.q, a.seg0, a.nseg,
+                       a.target, a.unit_best, a.gbest, a.gfirst);
+  return hipGetLastError();
+}
+
+// doubles per wave of the d-ary W-wave walk (d * NY <= 80, as the one-wave walk)
+constexpr int widef64_dary_nw(int d) { return d == 3 ? 24 : d == 4 ? 16 : 8; }
+
+template <int D>
+static hipError_t lwf64d_w(Mode mode, const Plain64Args &a, hipStream_t st) {
+  constexpr int NY = widef64_dary_nw(D);
+  if (a.ny_pad == 2 * NY) return lwf64d<D, NY, 2>(mode, a, st);
+  if (a.ny_pad == 4 * NY) return lwf64d<D, NY, 4>(mode, a, st);
+  if (a.ny_pad == 8 * NY) return lwf64d<D, NY, 8>(mode, a, st);
+  return hipErrorInvalidValue;
+}
+
+// Widths of the W-wave walks in doubles: L1 / LM 128, 256 or 512 columns (4
+// or 8 waves of 32 doubles, 8 of 64), L2 128 or 256 (4 or 8 waves of 32
+// pairs); d = 3..10 2, 4 or 8 waves of 24 (d = 3), 16 (d = 4) or 8 columns.
+int widef64_ny_pad(Kind kind, int d, int ny) {
+  if (kind == KIND_LD) {
+    if (d < 3 || d > 10) return 0;
+    for (int w : {2, 4, 8})
+      if (w * widef64_dary_nw(d) >= ny) return w * widef64_dary_nw(d);
+    return 0;
+  }
   for (int w : {128, 256, 512})
     if (w >= ny && (kind != KIND_L2 || w <= 256)) return w;
   return 0;
 }
 
 hipError_t launch_widef64(Kind kind, Mode mode, const Plain64Args &a, hipStream_t st) {
-  if (!a.A || !a.D || !a.unit_best || !a.gbest || !a.gfirst || !a.fp64) return hipErrorInvalidValue;
-  if (widef64_ny_pad(kind, a.ny_pad) != a.ny_pad) return hipErrorInvalidValue;
+  if (!a.A || (kind != KIND_LD && !a.D) || !a.unit_best || !a.gbest || !a.gfirst || !a.fp64)
+    return hipErrorInvalidValue;
+  if (widef64_ny_pad(kind, a.d, a.ny_pad) != a.ny_pad) return hipErrorInvalidValue;
   switch (kind) {
+    case KIND_LD:
+      switch (a.d) {
+        case 3: return lwf64d_w<3>(mode, a, st);
+        case 4: return lwf64d_w<4>(mode, a, st);
+        case 5: return lwf64d_w<5>(mode, a, st);
+        case 6: return lwf64d_w<6>(mode, a, st);
+        case 7: return lwf64d_w<7>(mode, a, st);
+        case 8: return lwf64d_w<8>(mode, a, st);
+        case 9: return lwf64d_w<9>(mode, a, st);
+        case 10: return lwf64d_w<10>(mode, a, st);
+        default: return hipErrorInvalidValue;
+      }
     case KIND_L1:
       if (a.ny_pad == 128) return lwf64<KIND_L1, 32, 4>(mode, a, st);
       if (a.ny_pad == 256) return lwf64<KIND_L1, 32, 8>(mode, a, st);

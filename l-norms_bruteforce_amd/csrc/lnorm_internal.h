@@ -209,9 +209,9 @@ hipError_t launch_first_unit64(const int64_t *unit_best, uint64_t n, const long 
 int plain64_ny_pad(Kind kind, int d, int ny);   // 0: no register-resident 64-bit width
 hipError_t launch_plain64(Kind kind, Mode mode, const Plain64Args &a, hipStream_t st);
 hipError_t launch_general64(Kind kind, Mode mode, const Plain64Args &a, int ny, hipStream_t st);
-// the binary walk in doubles over W waves (a.fp64 set): its width for ny
-// columns (0: none) and its launch
-int widef64_ny_pad(Kind kind, int ny);
+// the walks in doubles over W waves (a.fp64 set; binary and d-ary): the
+// width for ny columns (0: none) and the launch
+int widef64_ny_pad(Kind kind, int d, int ny);
 hipError_t launch_widef64(Kind kind, Mode mode, const Plain64Args &a, hipStream_t st);
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, uint64_t n, hipStream_t st);
 hipError_t launch_first_unit(const int32_t *unit_best, uint64_t n, const int32_t *gbest,

@@ -152,6 +152,48 @@ def test_int64_wide_double_walk(pkg, n, marg):
     assert w[:2] == g[:2] and w[2]["best_rank"] == g[2]["best_rank"]
 
 
+# (d, rows, columns): beyond the one-wave widths, every wave count (2, 4, 8)
+WIDE_DARY = [(3, 7, (25, 48, 49, 96, 97, 192)), (4, 6, (17, 32, 64, 65, 128)), (5, 5, (9, 16, 32, 64)),
+             (7, 5, (9, 16, 32, 33, 64)), (10, 4, (9, 16, 32, 64))]
+
+
+@pytest.mark.parametrize("n,R,cols", WIDE_DARY)
+def test_int64_wide_double_dary(pkg, n, R, cols):
+    """The d-ary walk in doubles over 2, 4 or 8 waves beyond one wave's
+    d * NY <= 80 doubles: against the oracle at every wave count, at forced
+    segment lengths (a segment shorter than one 8-step combine included),
+    over a rank range, and against the general kernel."""
+    rng = np.random.default_rng(690 + n)
+    nw = {3: 24, 4: 16}.get(n, 8)
+    seen = set()
+    for C in cols:
+        A = _rand64(rng, R, C, 36)
+        print("int64 wide dary", n, A.shape, flush=True)
+        want = oracle.calc_lnorm64(A, n, 0, 8)
+        norm, strat, res = pkg.calc_lnorm64(A, n, 0)
+        assert (norm, strat, res["best_rank"]) == want, (n, A.shape, res["kernel_name"])
+        assert res["kernel_name"].startswith("widef64<kind=3"), res["kernel_name"]
+        seen.add(int(res["kernel_name"].split("ny=")[1].split(",")[0]) // nw)
+    assert seen == {2, 4, 8}, seen
+    A = _rand64(rng, R + 1, cols[1], 36)
+    want = oracle.calc_lnorm64(A, n, 0, 8)
+    for q in (1, 2, 3):
+        print("int64 wide dary seg_log", n, q, flush=True)
+        norm, strat, res = pkg.calc_lnorm64(A, n, 0, seg_log=q)
+        assert (norm, strat, res["best_rank"]) == want, (q, res["kernel_name"])
+        assert f",q={q}>" in res["kernel_name"], res["kernel_name"]
+    total = n ** R
+    rb, re = 5, total - 3
+    norm, strat, res = pkg.calc_lnorm64(A, n, 0, rank_begin=rb, rank_end=re)
+    assert (norm, res["best_rank"], strat) == oracle.walk64(A, n, 0, rb, re - 1)
+    B = _rand64(rng, R + 3, cols[2], 30)
+    print("int64 wide dary vs general", n, B.shape, flush=True)
+    w = pkg.calc_lnorm64(B, n, 0)
+    g = pkg.calc_lnorm64(B, n, 0, kernel="general")
+    assert w[2]["kernel_name"].startswith("widef64<") and g[2]["kernel_name"].startswith("general64<")
+    assert w[:2] == g[:2] and w[2]["best_rank"] == g[2]["best_rank"]
+
+
 def test_int64_errors(pkg):
     with pytest.raises(pkg.LnormError) as e:
         pkg.calc_lnorm64(np.full((3, 2), 2 ** 62, np.int64), 1, 0)
