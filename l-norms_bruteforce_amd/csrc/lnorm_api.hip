@@ -2092,6 +2092,8 @@ int calc_impl64(const int64_t *A, int R, int C, int n, int marginal, const lnorm
   } else {
     rr.best_rank = R == 1 ? 0 : ~0ull;
   }
+  // without the strategy phase nothing has waited for the stream since ev[1]
+  HIPCHK(hipEventSynchronize(cx.ev[1]));
   float ms = 0;
   HIPCHK(hipEventElapsedTime(&ms, cx.ev[0], cx.ev[1]));
   rr.ms_enumerate = ms;

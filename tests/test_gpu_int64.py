@@ -194,6 +194,25 @@ def test_int64_wide_double_dary(pkg, n, R, cols):
     assert w[:2] == g[:2] and w[2]["best_rank"] == g[2]["best_rank"]
 
 
+def test_int64_double_walks_few_rows(pkg):
+    """1, 2 and 3 rows (a single labeling, segments of 1 and 2 steps) on
+    every walk in doubles, against the oracle."""
+    rng = np.random.default_rng(710)
+    for n, marg, C, name in ((1, 0, 40, "plainf64<"), (1, 1, 40, "plainf64<"), (1, 0, 200, "widef64<"),
+                             (1, 1, 300, "widef64<"), (2, 0, 100, "widef64<"), (3, 0, 20, "plainf64<"),
+                             (3, 0, 100, "widef64<"), (6, 0, 40, "widef64<")):
+        for R in (1, 2, 3):
+            A = _rand64(rng, R, C, 36)
+            print("int64 few rows", n, marg, A.shape, flush=True)
+            want = oracle.calc_lnorm64(A, n, marg, 8)
+            norm, strat, res = pkg.calc_lnorm64(A, n, marg)
+            assert (norm, strat, res["best_rank"]) == want, (n, marg, A.shape, res["kernel_name"])
+            assert res["kernel_name"].startswith(name), (n, marg, A.shape, res["kernel_name"])
+            # without the strategy (the call must still wait for its timing events)
+            norm2, _, res2 = pkg.calc_lnorm64(A, n, marg, want_strategy=False)
+            assert norm2 == want[0], (n, marg, A.shape)
+
+
 def test_int64_errors(pkg):
     with pytest.raises(pkg.LnormError) as e:
         pkg.calc_lnorm64(np.full((3, 2), 2 ** 62, np.int64), 1, 0)
