@@ -11,7 +11,7 @@ high rows, v_sad_u16) is the headline; the plain Gray kernel runs the same
 steps beside it, as do a 40 x 40 matrix (entries in [-100, 100], seed 106)
 and the reference L_OpenMP on the host cores over a bounded sample.
 
-Prints one JSON line.  --gpus above 1 is refused: this build runs one GPU.
+Prints one JSON line.  --gpus above 1 is refused: the bench runs on one GPU.
 """
 from __future__ import annotations
 
@@ -728,7 +728,7 @@ def main():
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py runs on one GPU only: the enumeration shards over GPUs by Gray-rank ranges "
-              "(one max/argmin exchange), but no multi-GPU run is part of this build.", file=sys.stderr)
+              "(one max/argmin exchange), and that runs one call per GPU; the bench itself runs on one.", file=sys.stderr)
         return 2
     if a.copy_calibration:
         import torch
