@@ -727,8 +727,8 @@ def main():
                          "kernel for scaling the profiler's FETCH_SIZE / WRITE_SIZE counters")
     a = ap.parse_args()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py runs on one GPU only: the enumeration shards over GPUs by Gray-rank ranges "
-              "(one max/argmin exchange), and that runs one call per GPU; the bench itself runs on one.", file=sys.stderr)
+        print("bench.py runs on one GPU only.  Over several GPUs the enumeration shards by Gray-rank ranges "
+              "(rank_begin / rank_end, one call per GPU, then one max / smallest-rank exchange).", file=sys.stderr)
         return 2
     if a.copy_calibration:
         import torch
