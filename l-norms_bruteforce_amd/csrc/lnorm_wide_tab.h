@@ -312,6 +312,8 @@ constexpr int wtb_m(int w) { return w > 8 ? 4 : 5; }   // partials: W * 2^M * 25
 // operation of the batch before it (the one that waits for that batch), so a
 // batch has KB - 1 operations of this wave to arrive in, not the few the
 // compiler's own order leaves.  Scheduling barriers pin that order.
+// KB divides NW: 40-word batches at 64 words per wave (batches straddling
+// patterns) ran 50x slower, the batch loop no longer fully unrolled.
 // NC: interleaved |.| chains; two measured faster only for 32-bit words at
 // 64 words per wave (1024-word shape: 298 against 315 ms on 34 x 1000
 // int16-range entries), one elsewhere (1 to 3 % ahead, one register fewer).
@@ -323,7 +325,7 @@ __device__ __forceinline__ void wtb_partials(const uint32_t (&u)[NW], int32_t u0
                                              const cint32_t *pos0, bool wave0, uint32_t *p, int lane) {
   constexpr int NT = (1 << M) * NW;   // table words of this wave
   constexpr int NB = (NT + KB - 1) / KB;
-  static_assert(KB <= NW, "at most one pattern starts in a batch");
+  static_assert(KB <= NW && NW % KB == 0, "batches tile each pattern's words");
   uint32_t buf[2][KB];
   int32_t pn[2] = {0, 0};   // LM: pos0 of the pattern starting in the batch
 #pragma unroll
