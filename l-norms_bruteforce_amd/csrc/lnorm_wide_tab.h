@@ -8,7 +8,10 @@
 // sum.  After a high step each wave computes the two changed labels' subset
 // tables over its words (2 * 2^M * NW v_sad_u16, as walk_blk16) and stores
 // these partial tables in LDS; after a barrier every wave sums the W
-// partials, so each wave holds the full tables.  The block's d^M low
+// partials, so each wave holds the full tables (above 4 waves: each wave
+// totals its share of the entries over the W partials, and after a second
+// barrier every wave reads all the totals; 1.2-1.6x at 8 waves, d = 2, 3).
+// A last barrier frees the memory for the next step.  The block's d^M low
 // labelings are split over the waves (wave wv takes l = wv, wv + W, ...), and
 // the per-wave maxima (MODE_MAX) or first matching ranks (MODE_FIRST) are
 // combined once at the end.  Labels, ranks and the pattern order are those of
@@ -72,8 +75,9 @@ __device__ __forceinline__ void wt_partial_pair(const uint32_t (&u)[D][NW], cons
   }
 }
 
-// The full tables of labels C and C + 1 from the W waves' partials
-// (T[c][0] = |u_c|; d <= 3: F_c(S); d >= 4: F_c(S) - |u_c|).
+// The full tables of labels C and C + 1 from the W waves' partials, every
+// wave summing every entry (T[c][0] = |u_c|; d <= 3: F_c(S); d >= 4:
+// F_c(S) - |u_c|).
 template <int C, int D, int W, int M>
 __device__ __forceinline__ void wt_sum_pair(uint32_t (&T)[D][1 << M], const uint32_t *part, int lane) {
 #pragma unroll
@@ -85,6 +89,47 @@ __device__ __forceinline__ void wt_sum_pair(uint32_t (&T)[D][1 << M], const uint
 #pragma unroll
       for (int w2 = 0; w2 < W; ++w2) acc += part[(((size_t)w2 * 2 + e) * (1 << M) + S) * 64 + lane];
       f[S] = acc;
+    }
+#pragma unroll
+    for (int S = 0; S < (1 << M); ++S) T[C + e][S] = (D <= 3 || S == 0) ? f[S] : f[S] - f[0];
+  }
+}
+
+// Above 4 waves, the W waves' partial tables in two steps, with a barrier
+// between them.
+// wt_scatter: wave wv totals the entries i = wv, wv + W, ... (i = e * 2^M + S
+// of label C + e, 2^(M+1) entries) over the W partials and writes each total
+// over its own partial of that entry, which no other wave reads.  Each lane
+// reads 2^(M+1) / W entries of W partials, where every wave summing every
+// entry read W * 2^(M+1).
+template <int W, int M>
+__device__ __forceinline__ void wt_scatter(uint32_t *part, int wv, int lane) {
+  constexpr int NE = 2 << M;
+#pragma unroll
+  for (int i = 0; i < (NE + W - 1) / W; ++i) {
+    const int idx = wv + W * i;
+    if (idx < NE) {   // wave-uniform
+      uint32_t acc = 0;
+#pragma unroll
+      for (int w2 = 0; w2 < W; ++w2) acc += part[((size_t)w2 * NE + idx) * 64 + lane];
+      part[((size_t)wv * NE + idx) * 64 + lane] = acc;
+    }
+  }
+}
+
+// wt_gather_pair: the full tables of labels C and C + 1 from the totals,
+// entry i in the slice of wave i mod W (T[c][0] = |u_c|; d <= 3: F_c(S);
+// d >= 4: F_c(S) - |u_c|).
+template <int C, int D, int W, int M>
+__device__ __forceinline__ void wt_gather_pair(uint32_t (&T)[D][1 << M], const uint32_t *part, int lane) {
+  constexpr int NE = 2 << M;
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    uint32_t f[1 << M];
+#pragma unroll
+    for (int S = 0; S < (1 << M); ++S) {
+      const int idx = e * (1 << M) + S;
+      f[S] = part[((size_t)(idx % W) * NE + idx) * 64 + lane];
     }
 #pragma unroll
     for (int S = 0; S < (1 << M); ++S) T[C + e][S] = (D <= 3 || S == 0) ? f[S] : f[S] - f[0];
@@ -104,6 +149,12 @@ struct WtPair {
     if constexpr (C + 1 < D) {
       if (lo == C) wt_sum_pair<C, D, W, M>(T, part, lane);
       else WtPair<C + 1, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
+    }
+  }
+  __device__ __forceinline__ static void gather(int lo, uint32_t (&T)[D][1 << M], const uint32_t *part, int lane) {
+    if constexpr (C + 1 < D) {
+      if (lo == C) wt_gather_pair<C, D, W, M>(T, part, lane);
+      else WtPair<C + 1, D, NW, NWT, W, M, WB>::gather(lo, T, part, lane);
     }
   }
 };
@@ -212,7 +263,13 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
     asm volatile("" : "+s"(negk));
     WtPair<0, D, NW, NWT, W, M, WB>::partial(lo, u, negk + wbase, mypart, lane);
     __syncthreads();
-    WtPair<0, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
+    if constexpr (W > 4) {   // wave-count constant: the barriers stay uniform
+      wt_scatter<W, M>(part, wv, lane);
+      __syncthreads();
+      WtPair<0, D, NW, NWT, W, M, WB>::gather(lo, T, part, lane);
+    } else {
+      WtPair<0, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
+    }
     __syncthreads();
   }
   uint32_t totU = 0;
@@ -244,7 +301,13 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
       asm volatile("" : "+s"(negk));
       WtPair<0, D, NW, NWT, W, M, WB>::partial(lo, u, negk + wbase, mypart, lane);
       __syncthreads();
-      WtPair<0, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
+      if constexpr (W > 4) {   // wave-count constant: the barriers stay uniform
+        wt_scatter<W, M>(part, wv, lane);
+        __syncthreads();
+        WtPair<0, D, NW, NWT, W, M, WB>::gather(lo, T, part, lane);
+      } else {
+        WtPair<0, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
+      }
       __syncthreads();
       totU = 0;
 #pragma unroll
