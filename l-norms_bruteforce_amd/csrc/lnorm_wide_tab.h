@@ -8,10 +8,12 @@
 // sum.  After a high step each wave computes the two changed labels' subset
 // tables over its words (2 * 2^M * NW v_sad_u16, as walk_blk16) and stores
 // these partial tables in LDS; after a barrier every wave sums the W
-// partials, so each wave holds the full tables (above 4 waves: each wave
-// totals its share of the entries over the W partials, and after a second
-// barrier every wave reads all the totals; 1.2-1.6x at 8 waves, d = 2, 3).
-// A last barrier frees the memory for the next step.  The block's d^M low
+// partials, so each wave holds the full tables, and a second barrier frees
+// the memory for the next step.  Above 4 waves each wave instead totals its
+// share of the entries over the W partials into a table of totals, and after
+// the second barrier every wave reads all the totals (1.2-1.6x at 8 waves,
+// d = 2, 3); the next step writes the totals only after its first barrier.
+// The block's d^M low
 // labelings are split over the waves (wave wv takes l = wv, wv + W, ...), and
 // the per-wave maxima (MODE_MAX) or first matching ranks (MODE_FIRST) are
 // combined once at the end.  Labels, ranks and the pattern order are those of
@@ -32,7 +34,8 @@ constexpr int wt16_nw(int d) { return d <= 7 ? 32 : (d == 8 || d == 9) ? 24 : d 
 constexpr int wt16_wmax(int d) { return d <= 6 ? 8 : 4; }
 // the table rows of the W-wave walk: the single wave's rows at its width,
 // capped by LDS and registers.  LDS: the W waves' partial tables of two
-// labels (W * 2 * 2^M * 256 B) fill up to 64 KiB for d = 3 and 32 KiB for
+// labels (W * 2 * 2^M * 256 B; above 4 waves one table more, the totals)
+// fill up to 64 KiB for d = 3 and 32 KiB for
 // other d (at d = 2 both labels change every step, so a row more does not
 // cut the table work per labeling: it ran 1.7x slower).  Registers: above 4
 // waves a workgroup puts 2 waves on a SIMD (256 VGPRs each), where 4 rows fit
@@ -52,6 +55,9 @@ constexpr int wt16_m(int d, int w) {
 // at d = 6 (its 2 table rows spilled there)
 constexpr int kWtWaves[] = {2, 3, 4, 5, 6, 8};
 constexpr bool wt16_has(int d, int w) { return w <= wt16_wmax(d) && !(d == 6 && w == 5); }
+// LDS words of the W-wave tabulated walk: W partial tables of two labels
+// (2^(M+1) x 64 words each), and above 4 waves one more for the totals
+constexpr int wt16_lds_words(int w, int m) { return (w + (w > 4 ? 1 : 0)) * (2 << m) * 64; }
 
 // This wave's partial tables of labels C and C + 1 into its LDS slice
 // (p[e][S][lane], e = 0 for C, 1 for C + 1).
@@ -98,12 +104,11 @@ __device__ __forceinline__ void wt_sum_pair(uint32_t (&T)[D][1 << M], const uint
 // Above 4 waves, the W waves' partial tables in two steps, with a barrier
 // between them.
 // wt_scatter: wave wv totals the entries i = wv, wv + W, ... (i = e * 2^M + S
-// of label C + e, 2^(M+1) entries) over the W partials and writes each total
-// over its own partial of that entry, which no other wave reads.  Each lane
-// reads 2^(M+1) / W entries of W partials, where every wave summing every
-// entry read W * 2^(M+1).
+// of label C + e, 2^(M+1) entries) over the W partials into tot[i].  Each
+// lane reads 2^(M+1) / W entries of W partials, where every wave summing
+// every entry read W * 2^(M+1).
 template <int W, int M>
-__device__ __forceinline__ void wt_scatter(uint32_t *part, int wv, int lane) {
+__device__ __forceinline__ void wt_scatter(const uint32_t *part, uint32_t *tot, int wv, int lane) {
   constexpr int NE = 2 << M;
 #pragma unroll
   for (int i = 0; i < (NE + W - 1) / W; ++i) {
@@ -112,24 +117,22 @@ __device__ __forceinline__ void wt_scatter(uint32_t *part, int wv, int lane) {
       uint32_t acc = 0;
 #pragma unroll
       for (int w2 = 0; w2 < W; ++w2) acc += part[((size_t)w2 * NE + idx) * 64 + lane];
-      part[((size_t)wv * NE + idx) * 64 + lane] = acc;
+      tot[(size_t)idx * 64 + lane] = acc;
     }
   }
 }
 
-// wt_gather_pair: the full tables of labels C and C + 1 from the totals,
-// entry i in the slice of wave i mod W (T[c][0] = |u_c|; d <= 3: F_c(S);
-// d >= 4: F_c(S) - |u_c|).
-template <int C, int D, int W, int M>
-__device__ __forceinline__ void wt_gather_pair(uint32_t (&T)[D][1 << M], const uint32_t *part, int lane) {
-  constexpr int NE = 2 << M;
+// wt_gather_pair: the full tables of labels C and C + 1 from the totals
+// (T[c][0] = |u_c|; d <= 3: F_c(S); d >= 4: F_c(S) - |u_c|).
+template <int C, int D, int M>
+__device__ __forceinline__ void wt_gather_pair(uint32_t (&T)[D][1 << M], const uint32_t *tot, int lane) {
 #pragma unroll
   for (int e = 0; e < 2; ++e) {
     uint32_t f[1 << M];
 #pragma unroll
     for (int S = 0; S < (1 << M); ++S) {
       const int idx = e * (1 << M) + S;
-      f[S] = part[((size_t)(idx % W) * NE + idx) * 64 + lane];
+      f[S] = tot[(size_t)idx * 64 + lane];
     }
 #pragma unroll
     for (int S = 0; S < (1 << M); ++S) T[C + e][S] = (D <= 3 || S == 0) ? f[S] : f[S] - f[0];
@@ -151,10 +154,10 @@ struct WtPair {
       else WtPair<C + 1, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
     }
   }
-  __device__ __forceinline__ static void gather(int lo, uint32_t (&T)[D][1 << M], const uint32_t *part, int lane) {
+  __device__ __forceinline__ static void gather(int lo, uint32_t (&T)[D][1 << M], const uint32_t *tot, int lane) {
     if constexpr (C + 1 < D) {
-      if (lo == C) wt_gather_pair<C, D, W, M>(T, part, lane);
-      else WtPair<C + 1, D, NW, NWT, W, M, WB>::gather(lo, T, part, lane);
+      if (lo == C) wt_gather_pair<C, D, M>(T, tot, lane);
+      else WtPair<C + 1, D, NW, NWT, W, M, WB>::gather(lo, T, tot, lane);
     }
   }
 };
@@ -211,10 +214,13 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
   constexpr int NWT = NW * W;
   constexpr int NPAT = LowCode<D, M>::NPAT;
   constexpr int PW = 2 * (1 << M) * 64;   // one wave's partial tables (words)
-  static_assert(W * PW * 4 <= 65536, "partial tables exceed 64 KiB");
-  __shared__ uint32_t part[W * PW];
+  static_assert(wt16_lds_words(W, M) * 4 <= 81920, "two workgroups per CU");
+  // dynamic LDS of wt16_lds_words(W, M) words: the W partial tables, and
+  // above 4 waves the totals
+  extern __shared__ uint32_t part[];
+  uint32_t *tot = part + (size_t)W * PW;
   // per-wave results, once the last partial tables are summed: every use of
-  // part ends with a barrier
+  // the partial tables ends with a barrier
   uint32_t *fin = part;
   const cuint32_t *Ap = as_const(Ap_);
   const int lane = wave_lane();
@@ -264,13 +270,15 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
     WtPair<0, D, NW, NWT, W, M, WB>::partial(lo, u, negk + wbase, mypart, lane);
     __syncthreads();
     if constexpr (W > 4) {   // wave-count constant: the barriers stay uniform
-      wt_scatter<W, M>(part, wv, lane);
+      // the totals are read before the next step's first barrier, and written
+      // after it
+      wt_scatter<W, M>(part, tot, wv, lane);
       __syncthreads();
-      WtPair<0, D, NW, NWT, W, M, WB>::gather(lo, T, part, lane);
+      WtPair<0, D, NW, NWT, W, M, WB>::gather(lo, T, tot, lane);
     } else {
       WtPair<0, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
+      __syncthreads();
     }
-    __syncthreads();
   }
   uint32_t totU = 0;
 #pragma unroll
@@ -302,13 +310,15 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
       WtPair<0, D, NW, NWT, W, M, WB>::partial(lo, u, negk + wbase, mypart, lane);
       __syncthreads();
       if constexpr (W > 4) {   // wave-count constant: the barriers stay uniform
-        wt_scatter<W, M>(part, wv, lane);
+        // the totals are read before the next step's first barrier, and written
+        // after it
+        wt_scatter<W, M>(part, tot, wv, lane);
         __syncthreads();
-        WtPair<0, D, NW, NWT, W, M, WB>::gather(lo, T, part, lane);
+        WtPair<0, D, NW, NWT, W, M, WB>::gather(lo, T, tot, lane);
       } else {
         WtPair<0, D, NW, NWT, W, M, WB>::sum(lo, T, part, lane);
+        __syncthreads();
       }
-      __syncthreads();
       totU = 0;
 #pragma unroll
       for (int c = 0; c < D; ++c) totU += T[c][0];
@@ -641,15 +651,21 @@ static hipError_t lwt(Mode mode, const PlainArgs &a, hipStream_t st) {
   constexpr int NW = wt16_nw(D), M = wt16_m(D, W);
   const uint64_t groups = 2 * ((a.nseg + 127) / 128);
   const dim3 grid((unsigned)groups), blk(64 * W);
+  const size_t lds = (size_t)wt16_lds_words(W, M) * 4;
   const uint32_t *Ap = (const uint32_t *)a.A;
   const uint32_t *neg = (const uint32_t *)a.D;
-  if (mode == MODE_MAX)
-    hipLaunchKernelGGL((k_wide_t16<D, NW, W, M, MODE_MAX, WB>), grid, blk, 0, st, Ap, neg, a.R, a.q, a.seg0, a.nseg,
-                       a.target, a.unit_best, a.gbest, a.gfirst);
-  else
-    hipLaunchKernelGGL((k_wide_t16<D, NW, W, M, MODE_FIRST, WB>), grid, blk, 0, st, Ap, neg, a.R, a.q, a.seg0,
-                       a.nseg, a.target, a.unit_best, a.gbest, a.gfirst);
-  return hipGetLastError();
+  auto launch = [&](auto kernel) {
+    if (lds > (64u << 10)) {   // beyond the default limit of dynamic LDS
+      const hipError_t e =
+          hipFuncSetAttribute((const void *)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+      if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(kernel, grid, blk, lds, st, Ap, neg, a.R, a.q, a.seg0, a.nseg, a.target, a.unit_best, a.gbest,
+                       a.gfirst);
+    return hipGetLastError();
+  };
+  if (mode == MODE_MAX) return launch(k_wide_t16<D, NW, W, M, MODE_MAX, WB>);
+  return launch(k_wide_t16<D, NW, W, M, MODE_FIRST, WB>);
 }
 
 template <int D, int WB>
