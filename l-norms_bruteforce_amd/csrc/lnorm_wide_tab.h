@@ -9,10 +9,11 @@
 // tables over its words (2 * 2^M * NW v_sad_u16, as walk_blk16) and stores
 // these partial tables in LDS; after a barrier every wave sums the W
 // partials, so each wave holds the full tables, and a second barrier frees
-// the memory for the next step.  Above 4 waves each wave instead totals its
-// share of the entries over the W partials into a table of totals, and after
-// the second barrier every wave reads all the totals (1.2-1.6x at 8 waves,
-// d = 2, 3); the next step writes the totals only after its first barrier.
+// the memory for the next step.  Above 4 waves and at d = 2 (wt16_two_step)
+// each wave instead totals its share of the entries over the W partials into
+// a table of totals, and after the second barrier every wave reads all the
+// totals (1.2-1.6x at 8 waves, d = 2, 3); the next step writes the totals
+// only after its first barrier.
 // The block's d^M low
 // labelings are split over the waves (wave wv takes l = wv, wv + W, ...), and
 // the per-wave maxima (MODE_MAX) or first matching ranks (MODE_FIRST) are
@@ -34,7 +35,7 @@ constexpr int wt16_nw(int d) { return d <= 7 ? 32 : (d == 8 || d == 9) ? 24 : d 
 constexpr int wt16_wmax(int d) { return d <= 6 ? 8 : 4; }
 // the table rows of the W-wave walk: the single wave's rows at its width,
 // capped by LDS and registers.  LDS: the W waves' partial tables of two
-// labels (W * 2 * 2^M * 256 B; above 4 waves one table more, the totals)
+// labels (W * 2 * 2^M * 256 B; with the two-step combine one table more)
 // fill up to 64 KiB for d = 3 and 32 KiB for
 // other d (at d = 2 both labels change every step, so a row more does not
 // cut the table work per labeling: it ran 1.7x slower).  Registers: above 4
@@ -55,9 +56,14 @@ constexpr int wt16_m(int d, int w) {
 // at d = 6 (its 2 table rows spilled there)
 constexpr int kWtWaves[] = {2, 3, 4, 5, 6, 8};
 constexpr bool wt16_has(int d, int w) { return w <= wt16_wmax(d) && !(d == 6 && w == 5); }
+// The W-wave tabulated walk's combine in two steps (a table of totals) above
+// 4 waves and at d = 2; elsewhere at 2-4 waves it measured 0.95-1.03x of the
+// single step (d = 2 over 4 waves: 1.21x).
+constexpr bool wt16_two_step(int d, int w) { return w > 4 || d == 2; }
 // LDS words of the W-wave tabulated walk: W partial tables of two labels
-// (2^(M+1) x 64 words each), and above 4 waves one more for the totals
-constexpr int wt16_lds_words(int w, int m) { return (w + (w > 4 ? 1 : 0)) * (2 << m) * 64; }
+// (2^(M+1) x 64 words each), and with the two-step combine one more for the
+// totals
+constexpr int wt16_lds_words(int d, int w, int m) { return (w + (wt16_two_step(d, w) ? 1 : 0)) * (2 << m) * 64; }
 
 // This wave's partial tables of labels C and C + 1 into its LDS slice
 // (p[e][S][lane], e = 0 for C, 1 for C + 1).
@@ -101,8 +107,8 @@ __device__ __forceinline__ void wt_sum_pair(uint32_t (&T)[D][1 << M], const uint
   }
 }
 
-// Above 4 waves, the W waves' partial tables in two steps, with a barrier
-// between them.
+// The two-step combine of the W waves' partial tables (wt16_two_step), with
+// a barrier between the steps.
 // wt_scatter: wave wv totals the entries i = wv, wv + W, ... (i = e * 2^M + S
 // of label C + e, 2^(M+1) entries) over the W partials into tot[i].  Each
 // lane reads 2^(M+1) / W entries of W partials, where every wave summing
@@ -214,9 +220,9 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
   constexpr int NWT = NW * W;
   constexpr int NPAT = LowCode<D, M>::NPAT;
   constexpr int PW = 2 * (1 << M) * 64;   // one wave's partial tables (words)
-  static_assert(wt16_lds_words(W, M) * 4 <= 81920, "two workgroups per CU");
-  // dynamic LDS of wt16_lds_words(W, M) words: the W partial tables, and
-  // above 4 waves the totals
+  static_assert(wt16_lds_words(D, W, M) * 4 <= 81920, "two workgroups per CU");
+  // dynamic LDS of wt16_lds_words(D, W, M) words: the W partial tables, and
+  // with the two-step combine the totals
   extern __shared__ uint32_t part[];
   uint32_t *tot = part + (size_t)W * PW;
   // per-wave results, once the last partial tables are summed: every use of
@@ -269,7 +275,7 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
     asm volatile("" : "+s"(negk));
     WtPair<0, D, NW, NWT, W, M, WB>::partial(lo, u, negk + wbase, mypart, lane);
     __syncthreads();
-    if constexpr (W > 4) {   // wave-count constant: the barriers stay uniform
+    if constexpr (wt16_two_step(D, W)) {   // constant: the barriers stay uniform
       // the totals are read before the next step's first barrier, and written
       // after it
       wt_scatter<W, M>(part, tot, wv, lane);
@@ -309,7 +315,7 @@ __global__ __launch_bounds__(64 * W) void k_wide_t16(
       asm volatile("" : "+s"(negk));
       WtPair<0, D, NW, NWT, W, M, WB>::partial(lo, u, negk + wbase, mypart, lane);
       __syncthreads();
-      if constexpr (W > 4) {   // wave-count constant: the barriers stay uniform
+      if constexpr (wt16_two_step(D, W)) {   // constant: the barriers stay uniform
         // the totals are read before the next step's first barrier, and written
         // after it
         wt_scatter<W, M>(part, tot, wv, lane);
@@ -651,7 +657,7 @@ static hipError_t lwt(Mode mode, const PlainArgs &a, hipStream_t st) {
   constexpr int NW = wt16_nw(D), M = wt16_m(D, W);
   const uint64_t groups = 2 * ((a.nseg + 127) / 128);
   const dim3 grid((unsigned)groups), blk(64 * W);
-  const size_t lds = (size_t)wt16_lds_words(W, M) * 4;
+  const size_t lds = (size_t)wt16_lds_words(D, W, M) * 4;
   const uint32_t *Ap = (const uint32_t *)a.A;
   const uint32_t *neg = (const uint32_t *)a.D;
   auto launch = [&](auto kernel) {
